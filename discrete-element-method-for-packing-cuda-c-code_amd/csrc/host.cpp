@@ -1,0 +1,346 @@
+// host.cpp -- host-side utilities of libdem.so: input deck, reduced units, restart file,
+// restitution <-> damping map and seeded workload generators.  Plain C++ (no device code).
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dem.h"
+
+namespace {
+
+// splitmix64: small, portable, seedable.
+struct Rng {
+    unsigned long long s;
+    explicit Rng(unsigned long long seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull) {}
+    unsigned long long next()
+    {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }   // [0,1)
+    double normal()
+    {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+bool parse_number(const std::string &tok, double &v)
+{
+    if (tok.empty()) return false;
+    std::string t = tok;
+    for (char &ch : t)
+        if (ch == 'D' || ch == 'd') ch = 'E';   // Fortran exponent
+    char *end = nullptr;
+    v = std::strtod(t.c_str(), &end);
+    return end && *end == '\0';
+}
+
+// Head-on Hertz + dashpot collision in normalised form (the law in law.hpp reduces to it for
+// any E*, R*, m* and impact speed): u'' = -u^1.5 - sqrt(6) g u^0.25 u', u(0)=0, u'(0)=1.
+// Returns the restitution |u'(t_end)| at the return to u = 0.
+double restitution_ode(double g)
+{
+    const double c = std::sqrt(6.0) * g;
+    double u = 0.0, v = 1.0, t = 0.0;
+    const double h = 2e-4;
+    auto acc = [&](double uu, double vv) {
+        double up = uu > 0.0 ? uu : 0.0;
+        return -up * std::sqrt(up) - c * std::sqrt(std::sqrt(up)) * vv;
+    };
+    for (int it = 0; it < 2000000; it++) {
+        double k1u = v, k1v = acc(u, v);
+        double k2u = v + 0.5 * h * k1v, k2v = acc(u + 0.5 * h * k1u, v + 0.5 * h * k1v);
+        double k3u = v + 0.5 * h * k2v, k3v = acc(u + 0.5 * h * k2u, v + 0.5 * h * k2v);
+        double k4u = v + h * k3v, k4v = acc(u + h * k3u, v + h * k3v);
+        double un = u + h / 6.0 * (k1u + 2 * k2u + 2 * k3u + k4u);
+        double vn = v + h / 6.0 * (k1v + 2 * k2v + 2 * k3v + k4v);
+        t += h;
+        if (un <= 0.0 && t > h) {
+            // linear interpolation of the crossing for the exit speed
+            double f = u / (u - un);
+            double ve = v + f * (vn - v);
+            return std::fabs(ve);
+        }
+        if (vn <= 0.0 && un <= 0.0) return 0.0;
+        u = un;
+        v = vn;
+        if (v < 0.0 && u < 1e-12) return std::fabs(v);
+    }
+    return 0.0;   // stuck (overdamped)
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_read_deck(const char *path, dem_deck *d)
+{
+    if (!path || !d) return DEM_ERR_INVALID;
+    std::FILE *fp = std::fopen(path, "rb");
+    if (!fp) return DEM_ERR_IO;
+    std::string txt;
+    char buf[4096];
+    size_t k;
+    while ((k = std::fread(buf, 1, sizeof(buf), fp)) > 0) txt.append(buf, k);
+    std::fclose(fp);
+    // leading numeric tokens, stopping at the first non-number (the deck's free-text notes)
+    std::vector<double> v;
+    size_t i = 0;
+    while (i < txt.size()) {
+        while (i < txt.size() && std::isspace((unsigned char)txt[i])) i++;
+        size_t j = i;
+        while (j < txt.size() && !std::isspace((unsigned char)txt[j])) j++;
+        if (j == i) break;
+        double x;
+        if (!parse_number(txt.substr(i, j - i), x)) break;
+        v.push_back(x);
+        i = j;
+    }
+    // dempacking.cu:112-125 reads 27 values, the first being iforcemode; a deck without
+    // that leading line (26 values) is accepted with iforcemode = 0.
+    size_t off;
+    std::memset(d, 0, sizeof(*d));
+    if (v.size() >= 27) { d->iforcemode = (int)v[0]; off = 1; }
+    else if (v.size() == 26) { d->iforcemode = 0; off = 0; }
+    else return DEM_ERR_IO;
+    const double *p = v.data() + off;
+    d->emod = p[0]; d->vpois = p[1]; d->denp = p[2];
+    d->diam = p[3];
+    d->ntot = (int)p[4];
+    d->dt = p[5]; d->tstop = (uint)p[6];
+    d->dmt1 = p[7]; d->dmt2 = p[8]; d->thick = p[9];
+    d->zht1 = p[10]; d->zht2 = p[11];
+    d->vel0 = p[12]; d->nrings = (int)p[13];
+    d->ufricp = p[14]; d->ufricw = p[15];
+    d->dampnp = p[16]; d->damptp = p[17]; d->dampnw = p[18]; d->damptw = p[19]; d->cdamp = p[20];
+    d->stiffness = p[21]; d->dampcoeffn = p[22]; d->dampcoefft = p[23];
+    d->upp = p[24]; d->upw = p[25];
+    if (!(d->diam > 0.0) || !(d->denp > 0.0) || d->ntot < 0) return DEM_ERR_IO;
+    return DEM_OK;
+}
+
+int dem_deck_material(const dem_deck *d, dem_material *m)
+{
+    if (!d || !m) return DEM_ERR_INVALID;
+    double emod = d->emod, emodw = d->emod;                                  // :141
+    double vw = d->vpois;                                                    // :142
+    emod = 6.0 * emod / (3.14159265 * d->denp * 9.81 * d->diam);             // :146
+    emodw = 6.0 * emodw / (3.14159265 * d->denp * 9.81 * d->diam);           // :147
+    m->p.estarp = 1.33333333 * emod / (2.0 * (1.0 - d->vpois * d->vpois));   // :148
+    m->w.estarw = 1.33333333 / ((1.0 - vw * vw) / emod + (1.0 - vw * vw) / emodw);   // :149
+    m->p.ufricp = d->ufricp;
+    m->p.vpois = d->vpois;
+    m->p.dampnp = d->dampnp;
+    m->p.damptp = d->damptp;
+    m->p.upp = d->upp / d->diam;                                             // :224
+    m->w.ufricw = d->ufricw;
+    m->w.vpoisw = vw;
+    m->w.dampnw = d->dampnw;
+    m->w.damptw = d->damptw;
+    m->w.upw = d->upw / d->diam;                                             // :225
+    return DEM_OK;
+}
+
+int dem_write_restart(const char *path)
+{
+    if (!path) return DEM_ERR_INVALID;
+    std::FILE *fp = std::fopen(path, "wb");   // dempacking.cu:908-913
+    if (!fp) return DEM_ERR_IO;
+    std::fprintf(fp, "program comes to end!\n");
+    std::fprintf(fp, "%8d %8d\n", 1, 1);
+    std::fclose(fp);
+    return DEM_OK;
+}
+
+double dem_restitution_for_damping(double g)
+{
+    if (!(g >= 0.0)) return -1.0;
+    return restitution_ode(g);
+}
+
+double dem_damping_for_restitution(double e)
+{
+    if (!(e > 0.0) || !(e <= 1.0)) return -1.0;
+    if (e >= 1.0) return 0.0;
+    double lo = 0.0, hi = 0.1;
+    while (restitution_ode(hi) > e && hi < 100.0) hi *= 2.0;
+    for (int it = 0; it < 60; it++) {
+        double mid = 0.5 * (lo + hi);
+        if (restitution_ode(mid) > e) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+int dem_gen_diameters(int kind, uint n, double p0, double p1, double p2, unsigned long long seed,
+                      double *diam)
+{
+    if (!diam) return DEM_ERR_INVALID;
+    Rng r(seed);
+    if (kind == 0) {
+        if (!(p0 > 0.0)) return DEM_ERR_INVALID;
+        for (uint i = 0; i < n; i++) diam[i] = p0;
+        return DEM_OK;
+    }
+    if (kind == 1) {
+        // ln d ~ N(-sigma^2/2, sigma): mean diameter 1, rejection-clipped to [p1, p2]
+        double sg = p0, lo = p1, hi = p2;
+        if (!(sg >= 0.0) || !(lo > 0.0) || !(hi >= lo)) return DEM_ERR_INVALID;
+        for (uint i = 0; i < n; i++) {
+            double d;
+            do { d = std::exp(-0.5 * sg * sg + sg * r.normal()); } while (d < lo || d > hi);
+            diam[i] = d;
+        }
+        return DEM_OK;
+    }
+    if (kind == 2) {
+        // bimodal: big p0, small p1, big volume fraction p2; exact counts, seeded shuffle
+        double db = p0, ds = p1, fv = p2;
+        if (!(db > 0.0) || !(ds > 0.0) || !(fv >= 0.0) || !(fv <= 1.0)) return DEM_ERR_INVALID;
+        double nb = fv / (db * db * db), ns = (1.0 - fv) / (ds * ds * ds);
+        uint nbig = (uint)std::llround((double)n * nb / (nb + ns));
+        for (uint i = 0; i < n; i++) diam[i] = i < nbig ? db : ds;
+        for (uint i = n; i > 1; i--) {
+            uint j = (uint)(r.next() % i);
+            std::swap(diam[i - 1], diam[j]);
+        }
+        return DEM_OK;
+    }
+    return DEM_ERR_INVALID;
+}
+
+int dem_gen_insert(const dem_walls *w, double z0, double z1, uint n, const double *diam,
+                   unsigned long long seed, double *pos)
+{
+    if (!w || !diam || !pos || !(z1 > z0)) return DEM_ERR_INVALID;
+    if (n == 0) return DEM_OK;
+    double dmax = 0.0;
+    for (uint i = 0; i < n; i++) dmax = std::max(dmax, diam[i]);
+    double hx = w->kind == DEM_WALLS_BOX ? 0.5 * w->size[0] : w->radius;
+    double hy = w->kind == DEM_WALLS_BOX ? 0.5 * w->size[1] : w->radius;
+    double c = dmax;
+    int gx = (int)(2.0 * hx / c) + 1, gy = (int)(2.0 * hy / c) + 1, gz = (int)((z1 - z0) / c) + 1;
+    size_t ncell = (size_t)gx * gy * gz;
+    if (ncell > (size_t)1 << 31) return DEM_ERR_INVALID;
+    std::vector<int> head(ncell, -1), next(n, -1);
+    auto cell_of = [&](double x, double y, double z, int &ix, int &iy, int &iz) {
+        ix = std::min(gx - 1, std::max(0, (int)std::floor((x + hx) / c)));
+        iy = std::min(gy - 1, std::max(0, (int)std::floor((y + hy) / c)));
+        iz = std::min(gz - 1, std::max(0, (int)std::floor((z - z0) / c)));
+    };
+    // insert larger spheres first (easier packing); positions keep the caller's indices
+    std::vector<uint> order(n);
+    for (uint i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint a, uint b) { return diam[a] > diam[b]; });
+    Rng r(seed);
+    const int kAttempts = 20000;
+    for (uint oi = 0; oi < n; oi++) {
+        uint i = order[oi];
+        double ri = 0.5 * diam[i];
+        bool placed = false;
+        for (int att = 0; att < kAttempts && !placed; att++) {
+            double x, y;
+            if (w->kind == DEM_WALLS_BOX) {
+                x = -hx + ri + (2.0 * (hx - ri)) * r.uniform();
+                y = -hy + ri + (2.0 * (hy - ri)) * r.uniform();
+            } else {
+                double rr = (w->radius - ri) * std::sqrt(r.uniform());
+                double th = 6.283185307179586 * r.uniform();
+                x = rr * std::cos(th);
+                y = rr * std::sin(th);
+            }
+            double z = z0 + ri + (z1 - z0 - 2.0 * ri) * r.uniform();
+            int ix, iy, iz;
+            cell_of(x, y, z, ix, iy, iz);
+            bool ok = true;
+            for (int dz = -1; dz <= 1 && ok; dz++)
+                for (int dy = -1; dy <= 1 && ok; dy++)
+                    for (int dx = -1; dx <= 1 && ok; dx++) {
+                        int X = ix + dx, Y = iy + dy, Z = iz + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= gx || Y >= gy || Z >= gz) continue;
+                        for (int j = head[((size_t)Z * gy + Y) * gx + X]; j >= 0; j = next[j]) {
+                            double ex = pos[3 * (size_t)j] - x, ey = pos[3 * (size_t)j + 1] - y,
+                                   ez = pos[3 * (size_t)j + 2] - z;
+                            double rs = ri + 0.5 * diam[j];
+                            if (ex * ex + ey * ey + ez * ez < rs * rs) { ok = false; break; }
+                        }
+                    }
+            if (!ok) continue;
+            pos[3 * (size_t)i] = x;
+            pos[3 * (size_t)i + 1] = y;
+            pos[3 * (size_t)i + 2] = z;
+            size_t cidx = ((size_t)iz * gy + iy) * gx + ix;
+            next[i] = head[cidx];
+            head[cidx] = (int)i;
+            placed = true;
+        }
+        if (!placed) return DEM_ERR_CAPACITY;
+    }
+    return DEM_OK;
+}
+
+int dem_gen_lattice(const dem_walls *w, double z0, double a, double jitter, uint n,
+                    unsigned long long seed, double *pos)
+{
+    if (!w || !pos || !(a > 0.0)) return DEM_ERR_INVALID;
+    double hx = w->kind == DEM_WALLS_BOX ? 0.5 * w->size[0] : w->radius;
+    double hy = w->kind == DEM_WALLS_BOX ? 0.5 * w->size[1] : w->radius;
+    double m = 0.5 + jitter;   // keep jittered unit spheres inside the walls
+    int nx = (int)std::floor((2.0 * (hx - m)) / a) + 1;
+    int ny = (int)std::floor((2.0 * (hy - m)) / a) + 1;
+    if (nx < 1 || ny < 1) return DEM_ERR_INVALID;
+    std::vector<double> layer;
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            double x = -hx + m + ix * a, y = -hy + m + iy * a;
+            if (w->kind == DEM_WALLS_CYLINDER && std::sqrt(x * x + y * y) > w->radius - m) continue;
+            layer.push_back(x);
+            layer.push_back(y);
+        }
+    size_t per = layer.size() / 2;
+    if (per == 0) return DEM_ERR_INVALID;
+    Rng r(seed);
+    for (uint i = 0; i < n; i++) {
+        size_t L = i / per, k = i % per;
+        pos[3 * (size_t)i] = layer[2 * k] + jitter * (2.0 * r.uniform() - 1.0);
+        pos[3 * (size_t)i + 1] = layer[2 * k + 1] + jitter * (2.0 * r.uniform() - 1.0);
+        pos[3 * (size_t)i + 2] = z0 + (double)L * a + jitter * (2.0 * r.uniform() - 1.0);
+    }
+    return DEM_OK;
+}
+
+int dem_particle_props(uint n, const double *diam, double *rad, double *rmass, double *inert)
+{
+    if (!diam || !rad || !rmass || !inert) return DEM_ERR_INVALID;
+    for (uint i = 0; i < n; i++) {
+        rad[i] = 0.5 * diam[i];
+        rmass[i] = 8 * rad[i] * rad[i] * rad[i];             // dempacking.cu:311
+        inert[i] = 0.4 * rmass[i] * (rad[i] * rad[i]);       // dempacking.cu:312
+    }
+    return DEM_OK;
+}
+
+int dem_make_grid(const dem_walls *w, double h, double c, uint gsize[3], double origin[3])
+{
+    if (!w || !gsize || !origin || !(c > 0.0) || !(h > 0.0)) return DEM_ERR_INVALID;
+    double wx = w->kind == DEM_WALLS_BOX ? w->size[0] : 2.0 * w->radius;
+    double wy = w->kind == DEM_WALLS_BOX ? w->size[1] : 2.0 * w->radius;
+    origin[0] = -0.5 * wx;                                   // dempacking.cu:333-335
+    origin[1] = -0.5 * wy;
+    origin[2] = 0.0;
+    gsize[0] = (uint)(int)(wx / c) + 1;                      // dempacking.cu:340-342
+    gsize[1] = (uint)(int)(wy / c) + 1;
+    gsize[2] = (uint)(int)(h / c) + 1;
+    return DEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,836 @@
+// kernels.hpp -- gfx950 kernels of the DEM step.
+//
+// Context path (cell-sorted slots, CSR pair lists):
+//   rebuild : k_hash -> radix sort -> k_fill / k_cell_range -> k_reorder -> k_nbr_count ->
+//             scans -> k_nbr_fill (pair list + history remap) -> k_transpose
+//   step    : k_pair_force (one lane per candidate pair) -> k_particle (per-particle
+//             gather of its pair forces in fixed order, walls, gravity, integration,
+//             rebuild trigger, per-instance reductions)
+// Reference-layout path (drop-in entry points, particle-index order, AoS double3):
+//   c_* kernels with the reference's buffers and semantics.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "law.hpp"
+
+namespace dem {
+
+constexpr unsigned kEmpty = 0xffffffffu;
+constexpr int kBlock = 256;
+
+// 128-byte particle record in slot order.  pos/rad, pdis/mass and angv/inert pack the data a
+// pair gathers into the first 96 bytes; q (displacement since the last rebuild), the
+// instance and the particle index fill the rest of the cache line.
+struct __attribute__((aligned(16))) PRec {
+    double px, py, pz, rad;
+    double vx, vy, vz, mass;
+    double wx, wy, wz, inert;
+    double qx, qy, qz;
+    unsigned inst, id;
+};
+static_assert(sizeof(PRec) == 128, "record must be one 128-byte line");
+
+// Per-pair tangential history: dispt (second relative to first) and fricp.
+struct __attribute__((aligned(16))) Hist {
+    double hx, hy, hz, fr;
+};
+
+// Per-pair force output of the last step (80 bytes): force on first, torque on first,
+// torque on second, contact flag (1.0 / 0.0).
+struct __attribute__((aligned(16))) POut {
+    double fx, fy, fz, flag;
+    double tax, tay, taz, tbx;
+    double tby, tbz;
+};
+static_assert(sizeof(POut) == 80, "pair output is five 16-byte chunks");
+
+// Per-instance parameters (device table for the batched form; passed by value otherwise).
+struct InstDev {
+    LawParams pp;          // particle-particle
+    LawParams pw;          // particle-wall
+    double dt, skin, ann2, pad0;
+    double origin[3], cell[3];
+    WallGeom walls;
+    unsigned gsize[3], cell_off;
+    unsigned p_off, n, pad1, pad2;
+};
+
+// Device control block.  Published to a host-mapped mirror by the last block of k_particle.
+struct DevCtl {
+    unsigned need_rebuild;   // trigger raised by the last step (atomicOr)
+    unsigned steps;          // steps completed (incremented by the last k_particle block)
+    unsigned npairs;         // written by the pair scan
+    unsigned overflow;       // pair capacity exceeded at the last rebuild
+    unsigned done_blocks;    // k_particle last-block ticket
+    unsigned pad[3];
+};
+// Per-instance accumulators follow DevCtl in the same allocation:
+//   unsigned long long maxz_bits[ninst]; unsigned pcntot[ninst];
+// The mirror has the same layout.
+
+__device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
+__device__ __forceinline__ void st2(double *p, double a, double b)
+{
+    *reinterpret_cast<double2 *>(p) = make_double2(a, b);
+}
+
+__device__ __forceinline__ bool is_active(const unsigned *active) { return !active || *active != 0u; }
+
+// ----------------------------------------------------------------------------------------
+// Rebuild
+// ----------------------------------------------------------------------------------------
+
+// calcHashD semantics (neiglist.cu:77-97) on slot-ordered records: the (cell, index) pair
+// of particle `id` lands at kv[id], i.e. in particle-index order, exactly the array the
+// reference sorts.  Cells are offset per instance so one sort serves every instance.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, const InstDev *inst,
+                                                 InstDev one, uint2 *kv, DevCtl *ctl,
+                                                 const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s == 0 && ctl) ctl->need_rebuild = 0u;
+    if (s >= n) return;
+    const PRec &r = rec[s];
+    double px = r.px, py = r.py, pz = r.pz;
+    unsigned id = r.id;
+    const InstDev &I = BATCHED ? inst[r.inst] : one;
+    unsigned x = cell_coord(px, I.origin[0], I.cell[0], I.gsize[0]);
+    unsigned y = cell_coord(py, I.origin[1], I.cell[1], I.gsize[1]);
+    unsigned z = cell_coord(pz, I.origin[2], I.cell[2], I.gsize[2]);
+    unsigned h = z * I.gsize[1] * I.gsize[0] + y * I.gsize[0] + x;
+    kv[id] = make_uint2(h + I.cell_off, id);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(unsigned *a, unsigned n, unsigned v,
+                                                 const unsigned *active)
+{
+    if (!is_active(active)) return;
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) a[i] = v;
+}
+
+// findCellStartD (neiglist.cu:102-120) plus the matching cell end.
+__global__ __launch_bounds__(kBlock) void k_cell_range(const uint2 *sorted, unsigned n,
+                                                       unsigned *cstart, unsigned *cend,
+                                                       const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    unsigned c = sorted[s].x;
+    if (s == 0 || sorted[s - 1].x != c) cstart[c] = s;
+    if (cend && (s == n - 1 || sorted[s + 1].x != c)) cend[c] = s + 1;
+}
+
+// Move every particle into its sorted slot; q restarts at zero (dempacking.cu:541).
+__global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigned n,
+                                                    const PRec *rec_old, PRec *rec_new,
+                                                    const Hist *wh_old, Hist *wh_new,
+                                                    const unsigned *inv_old, unsigned *inv_new,
+                                                    const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    unsigned id = sorted[s].y;
+    unsigned o = inv_old[id];
+    const double *src = reinterpret_cast<const double *>(rec_old + o);
+    double *dst = reinterpret_cast<double *>(rec_new + s);
+#pragma unroll
+    for (int k = 0; k < 12; k += 2) {
+        double2 v = ld2(src + k);
+        st2(dst + k, v.x, v.y);
+    }
+    st2(dst + 12, 0.0, 0.0);
+    uint2 tail = *reinterpret_cast<const uint2 *>(src + 15);
+    *reinterpret_cast<double *>(dst + 14) = 0.0;
+    *reinterpret_cast<uint2 *>(dst + 15) = tail;
+    wh_new[s] = wh_old[o];
+    inv_new[id] = s;
+}
+
+// Slot range [s0, s1) of the up-to-three consecutive cells of one stencil row.  Cells of a
+// row are adjacent in key order and slots are sorted by key, so their particles are
+// contiguous.  Returns false if the row is empty.
+__device__ __forceinline__ bool row_range(const unsigned *cstart, const unsigned *cend,
+                                          unsigned c0, unsigned c1, unsigned &s0, unsigned &s1)
+{
+    s0 = kEmpty;
+    s1 = 0;
+    for (unsigned c = c0; c <= c1; c++) {
+        unsigned a = cstart[c];
+        if (a == kEmpty) continue;
+        if (s0 == kEmpty) s0 = a;
+        s1 = cend[c];
+    }
+    return s0 != kEmpty;
+}
+
+// Visit every candidate partner b of slot a in ascending slot order (27-cell stencil,
+// z-, y-, x-offset ascending).  f(b) is called for b != a passing the cutoff test.
+template <class F>
+__device__ __forceinline__ void for_candidates(const PRec *rec, unsigned a, unsigned key,
+                                               const InstDev &I, const unsigned *cstart,
+                                               const unsigned *cend, F &&f)
+{
+    const PRec &ra = rec[a];
+    d3 xa = d3{ra.px, ra.py, ra.pz};
+    double rad_a = ra.rad;
+    unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2];
+    unsigned local = key - I.cell_off;
+    unsigned gxy = gx * gy;
+    int cz = (int)(local / gxy);
+    int cy = (int)((local - (unsigned)cz * gxy) / gx);
+    int cx = (int)(local - (unsigned)cz * gxy - (unsigned)cy * gx);
+    int x0 = cx > 0 ? cx - 1 : 0;
+    int x1 = cx + 1 < (int)gx ? cx + 1 : (int)gx - 1;
+    for (int dz = -1; dz <= 1; dz++) {
+        int z = cz + dz;
+        if (z < 0 || z >= (int)gz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int y = cy + dy;
+            if (y < 0 || y >= (int)gy) continue;
+            unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
+            unsigned s0, s1;
+            if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) continue;
+            for (unsigned b = s0; b < s1; b++) {
+                if (b == a) continue;
+                const PRec &rb = rec[b];
+                if (is_candidate(xa, d3{rb.px, rb.py, rb.pz}, rad_a, rb.rad, I.skin)) f(b);
+            }
+        }
+    }
+}
+
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const uint2 *sorted,
+                                                      unsigned n, const InstDev *inst,
+                                                      InstDev one, const unsigned *cstart,
+                                                      const unsigned *cend, unsigned *ng,
+                                                      unsigned *nl, const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
+    unsigned cg = 0, cl = 0;
+    for_candidates(rec, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
+        if (b > a) cg++;
+        else cl++;
+    });
+    ng[a] = cg;
+    nl[a] = cl;
+}
+
+// Binary search of `target` in the ascending second members of ab[lo, hi); kEmpty if absent.
+__device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, unsigned hi,
+                                                unsigned target)
+{
+    while (lo < hi) {
+        unsigned mid = (lo + hi) >> 1;
+        unsigned v = ab[mid].y;
+        if (v == target) return mid;
+        if (v < target) lo = mid + 1;
+        else hi = mid;
+    }
+    return kEmpty;
+}
+
+// Pair list of slot a ([off[a], off[a+1]), partners b > a ascending) with the tangential
+// history carried over by pair identity (the reference's pairlistD remap,
+// particleSystem.cu:143-191): the old list is searched under the particles' old slots, and
+// the stored displacement is negated when the pair's member order flipped.  Partners b < a
+// are written to the transposed list for k_transpose.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_nbr_fill(
+    const PRec *rec, const uint2 *sorted, unsigned n, const InstDev *inst, InstDev one,
+    const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
+    unsigned cap, uint2 *ab, Hist *hist, unsigned *jt, const unsigned *inv_old,
+    const uint2 *ab_old, const unsigned *off_old, const Hist *hist_old, unsigned n_old,
+    DevCtl *ctl, const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
+    unsigned pg = off[a];
+    unsigned pl = toff[a];
+    unsigned ida = rec[a].id;
+    unsigned oa = (inv_old && ida < n_old) ? inv_old[ida] : kEmpty;
+    bool over = false;
+    for_candidates(rec, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
+        if (b > a) {
+            if (pg >= cap) { over = true; return; }
+            ab[pg] = make_uint2(a, b);
+            Hist h = Hist{0.0, 0.0, 0.0, 0.0};
+            if (oa != kEmpty) {
+                unsigned idb = rec[b].id;
+                unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
+                if (ob != kEmpty) {
+                    if (oa < ob) {
+                        unsigned q = find_second(ab_old, off_old[oa], off_old[oa + 1], ob);
+                        if (q != kEmpty) h = hist_old[q];
+                    } else {
+                        unsigned q = find_second(ab_old, off_old[ob], off_old[ob + 1], oa);
+                        if (q != kEmpty) {
+                            Hist o = hist_old[q];
+                            h = Hist{-o.hx, -o.hy, -o.hz, o.fr};
+                        }
+                    }
+                }
+            }
+            hist[pg] = h;
+            pg++;
+        } else {
+            if (pl < cap) jt[pl] = b;
+            pl++;
+        }
+    });
+    if (over && ctl) atomicOr(&ctl->overflow, 1u);
+}
+
+// Replace each transposed entry (partner slot b < a) by the index of pair (b, a).
+__global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const unsigned *off,
+                                                      const unsigned *toff, unsigned n,
+                                                      unsigned cap, unsigned *jt,
+                                                      const unsigned *active)
+{
+    if (!is_active(active)) return;
+    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n) return;
+    unsigned e = toff[a + 1];
+    if (e > cap) e = cap;
+    for (unsigned k = toff[a]; k < e; k++) {
+        unsigned b = jt[k];
+        unsigned lo = off[b], hi = off[b + 1];
+        if (hi > cap) hi = cap;
+        jt[k] = lo < hi ? find_second(ab, lo, hi, a) : kEmpty;
+    }
+}
+
+// Pair-capacity check after the scan: npairs = min(total, cap); overflow when total > cap.
+// Every later kernel bounds its pair indices by npairs, so an overflowed list is truncated,
+// never read or written past its allocation.
+__global__ void k_check_cap(DevCtl *ctl, const unsigned *total_g, const unsigned *total_l,
+                            unsigned cap, const unsigned *active)
+{
+    if (!is_active(active)) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        unsigned p = *total_g;
+        bool over = p > cap || *total_l > cap;
+        ctl->npairs = over ? cap : p;
+        if (over) ctl->overflow = 1u;
+    }
+}
+
+__global__ void k_set_u32(unsigned *p, unsigned v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ----------------------------------------------------------------------------------------
+// Step
+// ----------------------------------------------------------------------------------------
+
+// One lane per candidate pair (the reference's calculateforcepairD, neiglist.cu:156-233).
+// Grid-stride over npairs read from the device, so a fixed grid serves any list size.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const unsigned *npairs,
+                                                       const PRec *rec, Hist *hist, POut *out,
+                                                       const InstDev *inst, LawParams pp,
+                                                       double dt)
+{
+    const unsigned np = *npairs;
+    for (unsigned p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+        uint2 e = ab[p];
+        const double *A = reinterpret_cast<const double *>(rec + e.x);
+        const double *B = reinterpret_cast<const double *>(rec + e.y);
+        double2 a0 = ld2(A), a1 = ld2(A + 2), a2 = ld2(A + 4), a3 = ld2(A + 6), a4 = ld2(A + 8),
+                a5 = ld2(A + 10);
+        double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6), b4 = ld2(B + 8),
+                b5 = ld2(B + 10);
+        const double *H = reinterpret_cast<const double *>(hist + p);
+        double2 h0 = ld2(H), h1 = ld2(H + 2);
+        LawParams P = pp;
+        double tdt = dt;
+        if (BATCHED) {
+            const InstDev &I = inst[rec[e.x].inst];
+            P = I.pp;
+            tdt = I.dt;
+        }
+        d3 fa, ta, tb, ho;
+        double fo;
+        bool c = law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
+                          d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x},
+                          a1.y, b1.y, a3.y, b3.y, d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho,
+                          fo);
+        double *O = reinterpret_cast<double *>(out + p);
+        st2(O, fa.x, fa.y);
+        st2(O + 2, fa.z, c ? 1.0 : 0.0);
+        st2(O + 4, ta.x, ta.y);
+        st2(O + 6, ta.z, tb.x);
+        st2(O + 8, tb.y, tb.z);
+        double *HW = reinterpret_cast<double *>(hist + p);
+        st2(HW, ho.x, ho.y);
+        st2(HW + 2, ho.z, fo);
+    }
+}
+
+// Per-particle kernel: gather of the particle's pair forces (own pairs in list order, then
+// pairs where it is second, via the transposed index), then gravity and walls
+// (calculateforceiwD, neiglist.cu:238-306), then the integrator (updatepositionD,
+// neiglist.cu:308-409) with the rebuild trigger |q|^2 > ann2 and per-instance max z and
+// contact totals.  The last block publishes the control block to the host mirror.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_particle(
+    PRec *rec, Hist *whist, const unsigned *off, const unsigned *toff, const unsigned *jt,
+    const POut *out, unsigned n, const InstDev *inst, InstDev one, unsigned *contacti,
+    double *fdbg, DevCtl *ctl, unsigned ninst, unsigned char *mirror)
+{
+    unsigned long long *maxz = reinterpret_cast<unsigned long long *>(ctl + 1);
+    unsigned *pcnt = reinterpret_cast<unsigned *>(maxz + ninst);
+    const unsigned np = ctl->npairs;
+    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    bool live = a < n;
+    unsigned my_inst = 0, flag = 0, cnt = 0;
+    unsigned long long zbits = 0ull;
+    if (live) {
+        double *R = reinterpret_cast<double *>(rec + a);
+        double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+        unsigned c = 0;
+        unsigned p0 = off[a], p1 = off[a + 1];
+        if (p1 > np) p1 = np;
+        for (unsigned p = p0; p < p1; p++) {
+            const double *O = reinterpret_cast<const double *>(out + p);
+            double2 o0 = ld2(O), o1 = ld2(O + 2), o2 = ld2(O + 4), o3 = ld2(O + 6);
+            fx += o0.x; fy += o0.y; fz += o1.x;
+            tx += o2.x; ty += o2.y; tz += o3.x;
+            c += o1.y != 0.0 ? 1u : 0u;
+        }
+        unsigned q0 = toff[a], q1 = toff[a + 1];
+        if (q1 > np) q1 = np;
+        for (unsigned k = q0; k < q1; k++) {
+            unsigned q = jt[k];
+            if (q >= np) continue;
+            const double *O = reinterpret_cast<const double *>(out + q);
+            double2 o0 = ld2(O), o1 = ld2(O + 2), o3 = ld2(O + 6), o4 = ld2(O + 8);
+            fx -= o0.x; fy -= o0.y; fz -= o1.x;
+            tx += o3.y; ty += o4.x; tz += o4.y;
+            c += o1.y != 0.0 ? 1u : 0u;
+        }
+        double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
+                r5 = ld2(R + 10), r6 = ld2(R + 12);
+        double qz = R[14];
+        uint2 tail = *reinterpret_cast<const uint2 *>(R + 15);
+        my_inst = tail.x;
+        const InstDev &I = BATCHED ? inst[my_inst] : one;
+        d3 x = d3{r0.x, r0.y, r1.x};
+        double rad = r1.y;
+        d3 v = d3{r2.x, r2.y, r3.x};
+        double m = r3.y;
+        d3 w = d3{r4.x, r4.y, r5.x};
+        double inert = r5.y;
+        // gravity (neiglist.cu:272, 284)
+        fx = fx + 0.0;
+        fy = fy + 0.0;
+        fz = fz + (-m);
+        // walls
+        const double *WH = reinterpret_cast<const double *>(whist + a);
+        double2 w0 = ld2(WH), w1 = ld2(WH + 2);
+        d3 fw, tw, hw;
+        double fow;
+        int hits = law_walls(I.walls, x, v, w, rad, m, d3{w0.x, w0.y, w1.x}, I.dt, I.pw, fw, tw,
+                             hw, fow);
+        double *WHW = reinterpret_cast<double *>(whist + a);
+        if (hits) {
+            fx += fw.x; fy += fw.y; fz += fw.z;
+            tx += tw.x; ty += tw.y; tz += tw.z;
+            st2(WHW, hw.x, hw.y);
+            st2(WHW + 2, hw.z, fow);
+        } else {
+            st2(WHW, 0.0, 0.0);
+            st2(WHW + 2, 0.0, 0.0);
+        }
+        if (fdbg) {
+            double *D = fdbg + 6 * (size_t)a;
+            D[0] = fx; D[1] = fy; D[2] = fz; D[3] = tx; D[4] = ty; D[5] = tz;
+        }
+        // integrator
+        double dt = I.dt;
+        v.x = v.x + fx * dt * dt / m;
+        v.y = v.y + fy * dt * dt / m;
+        v.z = v.z + fz * dt * dt / m;
+        x.x = x.x + v.x; x.y = x.y + v.y; x.z = x.z + v.z;
+        double qx = r6.x + v.x, qy = r6.y + v.y, qzz = qz + v.z;
+        w.x = w.x + tx * dt * dt / inert;
+        w.y = w.y + ty * dt * dt / inert;
+        w.z = w.z + tz * dt * dt / inert;
+        st2(R, x.x, x.y);
+        st2(R + 4, v.x, v.y);
+        R[6] = v.z;
+        R[2] = x.z;
+        st2(R + 8, w.x, w.y);
+        R[10] = w.z;
+        st2(R + 12, qx, qy);
+        R[14] = qzz;
+        contacti[a] = c;
+        double q2 = qx * qx + qy * qy + qzz * qzz;
+        flag = q2 > I.ann2 ? 1u : 0u;
+        cnt = c;
+        double zz = x.z > 0.0 ? x.z : 0.0;
+        zbits = (unsigned long long)__double_as_longlong(zz);
+    }
+    // per-instance reductions: wave-uniform fast path, per-lane atomics otherwise
+    unsigned inst0 = __builtin_amdgcn_readfirstlane(my_inst);
+    bool uniform = __all(!live || my_inst == inst0);
+    if (uniform) {
+        unsigned long long zb = zbits;
+        unsigned f = flag, cc = cnt;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            unsigned long long oz = __shfl_xor(zb, d, 64);
+            zb = oz > zb ? oz : zb;
+            f |= __shfl_xor(f, d, 64);
+            cc += __shfl_xor(cc, d, 64);
+        }
+        if (__lane_id() == 0 && __any(live)) {
+            if (zb) atomicMax(&maxz[inst0], zb);
+            if (cc) atomicAdd(&pcnt[inst0], cc);
+            if (f) atomicOr(&ctl->need_rebuild, 1u);
+        }
+    } else if (live) {
+        if (zbits) atomicMax(&maxz[my_inst], zbits);
+        if (cnt) atomicAdd(&pcnt[my_inst], cnt);
+        if (flag) atomicOr(&ctl->need_rebuild, 1u);
+    }
+    // last block publishes the control block and resets the per-step accumulators
+    __shared__ unsigned s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = atomicAdd(&ctl->done_blocks, 1u);
+        s_last = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        volatile unsigned *vc = reinterpret_cast<volatile unsigned *>(ctl);
+        if (mirror) {
+            unsigned *mc = reinterpret_cast<unsigned *>(mirror);
+            if (threadIdx.x < sizeof(DevCtl) / 4) mc[threadIdx.x] = vc[threadIdx.x];
+            unsigned long long *mz = reinterpret_cast<unsigned long long *>(mirror + sizeof(DevCtl));
+            unsigned *mp = reinterpret_cast<unsigned *>(mz + ninst);
+            volatile unsigned long long *vz = maxz;
+            volatile unsigned *vp = pcnt;
+            for (unsigned k = threadIdx.x; k < ninst; k += kBlock) { mz[k] = vz[k]; mp[k] = vp[k]; }
+        }
+        __syncthreads();
+        for (unsigned k = threadIdx.x; k < ninst; k += kBlock) { maxz[k] = 0ull; pcnt[k] = 0u; }
+        if (threadIdx.x == 0) {
+            ctl->done_blocks = 0u;
+            ctl->steps = ctl->steps + 1u;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Reference-layout kernels (drop-in entry points)
+// ----------------------------------------------------------------------------------------
+
+struct CompatGrid {
+    unsigned g[3];
+    double cell[3], origin[3];
+};
+
+__device__ __forceinline__ unsigned compat_hash(const double *p, const CompatGrid &G)
+{
+    unsigned x = cell_coord(p[0], G.origin[0], G.cell[0], G.g[0]);
+    unsigned y = cell_coord(p[1], G.origin[1], G.cell[1], G.g[1]);
+    unsigned z = cell_coord(p[2], G.origin[2], G.cell[2], G.g[2]);
+    return z * G.g[1] * G.g[0] + y * G.g[0] + x;
+}
+
+__global__ __launch_bounds__(kBlock) void c_calc_hash(const double *pos, unsigned n, CompatGrid G,
+                                                      uint2 *hash)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    hash[i] = make_uint2(compat_hash(pos + 3 * (size_t)i, G), i);
+}
+
+// neighborarrayD (absent; call particleSystem.cu:85-96): candidates |xj - xi| < cub with
+// unit-diameter spheres, written in stencil order into a row of `width` slots.
+__global__ __launch_bounds__(kBlock) void c_neighbor_array(const double *pos, const uint2 *sorted,
+                                                           const unsigned *cstart, unsigned n,
+                                                           CompatGrid G, int width, double skin,
+                                                           unsigned *anei, unsigned *njgi,
+                                                           unsigned *njli, unsigned *overflow)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double *pi = pos + 3 * (size_t)i;
+    d3 xi = d3{pi[0], pi[1], pi[2]};
+    unsigned h = compat_hash(pi, G);
+    unsigned gxy = G.g[0] * G.g[1];
+    int cz = (int)(h / gxy);
+    int cy = (int)((h - (unsigned)cz * gxy) / G.g[0]);
+    int cx = (int)(h - (unsigned)cz * gxy - (unsigned)cy * G.g[0]);
+    int cnt = 0;
+    unsigned ng = 0, nl = 0;
+    for (int dz = -1; dz <= 1; dz++) {
+        int z = cz + dz;
+        if (z < 0 || z >= (int)G.g[2]) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            int y = cy + dy;
+            if (y < 0 || y >= (int)G.g[1]) continue;
+            for (int dx = -1; dx <= 1; dx++) {
+                int x = cx + dx;
+                if (x < 0 || x >= (int)G.g[0]) continue;
+                unsigned c = (unsigned)z * gxy + (unsigned)y * G.g[0] + (unsigned)x;
+                unsigned s = cstart[c];
+                if (s == kEmpty) continue;
+                for (; s < n && sorted[s].x == c; s++) {
+                    unsigned j = sorted[s].y;
+                    if (j == i) continue;
+                    const double *pj = pos + 3 * (size_t)j;
+                    if (!is_candidate(xi, d3{pj[0], pj[1], pj[2]}, 0.5, 0.5, skin)) continue;
+                    if (cnt < width) anei[(size_t)i * width + cnt] = j;
+                    cnt++;
+                    if (j > i) ng++;
+                    else nl++;
+                }
+            }
+        }
+    }
+    for (int k = cnt; k < width; k++) anei[(size_t)i * width + k] = kEmpty;
+    njgi[i] = ng;
+    njli[i] = nl;
+    if (cnt > width) atomicOr(overflow, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void c_add_arrays(unsigned *b, const unsigned *a, unsigned n)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;   // addTwoArraysD, neiglist.cu:122-130
+    if (i < n) b[i] += a[i];
+}
+
+// pairlistD (absent; call particleSystem.cu:172-190).  Aref row i holds i's own pair slots
+// first (njgi entries, list order) and then the slots of pairs (j, i), j < i (njli entries,
+// Anei order), so sumallforcei can tell the side of each entry from its position.
+__global__ __launch_bounds__(kBlock) void c_pair_list(
+    const unsigned *sjgi, const unsigned *oldsjgi, const unsigned *oldln, const unsigned *anei,
+    const unsigned *njgi, const unsigned *njli, const double *olddispt, const double *oldfricp,
+    int width, unsigned n, unsigned *aref, unsigned *lp, unsigned *ln, double *dispt,
+    double *fricp)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    unsigned cnt = njgi[i] + njli[i];
+    unsigned start = sjgi[i] - njgi[i];
+    unsigned k = 0, r = 0;
+    unsigned os = i > 0 ? oldsjgi[i - 1] : 0u;
+    unsigned oe = oldsjgi[i];
+    for (unsigned m = 0; m < (unsigned)width; m++) aref[(size_t)i * width + m] = kEmpty;
+    for (unsigned m = 0; m < cnt && m < (unsigned)width; m++) {
+        unsigned j = anei[(size_t)i * width + m];
+        if (j > i) {
+            unsigned slot = start + k;
+            lp[slot] = i;
+            ln[slot] = j;
+            aref[(size_t)i * width + k] = slot;
+            k++;
+            double h0 = 0.0, h1 = 0.0, h2 = 0.0, f = 0.0;
+            for (unsigned q = os; q < oe; q++) {
+                if (oldln[q] == j) {
+                    h0 = olddispt[3 * (size_t)q];
+                    h1 = olddispt[3 * (size_t)q + 1];
+                    h2 = olddispt[3 * (size_t)q + 2];
+                    f = oldfricp[q];
+                    break;
+                }
+            }
+            dispt[3 * (size_t)slot] = h0;
+            dispt[3 * (size_t)slot + 1] = h1;
+            dispt[3 * (size_t)slot + 2] = h2;
+            fricp[slot] = f;
+        } else {
+            unsigned cj = njgi[j] + njli[j];
+            unsigned rr = 0, found = kEmpty;
+            for (unsigned mm = 0; mm < cj && mm < (unsigned)width; mm++) {
+                unsigned jj = anei[(size_t)j * width + mm];
+                if (jj == i) { found = rr; break; }
+                if (jj > j) rr++;
+            }
+            unsigned pos = njgi[i] + r;
+            if (pos < (unsigned)width)
+                aref[(size_t)i * width + pos] = found == kEmpty ? kEmpty : (sjgi[j] - njgi[j]) + found;
+            r++;
+        }
+    }
+}
+
+__device__ __forceinline__ d3 ld3(const double *p, size_t i)
+{
+    return d3{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+}
+__device__ __forceinline__ void st3(double *p, size_t i, d3 v)
+{
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// calculateforcepairD (neiglist.cu:156-233).
+__global__ __launch_bounds__(kBlock) void c_force_pair(
+    unsigned np, const double *pos, const double *pdis, const double *angv, const double *rad,
+    const double *rmass, const unsigned *lp, const unsigned *ln, unsigned *oldln, double *dispt,
+    double *fricp, double *olddispt, double *oldfricp, double dt, LawParams P, double *forcepair,
+    double *torqpairi, double *torqpairj, unsigned *contactpair)
+{
+    unsigned p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    unsigned i = lp[p], j = ln[p];
+    d3 fa, ta, tb, ho;
+    double fo;
+    bool c = law_pair(ld3(pos, i), ld3(pos, j), ld3(pdis, i), ld3(pdis, j), ld3(angv, i),
+                      ld3(angv, j), rad[i], rad[j], rmass[i], rmass[j], ld3(dispt, p), dt, P, fa,
+                      ta, tb, ho, fo);
+    st3(forcepair, p, fa);
+    st3(torqpairi, p, ta);
+    st3(torqpairj, p, tb);
+    st3(dispt, p, ho);
+    fricp[p] = fo;
+    contactpair[p] = c ? 1u : 0u;
+    st3(olddispt, p, ho);
+    oldfricp[p] = fo;
+    oldln[p] = j;
+}
+
+// sumallforceiD (absent; call particleSystem.cu:262-276): Aref row order, own pairs
+// (+forcepair, torqpairi) then pairs where the particle is second (-forcepair, torqpairj).
+__global__ __launch_bounds__(kBlock) void c_sum_forces(int n, int width, const unsigned *njgi,
+                                                       const unsigned *njli,
+                                                       const double *forcepair,
+                                                       const double *torqpairi,
+                                                       const double *torqpairj,
+                                                       const unsigned *contactpair,
+                                                       const unsigned *aref, double *forcei,
+                                                       double *torquei, unsigned *contacti)
+{
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    unsigned c = 0;
+    unsigned g = njgi[i];
+    unsigned cnt = g + njli[i];
+    for (unsigned m = 0; m < cnt && m < (unsigned)width; m++) {
+        unsigned p = aref[(size_t)i * width + m];
+        if (p == kEmpty) continue;
+        if (m < g) {
+            f0 += forcepair[3 * (size_t)p]; f1 += forcepair[3 * (size_t)p + 1];
+            f2 += forcepair[3 * (size_t)p + 2];
+            t0 += torqpairi[3 * (size_t)p]; t1 += torqpairi[3 * (size_t)p + 1];
+            t2 += torqpairi[3 * (size_t)p + 2];
+        } else {
+            f0 -= forcepair[3 * (size_t)p]; f1 -= forcepair[3 * (size_t)p + 1];
+            f2 -= forcepair[3 * (size_t)p + 2];
+            t0 += torqpairj[3 * (size_t)p]; t1 += torqpairj[3 * (size_t)p + 1];
+            t2 += torqpairj[3 * (size_t)p + 2];
+        }
+        c += contactpair[p];
+    }
+    st3(forcei, i, d3{f0, f1, f2});
+    st3(torquei, i, d3{t0, t1, t2});
+    contacti[i] = c;
+}
+
+// calculateforceiwD (neiglist.cu:238-306).
+__global__ __launch_bounds__(kBlock) void c_wall_gravity(int n, WallGeom G, const double *pos,
+                                                         const double *pdis, const double *angv,
+                                                         const double *rad, const double *rmass,
+                                                         double dt, unsigned np, LawParams P,
+                                                         double *disptw, double *fricpw,
+                                                         double *forcei, double *torquei)
+{
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    d3 f = d3{0.0, 0.0, 0.0}, t = d3{0.0, 0.0, 0.0};
+    if (np != 0u) { f = ld3(forcei, i); t = ld3(torquei, i); }
+    double m = rmass[i];
+    f.x = f.x + 0.0;
+    f.y = f.y + 0.0;
+    f.z = f.z + (-m);
+    d3 fw, tw, hw;
+    double fo;
+    int hit = law_walls(G, ld3(pos, i), ld3(pdis, i), ld3(angv, i), rad[i], m, ld3(disptw, i),
+                        dt, P, fw, tw, hw, fo);
+    if (hit) {
+        f = d3{f.x + fw.x, f.y + fw.y, f.z + fw.z};
+        t = d3{t.x + tw.x, t.y + tw.y, t.z + tw.z};
+        st3(disptw, i, hw);
+        fricpw[i] = fo;
+    } else {
+        st3(disptw, i, d3{0.0, 0.0, 0.0});
+        fricpw[i] = 0.0;
+    }
+    st3(forcei, i, f);
+    st3(torquei, i, t);
+}
+
+// updatepositionD (neiglist.cu:308-409); block maxima go to partials, reduced on the device
+// into out2 = (iall, maxhz bits) by the caller's second kernel.
+__global__ __launch_bounds__(kBlock) void c_update(int n, double *pos, double *pdis, double *angv,
+                                                   const double *rmass, const double *inert,
+                                                   const double *forcei, const double *torquei,
+                                                   double *qi, double dt, double ann2,
+                                                   unsigned *iall, unsigned long long *maxz)
+{
+    int i = blockIdx.x * kBlock + threadIdx.x;
+    unsigned flag = 0;
+    unsigned long long zb = 0ull;
+    if (i < n) {
+        double m = rmass[i], I = inert[i];
+        d3 F = ld3(forcei, i), T = ld3(torquei, i);
+        d3 v = ld3(pdis, i), x = ld3(pos, i), q = ld3(qi, i), w = ld3(angv, i);
+        v.x = v.x + F.x * dt * dt / m;
+        v.y = v.y + F.y * dt * dt / m;
+        v.z = v.z + F.z * dt * dt / m;
+        x = d3{x.x + v.x, x.y + v.y, x.z + v.z};
+        q = d3{q.x + v.x, q.y + v.y, q.z + v.z};
+        w.x = w.x + T.x * dt * dt / I;
+        w.y = w.y + T.y * dt * dt / I;
+        w.z = w.z + T.z * dt * dt / I;
+        st3(pos, i, x);
+        st3(pdis, i, v);
+        st3(qi, i, q);
+        st3(angv, i, w);
+        double q2 = q.x * q.x + q.y * q.y + q.z * q.z;
+        flag = q2 > ann2 ? 1u : 0u;
+        double zz = x.z > 0.0 ? x.z : 0.0;
+        zb = (unsigned long long)__double_as_longlong(zz);
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        unsigned long long oz = __shfl_xor(zb, d, 64);
+        zb = oz > zb ? oz : zb;
+        flag |= __shfl_xor(flag, d, 64);
+    }
+    if (__lane_id() == 0) {
+        if (zb) atomicMax(maxz, zb);
+        if (flag) atomicOr(iall, 1u);
+    }
+}
+
+// outputporositycnD (absent; call particleSystem.cu:405-410): total contact count.
+__global__ __launch_bounds__(kBlock) void c_contact_total(unsigned n, const unsigned *contacti,
+                                                          unsigned *total)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    unsigned c = i < n ? contacti[i] : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (__lane_id() == 0 && c) atomicAdd(total, c);
+}
+
+}  // namespace dem

@@ -1,0 +1,1167 @@
+// libdem.hip -- host side of libdem.so: context API and the reference-named drop-in entry
+// points.  Single translation unit: the kernels live in kernels.hpp / sort.hpp / law.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/dem.h"
+#include "kernels.hpp"
+#include "sort.hpp"
+
+using namespace dem;
+
+namespace {
+
+int g_last_error = 0;
+
+#define DEM_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            std::fprintf(stderr, "GPUassert: %s %s %d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                        \
+            g_last_error = DEM_ERR_HIP;                                                    \
+            return DEM_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+#define DEM_HIP_V(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            std::fprintf(stderr, "GPUassert: %s %s %d\n", hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                        \
+            g_last_error = DEM_ERR_HIP;                                                    \
+            return;                                                                        \
+        }                                                                                  \
+    } while (0)
+
+inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+int key_bits_for(unsigned ncells)
+{
+    int b = 1;
+    while (b < 32 && (1ull << b) < (unsigned long long)ncells) b++;
+    return b;
+}
+
+LawParams law_from(const Matproperties &m)
+{
+    return LawParams{m.estarp, m.ufricp, m.vpois, m.dampnp, m.damptp, m.upp};
+}
+LawParams law_from(const Wallproperties &w)
+{
+    return LawParams{w.estarw, w.ufricw, w.vpoisw, w.dampnw, w.damptw, w.upw};
+}
+
+// Scratch for scans and sorts, grown on demand (never per step once sized).
+struct Scratch {
+    unsigned *tiles = nullptr;
+    size_t tiles_cap = 0;
+    unsigned *counts = nullptr;   // radix digit counts [256 * ntiles]
+    unsigned *offs = nullptr;     // scanned counts (+1)
+    size_t counts_cap = 0;
+    unsigned *total = nullptr;    // 4 device words
+    int ensure(size_t ntiles_scan, size_t ncounts)
+    {
+        if (!total) DEM_HIP(hipMalloc(&total, 4 * sizeof(unsigned)));
+        if (ntiles_scan > tiles_cap) {
+            if (tiles) DEM_HIP(hipFree(tiles));
+            tiles_cap = std::max(ntiles_scan, (size_t)64);
+            DEM_HIP(hipMalloc(&tiles, tiles_cap * sizeof(unsigned)));
+        }
+        if (ncounts > counts_cap) {
+            if (counts) DEM_HIP(hipFree(counts));
+            if (offs) DEM_HIP(hipFree(offs));
+            counts_cap = ncounts;
+            DEM_HIP(hipMalloc(&counts, counts_cap * sizeof(unsigned)));
+            DEM_HIP(hipMalloc(&offs, (counts_cap + 1) * sizeof(unsigned)));
+        }
+        return DEM_OK;
+    }
+    void release()
+    {
+        if (tiles) (void)hipFree(tiles);
+        if (counts) (void)hipFree(counts);
+        if (offs) (void)hipFree(offs);
+        if (total) (void)hipFree(total);
+        tiles = counts = offs = total = nullptr;
+        tiles_cap = counts_cap = 0;
+    }
+};
+
+// Exclusive scan of n words: out[i] = sum(in[0..i)); if `total` is non-null it receives the
+// total and out[n] is written too.
+int scan_excl(Scratch &S, const unsigned *in, unsigned n, unsigned *out, unsigned *total,
+              hipStream_t st, const unsigned *active)
+{
+    unsigned nt = std::max(1u, cdiv(n, kScanTile));
+    if (S.ensure(nt, 0)) return DEM_ERR_HIP;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, in, n, S.tiles, active);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, S.tiles, nt, total, active);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kScanThreads), 0, st, in, n, S.tiles, out,
+                       (const unsigned *)total, active);
+    return hipGetLastError() == hipSuccess ? DEM_OK : DEM_ERR_HIP;
+}
+
+// Stable LSD radix sort of n (key, value) pairs over the low `bits` key bits.  Reads `in`
+// (left untouched unless it is one of the ping-pong buffers), ping-pongs a <-> b, returns
+// the buffer holding the result.
+uint2 *radix_sort(Scratch &S, const uint2 *in, uint2 *a, uint2 *b, unsigned n, int bits,
+                  hipStream_t st, const unsigned *active, int *rc)
+{
+    *rc = DEM_OK;
+    int passes = (bits + 7) / 8;
+    unsigned nt = std::max(1u, cdiv(n, kSortTile));
+    size_t ncounts = (size_t)256 * nt;
+    if (S.ensure(std::max(1u, cdiv(ncounts, kScanTile)), ncounts)) { *rc = DEM_ERR_HIP; return nullptr; }
+    const uint2 *src = in;
+    uint2 *dst = a;
+    if (passes == 0) return const_cast<uint2 *>(in);
+    for (int p = 0; p < passes; p++) {
+        if (src == dst) dst = (dst == a) ? b : a;
+        int sh = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kSortThreads), 0, st, src, n, sh, nt,
+                           S.counts, active);
+        if (scan_excl(S, S.counts, (unsigned)ncounts, S.offs, nullptr, st, active)) {
+            *rc = DEM_ERR_HIP;
+            return nullptr;
+        }
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(kSortThreads), 0, st, src, n, sh, nt,
+                           (const unsigned *)S.offs, dst, active);
+        src = dst;
+        dst = (dst == a) ? b : a;
+    }
+    if (hipGetLastError() != hipSuccess) *rc = DEM_ERR_HIP;
+    return const_cast<uint2 *>(src);
+}
+
+int current_device_cus()
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.multiProcessorCount > 0)
+            cus = p.multiProcessorCount;
+    }
+    return cus;
+}
+
+}  // namespace
+
+// =========================================================================================
+// Context
+// =========================================================================================
+
+struct dem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ninst = 0;
+    bool batched = false;
+    std::vector<InstDev> hinst;
+    InstDev *dinst = nullptr;
+    unsigned n = 0, ncells = 0;
+    int key_bits = 1;
+    int cus = 256;
+
+    PRec *rec[2] = {nullptr, nullptr};
+    Hist *whist[2] = {nullptr, nullptr};
+    unsigned *inv[2] = {nullptr, nullptr};
+    int rc = 0;   // index of the current particle buffers
+    unsigned *contacti = nullptr;
+    double *fdbg = nullptr;
+    bool record_forces = false;
+
+    uint2 *kv_hash = nullptr, *kv_a = nullptr, *kv_b = nullptr, *kv_sorted = nullptr;
+    unsigned *cstart = nullptr, *cend = nullptr;
+    unsigned *ng = nullptr, *nl = nullptr, *toff = nullptr;
+    unsigned *off[2] = {nullptr, nullptr};
+    uint2 *ab[2] = {nullptr, nullptr};
+    Hist *hist[2] = {nullptr, nullptr};
+    unsigned cap[2] = {0, 0};
+    int lc = 0;   // index of the current pair list
+    POut *out = nullptr;
+    unsigned *jt = nullptr;
+    unsigned out_cap = 0;
+    unsigned npairs = 0;
+    bool have_list = false;
+    unsigned n_list = 0;
+
+    Scratch S;
+    DevCtl *ctl = nullptr;
+    size_t ctl_bytes = 0;
+    unsigned char *mirror_h = nullptr, *mirror_d = nullptr;
+
+    int interval = 0;
+    unsigned long long steps = 0, rebuilds = 0;
+    unsigned since = 0;
+    bool need_rebuild = true;
+    unsigned last_flag = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evs = nullptr;
+    float last_ms = 0.f;
+
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    int graph_k = 0;
+    unsigned graph_np_grid = 0;
+    unsigned long long buf_gen = 0;     // bumped whenever a pair buffer is reallocated
+    unsigned long long graph_gen = ~0ull;
+};
+
+namespace {
+
+int alloc_pairs(dem_ctx *c, int side, unsigned cap)
+{
+    if (c->ab[side]) DEM_HIP(hipFree(c->ab[side]));
+    if (c->hist[side]) DEM_HIP(hipFree(c->hist[side]));
+    c->ab[side] = nullptr;
+    c->hist[side] = nullptr;
+    DEM_HIP(hipMalloc(&c->ab[side], (size_t)std::max(cap, 1u) * sizeof(uint2)));
+    DEM_HIP(hipMalloc(&c->hist[side], (size_t)std::max(cap, 1u) * sizeof(Hist)));
+    c->cap[side] = cap;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
+int alloc_out(dem_ctx *c, unsigned cap)
+{
+    if (c->out) DEM_HIP(hipFree(c->out));
+    if (c->jt) DEM_HIP(hipFree(c->jt));
+    c->out = nullptr;
+    c->jt = nullptr;
+    DEM_HIP(hipMalloc(&c->out, (size_t)std::max(cap, 1u) * sizeof(POut)));
+    DEM_HIP(hipMalloc(&c->jt, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
+    c->out_cap = cap;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
+unsigned pair_grid(const dem_ctx *c, unsigned np)
+{
+    unsigned g = cdiv(std::max(np, 1u), kBlock);
+    unsigned gmax = (unsigned)c->cus * 16u;
+    return std::max(1u, std::min(g, gmax));
+}
+
+// Phase A+B of a rebuild: hash, sort, cell ranges, reorder, neighbour counts and scans.
+int rebuild_count(dem_ctx *c, const unsigned *active)
+{
+    hipStream_t st = c->stream;
+    unsigned n = c->n;
+    unsigned gb = cdiv(n, kBlock);
+    InstDev one = c->hinst[0];
+    if (c->batched)
+        hipLaunchKernelGGL(k_hash<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc], n, c->dinst,
+                           one, c->kv_hash, c->ctl, active);
+    else
+        hipLaunchKernelGGL(k_hash<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc], n,
+                           c->dinst, one, c->kv_hash, c->ctl, active);
+    int rc;
+    c->kv_sorted = radix_sort(c->S, c->kv_hash, c->kv_a, c->kv_b, n, c->key_bits, st, active, &rc);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(c->ncells, kBlock), 4096u)), dim3(kBlock), 0, st,
+                       c->cstart, c->ncells, kEmpty, active);
+    hipLaunchKernelGGL(k_cell_range, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
+                       c->cstart, c->cend, active);
+    int nr = 1 - c->rc;
+    hipLaunchKernelGGL(k_reorder, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
+                       c->rec[c->rc], c->rec[nr], c->whist[c->rc], c->whist[nr], c->inv[c->rc],
+                       c->inv[nr], active);
+    c->rc = nr;
+    if (c->batched)
+        hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->ng, c->nl, active);
+    else
+        hipLaunchKernelGGL(k_nbr_count<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->ng, c->nl, active);
+    int nl = 1 - c->lc;
+    if (scan_excl(c->S, c->ng, n, c->off[nl], c->S.total, st, active)) return DEM_ERR_HIP;
+    if (scan_excl(c->S, c->nl, n, c->toff, c->S.total + 1, st, active)) return DEM_ERR_HIP;
+    hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, c->ctl, (const unsigned *)c->S.total,
+                       (const unsigned *)(c->S.total + 1), std::min(c->cap[nl], c->out_cap),
+                       active);
+    DEM_HIP(hipGetLastError());
+    return DEM_OK;
+}
+
+// Phase C: fill the pair list with the remapped history and build the transposed index.
+int rebuild_fill(dem_ctx *c, const unsigned *active)
+{
+    hipStream_t st = c->stream;
+    unsigned n = c->n;
+    unsigned gb = cdiv(n, kBlock);
+    InstDev one = c->hinst[0];
+    int nl = 1 - c->lc;
+    int ol = c->lc;
+    const unsigned *inv_old = c->have_list ? c->inv[1 - c->rc] : nullptr;
+    unsigned capn = std::min(c->cap[nl], c->out_cap);
+    if (c->batched)
+        hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
+                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list, c->ctl, active);
+    else
+        hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
+                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list, c->ctl, active);
+    hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
+                       (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
+                       active);
+    DEM_HIP(hipGetLastError());
+    c->lc = nl;
+    c->have_list = true;
+    c->n_list = n;
+    return DEM_OK;
+}
+
+// Host-driven rebuild: count, read the pair total, grow capacities if needed, fill.
+int rebuild_host(dem_ctx *c)
+{
+    int rc = rebuild_count(c, nullptr);
+    if (rc) return rc;
+    unsigned tot[2];
+    DEM_HIP(hipMemcpyAsync(tot, c->S.total, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    unsigned np = tot[0];
+    int nl = 1 - c->lc;
+    if (np > c->cap[nl]) {
+        unsigned nc = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)np * 3 / 2 + 1024);
+        if ((rc = alloc_pairs(c, nl, nc))) return rc;
+    }
+    if (np > c->out_cap) {
+        unsigned nc = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)np * 3 / 2 + 1024);
+        if ((rc = alloc_out(c, nc))) return rc;
+    }
+    // k_check_cap clamped npairs to the capacity seen before growing; store the real total
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->npairs, np);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->overflow, 0u);
+    if ((rc = rebuild_fill(c, nullptr))) return rc;
+    c->npairs = np;
+    c->rebuilds++;
+    c->since = 0;
+    c->need_rebuild = false;
+    return DEM_OK;
+}
+
+void launch_force_step(dem_ctx *c, unsigned np_grid)
+{
+    hipStream_t st = c->stream;
+    unsigned gb = cdiv(c->n, kBlock);
+    InstDev one = c->hinst[0];
+    const unsigned *np = &c->ctl->npairs;
+    if (c->batched) {
+        hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(kBlock), 0, st,
+                           (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
+                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           c->whist[c->rc], (const unsigned *)c->off[c->lc],
+                           (const unsigned *)c->toff, (const unsigned *)c->jt,
+                           (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
+                           (unsigned)c->ninst, c->mirror_d);
+    } else {
+        hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(kBlock), 0, st,
+                           (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
+                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           c->whist[c->rc], (const unsigned *)c->off[c->lc],
+                           (const unsigned *)c->toff, (const unsigned *)c->jt,
+                           (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
+                           (unsigned)c->ninst, c->mirror_d);
+    }
+}
+
+InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
+{
+    InstDev I;
+    std::memset(&I, 0, sizeof(I));
+    I.pp = law_from(d.mat.p);
+    I.pw = law_from(d.mat.w);
+    I.dt = d.dt;
+    I.skin = d.skin;
+    double ann = 0.5 * d.skin;
+    I.ann2 = ann * ann;
+    for (int k = 0; k < 3; k++) {
+        I.origin[k] = d.origin[k];
+        I.cell[k] = d.cell[k];
+        I.gsize[k] = d.gsize[k];
+    }
+    I.walls.kind = d.walls.kind;
+    I.walls.hx = 0.5 * d.walls.size[0];
+    I.walls.hy = 0.5 * d.walls.size[1];
+    I.walls.radius = d.walls.radius;
+    I.cell_off = cell_off;
+    I.p_off = p_off;
+    I.n = d.n;
+    return I;
+}
+
+void free_ctx(dem_ctx *c)
+{
+    if (!c) return;
+    void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
+                    c->contacti, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
+                    c->hist[1], c->out, c->jt, c->ctl};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    c->S.release();
+    if (c->mirror_h) (void)hipHostFree(c->mirror_h);
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    }
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evs) (void)hipEventDestroy(c->evs);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sync_mirror(dem_ctx *c)
+{
+    DEM_HIP(hipEventRecord(c->evs, c->stream));
+    DEM_HIP(hipEventSynchronize(c->evs));
+    const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
+    c->last_flag = m->need_rebuild;
+    return DEM_OK;
+}
+
+void drop_graph(dem_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+        c->gexec[k] = nullptr;
+        c->graph[k] = nullptr;
+    }
+    c->graph_k = 0;
+}
+
+// Grow one side of the pair buffers keeping its contents (the live list and history).
+int grow_pairs_keep(dem_ctx *c, int side, unsigned cap)
+{
+    if (c->cap[side] >= cap) return DEM_OK;
+    uint2 *nab = nullptr;
+    Hist *nh = nullptr;
+    DEM_HIP(hipMalloc(&nab, (size_t)cap * sizeof(uint2)));
+    DEM_HIP(hipMalloc(&nh, (size_t)cap * sizeof(Hist)));
+    DEM_HIP(hipMemcpyAsync(nab, c->ab[side], (size_t)c->cap[side] * sizeof(uint2), hipMemcpyDeviceToDevice, c->stream));
+    DEM_HIP(hipMemcpyAsync(nh, c->hist[side], (size_t)c->cap[side] * sizeof(Hist), hipMemcpyDeviceToDevice, c->stream));
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipFree(c->ab[side]));
+    DEM_HIP(hipFree(c->hist[side]));
+    c->ab[side] = nab;
+    c->hist[side] = nh;
+    c->cap[side] = cap;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_last_error(void)
+{
+    int e = g_last_error;
+    g_last_error = 0;
+    return e;
+}
+
+const char *dem_status_string(int s)
+{
+    switch (s) {
+    case DEM_OK: return "ok";
+    case DEM_ERR_INVALID: return "invalid argument";
+    case DEM_ERR_HIP: return "HIP runtime error";
+    case DEM_ERR_CAPACITY: return "capacity exceeded";
+    case DEM_ERR_NO_DEVICE: return "no usable device";
+    case DEM_ERR_IO: return "file error";
+    default: return "unknown status";
+    }
+}
+
+int dem_device_info(int *cu_count, char *name, int name_len)
+{
+    int dev = 0;
+    DEM_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    DEM_HIP(hipGetDeviceProperties(&p, dev));
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s", p.gcnArchName);
+    }
+    return DEM_OK;
+}
+
+int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_capacity)
+{
+    if (!out || ninst < 1 || !inst) return DEM_ERR_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return DEM_ERR_NO_DEVICE;
+    dem_ctx *c = new dem_ctx();
+    DEM_HIP(hipGetDevice(&c->device));
+    c->cus = current_device_cus();
+    c->ninst = ninst;
+    c->batched = ninst > 1;
+    unsigned cell_off = 0, p_off = 0;
+    for (int i = 0; i < ninst; i++) {
+        const dem_instance &d = inst[i];
+        if (d.n == 0 || d.gsize[0] == 0 || d.gsize[1] == 0 || d.gsize[2] == 0 || !(d.dt > 0.0) ||
+            !(d.cell[0] > 0.0) || !(d.cell[1] > 0.0) || !(d.cell[2] > 0.0) || d.skin < 0.0) {
+            free_ctx(c);
+            return DEM_ERR_INVALID;
+        }
+        unsigned long long cells = (unsigned long long)d.gsize[0] * d.gsize[1] * d.gsize[2];
+        if (cells + cell_off >= 0xffffffffull || (unsigned long long)p_off + d.n >= 0xffffffffull) {
+            free_ctx(c);
+            return DEM_ERR_INVALID;
+        }
+        c->hinst.push_back(make_inst(d, cell_off, p_off));
+        cell_off += (unsigned)cells;
+        p_off += d.n;
+    }
+    c->n = p_off;
+    c->ncells = cell_off;
+    c->key_bits = key_bits_for(c->ncells);
+    unsigned n = c->n;
+    unsigned pc = pair_capacity ? pair_capacity : (unsigned)std::min<unsigned long long>(0xfffffff0ull, 12ull * n + 1024);
+
+    DEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    DEM_HIP(hipEventCreate(&c->ev0));
+    DEM_HIP(hipEventCreate(&c->ev1));
+    DEM_HIP(hipEventCreateWithFlags(&c->evs, hipEventDisableTiming));
+    DEM_HIP(hipMalloc(&c->dinst, sizeof(InstDev) * ninst));
+    DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice));
+    for (int k = 0; k < 2; k++) {
+        DEM_HIP(hipMalloc(&c->rec[k], sizeof(PRec) * (size_t)n));
+        DEM_HIP(hipMalloc(&c->whist[k], sizeof(Hist) * (size_t)n));
+        DEM_HIP(hipMalloc(&c->inv[k], sizeof(unsigned) * (size_t)n));
+        DEM_HIP(hipMalloc(&c->off[k], sizeof(unsigned) * ((size_t)n + 1)));
+        if (alloc_pairs(c, k, pc)) { free_ctx(c); return DEM_ERR_HIP; }
+    }
+    if (alloc_out(c, pc)) { free_ctx(c); return DEM_ERR_HIP; }
+    DEM_HIP(hipMalloc(&c->contacti, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->kv_b, sizeof(uint2) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * (size_t)c->ncells));
+    DEM_HIP(hipMalloc(&c->cend, sizeof(unsigned) * (size_t)c->ncells));
+    DEM_HIP(hipMalloc(&c->ng, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->toff, sizeof(unsigned) * ((size_t)n + 1)));
+    c->ctl_bytes = sizeof(DevCtl) + (size_t)ninst * (sizeof(unsigned long long) + sizeof(unsigned));
+    DEM_HIP(hipMalloc(&c->ctl, c->ctl_bytes));
+    DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
+    DEM_HIP(hipHostMalloc((void **)&c->mirror_h, c->ctl_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(c->mirror_h, 0, c->ctl_bytes);
+    DEM_HIP(hipHostGetDevicePointer((void **)&c->mirror_d, c->mirror_h, 0));
+    if (c->S.ensure(64, 256 * 64)) { free_ctx(c); return DEM_ERR_HIP; }
+    *out = c;
+    return DEM_OK;
+}
+
+int dem_destroy(dem_ctx *c)
+{
+    if (!c) return DEM_ERR_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return DEM_OK;
+}
+
+int dem_set_option(dem_ctx *c, int option, double value)
+{
+    if (!c) return DEM_ERR_INVALID;
+    switch (option) {
+    case DEM_OPT_RECORD_FORCES:
+        c->record_forces = value != 0.0;
+        if (c->record_forces && !c->fdbg) DEM_HIP(hipMalloc(&c->fdbg, sizeof(double) * 6 * (size_t)c->n));
+        drop_graph(c);
+        return DEM_OK;
+    case DEM_OPT_REBUILD_INTERVAL:
+        if (value < 0.0) return DEM_ERR_INVALID;
+        c->interval = (int)value;
+        drop_graph(c);
+        return DEM_OK;
+    default: return DEM_ERR_INVALID;
+    }
+}
+
+int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const double *angv,
+                  const double *rad, const double *rmass, const double *inert, int on_device)
+{
+    if (!c || !pos || !pdis || !angv || !rad || !rmass || !inert) return DEM_ERR_INVALID;
+    size_t n = c->n;
+    std::vector<double> hp, hv, hw, hr, hm, hi;
+    if (on_device) {
+        hp.resize(3 * n); hv.resize(3 * n); hw.resize(3 * n); hr.resize(n); hm.resize(n); hi.resize(n);
+        DEM_HIP(hipMemcpy(hp.data(), pos, 24 * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(hv.data(), pdis, 24 * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(hw.data(), angv, 24 * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(hr.data(), rad, 8 * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(hm.data(), rmass, 8 * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(hi.data(), inert, 8 * n, hipMemcpyDeviceToHost));
+        pos = hp.data(); pdis = hv.data(); angv = hw.data(); rad = hr.data(); rmass = hm.data(); inert = hi.data();
+    }
+    std::vector<PRec> recs(n);
+    std::vector<unsigned> inv(n);
+    for (int k = 0; k < c->ninst; k++) {
+        const InstDev &I = c->hinst[k];
+        for (unsigned j = 0; j < I.n; j++) {
+            size_t i = (size_t)I.p_off + j;
+            PRec &r = recs[i];
+            r.px = pos[3 * i]; r.py = pos[3 * i + 1]; r.pz = pos[3 * i + 2]; r.rad = rad[i];
+            r.vx = pdis[3 * i]; r.vy = pdis[3 * i + 1]; r.vz = pdis[3 * i + 2]; r.mass = rmass[i];
+            r.wx = angv[3 * i]; r.wy = angv[3 * i + 1]; r.wz = angv[3 * i + 2]; r.inert = inert[i];
+            r.qx = r.qy = r.qz = 0.0;
+            r.inst = (unsigned)k;
+            r.id = (unsigned)i;
+            inv[i] = (unsigned)i;
+        }
+    }
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(c->rec[c->rc], recs.data(), sizeof(PRec) * n, hipMemcpyHostToDevice));
+    DEM_HIP(hipMemset(c->whist[c->rc], 0, sizeof(Hist) * n));
+    DEM_HIP(hipMemcpy(c->inv[c->rc], inv.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    c->have_list = false;
+    c->need_rebuild = true;
+    c->npairs = 0;
+    DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
+    return DEM_OK;
+}
+
+static int fetch_records(dem_ctx *c, std::vector<PRec> &recs)
+{
+    recs.resize(c->n);
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(recs.data(), c->rec[c->rc], sizeof(PRec) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return DEM_OK;
+}
+
+int dem_get_state(dem_ctx *c, double *pos, double *pdis, double *angv, double *qi, int on_device)
+{
+    if (!c) return DEM_ERR_INVALID;
+    std::vector<PRec> recs;
+    int rc = fetch_records(c, recs);
+    if (rc) return rc;
+    size_t n = c->n;
+    std::vector<double> p(3 * n), v(3 * n), w(3 * n), q(3 * n);
+    for (size_t s = 0; s < n; s++) {
+        const PRec &r = recs[s];
+        size_t i = r.id;
+        p[3 * i] = r.px; p[3 * i + 1] = r.py; p[3 * i + 2] = r.pz;
+        v[3 * i] = r.vx; v[3 * i + 1] = r.vy; v[3 * i + 2] = r.vz;
+        w[3 * i] = r.wx; w[3 * i + 1] = r.wy; w[3 * i + 2] = r.wz;
+        q[3 * i] = r.qx; q[3 * i + 1] = r.qy; q[3 * i + 2] = r.qz;
+    }
+    hipMemcpyKind k = on_device ? hipMemcpyHostToDevice : hipMemcpyHostToHost;
+    if (pos) DEM_HIP(hipMemcpy(pos, p.data(), 24 * n, k));
+    if (pdis) DEM_HIP(hipMemcpy(pdis, v.data(), 24 * n, k));
+    if (angv) DEM_HIP(hipMemcpy(angv, w.data(), 24 * n, k));
+    if (qi) DEM_HIP(hipMemcpy(qi, q.data(), 24 * n, k));
+    return DEM_OK;
+}
+
+int dem_get_wall_history(dem_ctx *c, double *disptw, double *fricpw)
+{
+    if (!c) return DEM_ERR_INVALID;
+    std::vector<PRec> recs;
+    int rc = fetch_records(c, recs);
+    if (rc) return rc;
+    size_t n = c->n;
+    std::vector<Hist> wh(n);
+    DEM_HIP(hipMemcpy(wh.data(), c->whist[c->rc], sizeof(Hist) * n, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; s++) {
+        size_t i = recs[s].id;
+        if (disptw) { disptw[3 * i] = wh[s].hx; disptw[3 * i + 1] = wh[s].hy; disptw[3 * i + 2] = wh[s].hz; }
+        if (fricpw) fricpw[i] = wh[s].fr;
+    }
+    return DEM_OK;
+}
+
+int dem_request_rebuild(dem_ctx *c)
+{
+    if (!c) return DEM_ERR_INVALID;
+    c->need_rebuild = true;
+    return DEM_OK;
+}
+
+int dem_rebuild(dem_ctx *c)
+{
+    if (!c) return DEM_ERR_INVALID;
+    int rc = rebuild_host(c);
+    if (rc) return rc;
+    return DEM_OK;
+}
+
+int dem_step(dem_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return DEM_ERR_INVALID;
+    DEM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; s++) {
+        if (c->need_rebuild || (c->interval > 0 && c->since >= (unsigned)c->interval)) {
+            int rc = rebuild_host(c);
+            if (rc) return rc;
+        }
+        launch_force_step(c, pair_grid(c, c->npairs));
+        DEM_HIP(hipGetLastError());
+        c->steps++;
+        c->since++;
+        if (c->interval == 0) {
+            int rc = sync_mirror(c);
+            if (rc) return rc;
+            c->need_rebuild = c->last_flag != 0u;
+        }
+    }
+    DEM_HIP(hipEventRecord(c->ev1, c->stream));
+    DEM_HIP(hipEventSynchronize(c->ev1));
+    DEM_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (c->interval > 0) {
+        const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
+        c->last_flag = m->need_rebuild;
+    }
+    return DEM_OK;
+}
+
+// Graph form: groups of K force steps followed by a list rebuild, captured once per buffer
+// parity (each rebuild swaps the particle and pair buffers, so two graphs alternate) and
+// replayed without host synchronisation.  The schedule equals dem_step with
+// DEM_OPT_REBUILD_INTERVAL = K.  A rebuild inside a graph cannot grow buffers: capacities
+// get 50% headroom over the current list, and an overflow is reported as DEM_ERR_CAPACITY
+// after the replay (the state is then invalid).
+int dem_step_graph(dem_ctx *c, int nsteps)
+{
+    if (!c || nsteps < 0) return DEM_ERR_INVALID;
+    int K = c->interval;
+    if (K <= 0 || nsteps % K != 0) return DEM_ERR_INVALID;
+    int rc;
+    if (c->need_rebuild || !c->have_list || c->since != 0) {
+        if ((rc = rebuild_host(c))) return rc;
+    }
+    unsigned want = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)c->npairs * 3 / 2 + 4096);
+    if (c->cap[1 - c->lc] < want && (rc = alloc_pairs(c, 1 - c->lc, want))) return rc;
+    if ((rc = grow_pairs_keep(c, c->lc, want))) return rc;
+    if (c->out_cap < want) {
+        // out / jt hold the live transposed index: keep jt's contents
+        POut *no = nullptr;
+        unsigned *nj = nullptr;
+        DEM_HIP(hipMalloc(&no, (size_t)want * sizeof(POut)));
+        DEM_HIP(hipMalloc(&nj, (size_t)want * sizeof(unsigned)));
+        DEM_HIP(hipMemcpyAsync(nj, c->jt, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
+        DEM_HIP(hipStreamSynchronize(c->stream));
+        DEM_HIP(hipFree(c->out));
+        DEM_HIP(hipFree(c->jt));
+        c->out = no;
+        c->jt = nj;
+        c->out_cap = want;
+        c->buf_gen++;
+    }
+    unsigned capmin = std::min(std::min(c->cap[0], c->cap[1]), c->out_cap);
+    unsigned np_grid = pair_grid(c, capmin);
+    if (c->cap[0] != c->cap[1]) {
+        // both sides share one capacity so the two graphs agree on it
+        unsigned cm = std::max(c->cap[0], c->cap[1]);
+        if ((rc = alloc_pairs(c, 1 - c->lc, cm))) return rc;
+        if ((rc = grow_pairs_keep(c, c->lc, cm))) return rc;
+        capmin = std::min(cm, c->out_cap);
+        np_grid = pair_grid(c, capmin);
+    }
+    if (!c->gexec[0] || c->graph_k != K || c->graph_np_grid != np_grid || c->graph_gen != c->buf_gen) {
+        drop_graph(c);
+        hipStream_t st = c->stream;
+        for (int g = 0; g < 2; g++) {
+            DEM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; k++) launch_force_step(c, np_grid);
+            rc = rebuild_count(c, nullptr);
+            if (!rc) rc = rebuild_fill(c, nullptr);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &gr);
+            if (rc || e != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                drop_graph(c);
+                return rc ? rc : DEM_ERR_HIP;
+            }
+            c->graph[g] = gr;
+            DEM_HIP(hipGraphInstantiate(&c->gexec[g], gr, nullptr, nullptr, 0));
+        }
+        // two captures flipped the buffer indices twice: back to the live parity
+        c->graph_k = K;
+        c->graph_np_grid = np_grid;
+        c->graph_gen = c->buf_gen;
+    }
+    int groups = nsteps / K;
+    DEM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int g = 0; g < groups; g++) DEM_HIP(hipGraphLaunch(c->gexec[g & 1], c->stream));
+    DEM_HIP(hipEventRecord(c->ev1, c->stream));
+    DEM_HIP(hipEventSynchronize(c->ev1));
+    DEM_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+    if (groups & 1) {
+        // odd number of groups: the live buffers are the other parity, and the graphs
+        // must start from the other one next time
+        c->rc = 1 - c->rc;
+        c->lc = 1 - c->lc;
+        std::swap(c->gexec[0], c->gexec[1]);
+        std::swap(c->graph[0], c->graph[1]);
+    }
+    unsigned w[2];
+    DEM_HIP(hipMemcpy(w, &c->ctl->npairs, 2 * sizeof(unsigned), hipMemcpyDeviceToHost));
+    c->npairs = w[0];
+    c->steps += (unsigned long long)nsteps;
+    c->rebuilds += (unsigned long long)groups;
+    c->since = 0;
+    c->need_rebuild = false;
+    const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
+    c->last_flag = m->need_rebuild;
+    if (w[1]) return DEM_ERR_CAPACITY;
+    return DEM_OK;
+}
+
+int dem_get_stats(dem_ctx *c, dem_stats *o)
+{
+    if (!c || !o) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
+    std::memset(o, 0, sizeof(*o));
+    o->steps = c->steps;
+    o->rebuilds = c->rebuilds;
+    o->n = c->n;
+    o->npairs = c->npairs;
+    o->pair_capacity = std::min(c->cap[c->lc], c->out_cap);
+    o->ninstances = (unsigned)c->ninst;
+    o->last_rebuild_flag = m->need_rebuild;
+    o->overflow = m->overflow;
+    const volatile unsigned long long *mz = reinterpret_cast<const volatile unsigned long long *>(c->mirror_h + sizeof(DevCtl));
+    const volatile unsigned *mp = reinterpret_cast<const volatile unsigned *>(mz + c->ninst);
+    unsigned long long zb = mz[0];
+    double z;
+    std::memcpy(&z, &zb, 8);
+    o->maxhz = z;
+    o->pcntot = mp[0];
+    return DEM_OK;
+}
+
+int dem_get_instance_stats(dem_ctx *c, double *maxhz, uint *pcntot)
+{
+    if (!c) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    const volatile unsigned long long *mz = reinterpret_cast<const volatile unsigned long long *>(c->mirror_h + sizeof(DevCtl));
+    const volatile unsigned *mp = reinterpret_cast<const volatile unsigned *>(mz + c->ninst);
+    for (int k = 0; k < c->ninst; k++) {
+        if (maxhz) {
+            unsigned long long zb = mz[k];
+            std::memcpy(&maxhz[k], &zb, 8);
+        }
+        if (pcntot) pcntot[k] = mp[k];
+    }
+    return DEM_OK;
+}
+
+int dem_sync(dem_ctx *c)
+{
+    if (!c) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    return DEM_OK;
+}
+
+double dem_last_ms(dem_ctx *c) { return c ? (double)c->last_ms : -1.0; }
+
+int dem_get_hash(dem_ctx *c, uint *kv)
+{
+    if (!c || !kv) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(kv, c->kv_hash, sizeof(uint2) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return DEM_OK;
+}
+
+int dem_get_sorted(dem_ctx *c, uint *kv)
+{
+    if (!c || !kv || !c->kv_sorted) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(kv, c->kv_sorted, sizeof(uint2) * (size_t)c->n, hipMemcpyDeviceToHost));
+    return DEM_OK;
+}
+
+int dem_get_cell_start(dem_ctx *c, uint *cs, uint ncells)
+{
+    if (!c || !cs || ncells < c->ncells) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(cs, c->cstart, sizeof(unsigned) * (size_t)c->ncells, hipMemcpyDeviceToHost));
+    return DEM_OK;
+}
+
+int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
+{
+    if (!c) return DEM_ERR_INVALID;
+    if (cap < c->npairs) return DEM_ERR_CAPACITY;
+    size_t np = c->npairs;
+    if (!np) return DEM_OK;
+    std::vector<PRec> recs;
+    int rc = fetch_records(c, recs);
+    if (rc) return rc;
+    if (ids) {
+        std::vector<uint2> ab(np);
+        DEM_HIP(hipMemcpy(ab.data(), c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < np; p++) {
+            ids[2 * p] = recs[ab[p].x].id;
+            ids[2 * p + 1] = recs[ab[p].y].id;
+        }
+    }
+    if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
+    if (out) DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
+    return DEM_OK;
+}
+
+int dem_get_forces(dem_ctx *c, double *force, double *torque, uint *contacti)
+{
+    if (!c || !c->record_forces || !c->fdbg) return DEM_ERR_INVALID;
+    std::vector<PRec> recs;
+    int rc = fetch_records(c, recs);
+    if (rc) return rc;
+    size_t n = c->n;
+    std::vector<double> f(6 * n);
+    std::vector<unsigned> ct(n);
+    DEM_HIP(hipMemcpy(f.data(), c->fdbg, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(ct.data(), c->contacti, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < n; s++) {
+        size_t i = recs[s].id;
+        for (int k = 0; k < 3; k++) {
+            if (force) force[3 * i + k] = f[6 * s + k];
+            if (torque) torque[3 * i + k] = f[6 * s + 3 + k];
+        }
+        if (contacti) contacti[i] = ct[s];
+    }
+    return DEM_OK;
+}
+
+// =========================================================================================
+// Drop-in: reference-named entry points (particleSystem.cu:13-424)
+// =========================================================================================
+
+namespace {
+Matproperties g_par = {0, 0, 0, 0, 0, 0};
+Wallproperties g_parw = {0, 0, 0, 0, 0, 0};
+bool g_walls_set = false;
+dem_walls g_walls = {DEM_WALLS_BOX, 0, {0, 0, 0}, 0};
+Scratch g_S;
+unsigned *g_words = nullptr;                 // [0] iall, [1] overflow, [2] pcntot, [4..5] maxz
+
+constexpr double kCua = 1.0;                 // neiglist.cu:27
+constexpr double kCub = 1.55;                // neiglist.cu:28
+
+int compat_words()
+{
+    if (!g_words) DEM_HIP(hipMalloc(&g_words, 8 * sizeof(unsigned)));
+    return DEM_OK;
+}
+
+CompatGrid make_grid(const uint g[3], const double cell[3], const double origin[3])
+{
+    CompatGrid G;
+    for (int k = 0; k < 3; k++) { G.g[k] = g[k]; G.cell[k] = cell[k]; G.origin[k] = origin[k]; }
+    return G;
+}
+
+void check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "GPUassert: %s (%s)\n", hipGetErrorString(e), what);
+        g_last_error = DEM_ERR_HIP;
+    }
+}
+}  // namespace
+
+void setMaterialProperties(const Matproperties *par, const Wallproperties *parw)
+{
+    if (par) g_par = *par;
+    if (parw) g_parw = *parw;
+}
+
+void setWallGeometry(const dem_walls *walls)
+{
+    if (walls) { g_walls = *walls; g_walls_set = true; }
+    else g_walls_set = false;
+}
+
+void calcHash(double *Pos, uint *particleHash, uint gridSize[3], double cellSize[3],
+              double worldOrigin[3], int numBodies)
+{
+    if (numBodies <= 0) return;
+    hipLaunchKernelGGL(c_calc_hash, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
+                       (const double *)Pos, (unsigned)numBodies,
+                       make_grid(gridSize, cellSize, worldOrigin), (uint2 *)particleHash);
+    check_launch("calcHash");
+}
+
+void RadixSort(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint bits)
+{
+    if (elements == 0) return;
+    int rc;
+    uint2 *a = reinterpret_cast<uint2 *>(pData0), *b = reinterpret_cast<uint2 *>(pData1);
+    int nb = (int)std::min(bits, 32u);
+    // ping-pong starting from pData0: pass k reads one buffer and writes the other
+    uint2 *res = radix_sort(g_S, a, b, a, elements, nb, 0, nullptr, &rc);
+    if (rc) { g_last_error = rc; return; }
+    if (res != a) DEM_HIP_V(hipMemcpyAsync(a, res, sizeof(uint2) * elements, hipMemcpyDeviceToDevice, 0));
+    check_launch("RadixSort");
+}
+
+void findCellStart(uint *particleHash, uint *cellStart, int numBodies, uint numGridCells)
+{
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(numGridCells, kBlock), 4096u)), dim3(kBlock), 0, 0,
+                       cellStart, numGridCells, kEmpty, (const unsigned *)nullptr);
+    if (numBodies > 0)
+        hipLaunchKernelGGL(k_cell_range, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
+                           (const uint2 *)particleHash, (unsigned)numBodies, cellStart,
+                           (unsigned *)nullptr, (const unsigned *)nullptr);
+    check_launch("findCellStart");
+}
+
+void neighborarray(double *pos, uint *particleHash, uint *cellStart, uint gridSize[3],
+                   double cellSize[3], double worldOrigin[3], int maxParticlesPerCell,
+                   int numBodies, uint *Anei, uint *njgi, uint *njli)
+{
+    if (numBodies <= 0 || maxParticlesPerCell <= 0) return;
+    if (compat_words()) return;
+    DEM_HIP_V(hipMemsetAsync(g_words + 1, 0, sizeof(unsigned), 0));
+    hipLaunchKernelGGL(c_neighbor_array, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
+                       (const double *)pos, (const uint2 *)particleHash, (const unsigned *)cellStart,
+                       (unsigned)numBodies, make_grid(gridSize, cellSize, worldOrigin),
+                       maxParticlesPerCell, kCub - kCua, Anei, njgi, njli, g_words + 1);
+    check_launch("neighborarray");
+    unsigned ov = 0;
+    DEM_HIP_V(hipMemcpy(&ov, g_words + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (ov) {
+        std::fprintf(stderr, "neighborarray: a particle has more than %d candidates\n", maxParticlesPerCell);
+        g_last_error = DEM_ERR_CAPACITY;
+    }
+}
+
+void prefixsum(uint numBodies, uint *njgi, uint *sjgi, uint *oldsjgi, uint *totalcontacts)
+{
+    if (numBodies == 0) { if (totalcontacts) *totalcontacts = 0; return; }
+    DEM_HIP_V(hipMemcpyAsync(oldsjgi, sjgi, numBodies * sizeof(uint), hipMemcpyDeviceToDevice, 0));
+    if (scan_excl(g_S, njgi, numBodies, sjgi, nullptr, 0, nullptr)) { g_last_error = DEM_ERR_HIP; return; }
+    hipLaunchKernelGGL(c_add_arrays, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0, sjgi,
+                       (const unsigned *)njgi, numBodies);
+    check_launch("prefixsum");
+    DEM_HIP_V(hipMemcpy(totalcontacts, sjgi + numBodies - 1, sizeof(uint), hipMemcpyDeviceToHost));
+}
+
+void pairlist(uint *sjgi, uint *oldsjgi, uint *oldLn, uint *Anei, uint *njgi, uint *njli,
+              double *olddispt, double *oldfricp, int maxParticlesPerCell, int maxCnPerParticle,
+              uint numBodies, uint *Aref, uint *Lp, uint *Ln, double *dispt, double *fricp)
+{
+    if (numBodies == 0) return;
+    size_t slots = (size_t)numBodies * (size_t)maxCnPerParticle;
+    // sentinel fills of particleSystem.cu:166-170 (Lp/Ln 0xff; dispt/fricp zero)
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(slots, kBlock), 4096u)), dim3(kBlock), 0, 0, Lp,
+                       (unsigned)slots, kEmpty, (const unsigned *)nullptr);
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(slots, kBlock), 4096u)), dim3(kBlock), 0, 0, Ln,
+                       (unsigned)slots, kEmpty, (const unsigned *)nullptr);
+    DEM_HIP_V(hipMemsetAsync(dispt, 0, slots * 3 * sizeof(double), 0));
+    DEM_HIP_V(hipMemsetAsync(fricp, 0, slots * sizeof(double), 0));
+    hipLaunchKernelGGL(c_pair_list, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
+                       (const unsigned *)sjgi, (const unsigned *)oldsjgi, (const unsigned *)oldLn,
+                       (const unsigned *)Anei, (const unsigned *)njgi, (const unsigned *)njli,
+                       (const double *)olddispt, (const double *)oldfricp, maxParticlesPerCell,
+                       numBodies, Aref, Lp, Ln, dispt, fricp);
+    check_launch("pairlist");
+}
+
+void calculateforcepair(double *pos, double *pdis, double *angv, double *rad, double *rmass,
+                        uint *Lp, uint *Ln, uint *oldLn, double *dispt, double *fricp,
+                        double *olddispt, double *oldfricp, double dt, double *forcepair,
+                        double *torqpairi, double *torqpairj, uint *contactpair, uint totalcontacts)
+{
+    if (totalcontacts == 0) return;   // the reference launches zero threads (no-op)
+    hipLaunchKernelGGL(c_force_pair, dim3(cdiv(totalcontacts, kBlock)), dim3(kBlock), 0, 0,
+                       totalcontacts, (const double *)pos, (const double *)pdis,
+                       (const double *)angv, (const double *)rad, (const double *)rmass,
+                       (const unsigned *)Lp, (const unsigned *)Ln, oldLn, dispt, fricp, olddispt,
+                       oldfricp, dt, law_from(g_par), forcepair, torqpairi, torqpairj, contactpair);
+    check_launch("calculateforcepair");
+}
+
+void sumallforcei(int maxParticlesPerCell, int numBodies, uint *njgi, uint *njli, double *forcepair,
+                  double *torqpairi, double *torqpairj, uint *contactpair, uint *Aref,
+                  double *forcei, double *torquei, uint *contacti)
+{
+    if (numBodies <= 0) return;
+    hipLaunchKernelGGL(c_sum_forces, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0, numBodies,
+                       maxParticlesPerCell, (const unsigned *)njgi, (const unsigned *)njli,
+                       (const double *)forcepair, (const double *)torqpairi,
+                       (const double *)torqpairj, (const unsigned *)contactpair,
+                       (const unsigned *)Aref, forcei, torquei, contacti);
+    check_launch("sumallforcei");
+}
+
+void calculateforceiw(double worldSize[3], double *pos, double *pdis, double *angv, double *rad,
+                      double *rmass, double dt, uint totalcontacts, int numBodies, double *disptw,
+                      double *fricpw, double *forcei, double *torquei)
+{
+    if (numBodies <= 0) return;
+    WallGeom G;
+    if (g_walls_set) {
+        G.kind = g_walls.kind;
+        G.hx = 0.5 * g_walls.size[0];
+        G.hy = 0.5 * g_walls.size[1];
+        G.radius = g_walls.radius;
+    } else {
+        G.kind = DEM_WALLS_BOX;
+        G.hx = 0.5 * worldSize[0];
+        G.hy = 0.5 * worldSize[1];
+        G.radius = 0.0;
+    }
+    hipLaunchKernelGGL(c_wall_gravity, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0, numBodies,
+                       G, (const double *)pos, (const double *)pdis, (const double *)angv,
+                       (const double *)rad, (const double *)rmass, dt, totalcontacts,
+                       law_from(g_parw), disptw, fricpw, forcei, torquei);
+    check_launch("calculateforceiw");
+}
+
+void updateposition(double worldSize[3], int numBodies, double *pos, double *pdis, double *angv,
+                    double *rmass, double *inert, double *forcei, double *torquei, double *qi,
+                    double dt, double cdamp, uint *iall, double *maxhz)
+{
+    (void)worldSize;
+    (void)cdamp;   // the reference reads cdamp but integrates with adamp = bdamp = 1 (neiglist.cu:354-355)
+    if (numBodies <= 0) { if (iall) *iall = 0; if (maxhz) *maxhz = 0.0; return; }
+    if (compat_words()) return;
+    DEM_HIP_V(hipMemsetAsync(g_words, 0, 8 * sizeof(unsigned), 0));
+    double ann = 0.5 * (kCub - kCua);
+    hipLaunchKernelGGL(c_update, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0, numBodies, pos,
+                       pdis, angv, (const double *)rmass, (const double *)inert,
+                       (const double *)forcei, (const double *)torquei, qi, dt, ann * ann,
+                       g_words, reinterpret_cast<unsigned long long *>(g_words + 4));
+    check_launch("updateposition");
+    unsigned w[8];
+    DEM_HIP_V(hipMemcpy(w, g_words, sizeof(w), hipMemcpyDeviceToHost));
+    if (iall) *iall = w[0];
+    if (maxhz) std::memcpy(maxhz, &w[4], sizeof(double));
+}
+
+void outputporositycn(uint numBodies, uint *contacti, double maxhz, double worldSize[3], uint *pcntot)
+{
+    (void)maxhz;
+    (void)worldSize;
+    if (numBodies == 0) { if (pcntot) *pcntot = 0; return; }
+    if (compat_words()) return;
+    DEM_HIP_V(hipMemsetAsync(g_words + 2, 0, sizeof(unsigned), 0));
+    hipLaunchKernelGGL(c_contact_total, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0, numBodies,
+                       (const unsigned *)contacti, g_words + 2);
+    check_launch("outputporositycn");
+    DEM_HIP_V(hipMemcpy(pcntot, g_words + 2, sizeof(uint), hipMemcpyDeviceToHost));
+}
+
+}  // extern "C"

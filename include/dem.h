@@ -1,0 +1,301 @@
+/*
+ * dem.h -- C ABI of the MI355X (gfx950) DEM packing step, libdem.so.
+ *
+ * Two layers, one set of HIP kernels:
+ *
+ *  1. Reference-named entry points (section "drop-in").  Same names, argument order and
+ *     meaning as the reference's extern "C" operator API in particleSystem.cu, plus the
+ *     SDK RadixSort it links, so a dempacking.cu-shaped driver links against this library
+ *     unchanged.  Buffers are caller-owned device pointers in particle-index order, AoS
+ *     double3 as in the reference.  Launches go to the null stream; host scalars are
+ *     returned after a synchronising copy exactly where the reference copies them.
+ *     Errors: the reference's gpuErrchk prints and exit()s (particleSystem.cu:15-23); a
+ *     library must not end its host process, so these functions print the same message,
+ *     record the error and return; dem_last_error() reads (and clears) it.
+ *
+ *  2. Context API (dem_*).  Owns every buffer (no per-step allocation), keeps particles
+ *     in cell-sorted slots, builds CSR pair lists with a transposed index, fuses the
+ *     per-particle gather, wall force, gravity and integration into one kernel, and runs
+ *     one or many independent instances (batched form) per launch.  Functions return a
+ *     dem_status.
+ *
+ * All arithmetic is IEEE FP64.  Units are the reference's reduced units: length = particle
+ * diameter d, mass = rho*pi*d^3/6, force = m*g, time = sqrt(d/g) (dempacking.cu:146-153,
+ * neiglist.cu:272).
+ */
+#ifndef DEM_H
+#define DEM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef unsigned int uint;
+
+/* ----------------------------------------------------------------------------------- */
+/* Shared types                                                                         */
+/* ----------------------------------------------------------------------------------- */
+
+/* Reference Matproperties / Wallproperties (neiglist.cu:32-47), same member order. */
+typedef struct Matproperties {
+    double estarp, ufricp, vpois, dampnp, damptp, upp;
+} Matproperties;
+
+typedef struct Wallproperties {
+    double estarw, ufricw, vpoisw, dampnw, damptw, upw;
+} Wallproperties;
+
+/* SDK KeyValuePair used by RadixSort (dempacking.cu:553-555): (cell hash, index). */
+typedef struct KeyValuePair {
+    uint key;
+    uint value;
+} KeyValuePair;
+
+typedef struct dem_material {
+    Matproperties p;     /* particle-particle */
+    Wallproperties w;    /* particle-wall */
+} dem_material;
+
+enum { DEM_WALLS_BOX = 0, DEM_WALLS_CYLINDER = 1 };
+
+/* Container walls.  Box: side walls x = +-size[0]/2, y = +-size[1]/2 and floor z = 0 (the
+ * reference's worldSize box, dempacking.cu:333-339).  Cylinder: vertical wall of radius
+ * `radius` about the z axis, floor z = 0.  No lid in either case. */
+typedef struct dem_walls {
+    int kind;
+    int pad;
+    double size[3];
+    double radius;
+} dem_walls;
+
+/* One independent packing.  Grid as in dempacking.cu:329-344 (cell, origin, gridSize). */
+typedef struct dem_instance {
+    uint n;              /* particles */
+    uint gsize[3];       /* cells per axis */
+    double cell[3];      /* cell edge per axis; must be >= 2*max radius + skin */
+    double origin[3];    /* grid origin */
+    dem_walls walls;
+    dem_material mat;
+    double dt;           /* reduced time step */
+    double skin;         /* list skin = cub - cua (neiglist.cu:27-28): 0.55 */
+} dem_instance;
+
+typedef enum dem_status {
+    DEM_OK = 0,
+    DEM_ERR_INVALID = 1,      /* bad argument or size */
+    DEM_ERR_HIP = 2,          /* a HIP runtime call failed */
+    DEM_ERR_CAPACITY = 3,     /* a fixed capacity (pairs, neighbour row, no-alloc cap) exceeded */
+    DEM_ERR_NO_DEVICE = 4,    /* no usable gfx950 device */
+    DEM_ERR_IO = 5            /* deck / restart file error */
+} dem_status;
+
+/* Last recorded error of the drop-in layer (0 if none); reading clears it. */
+int dem_last_error(void);
+/* Human-readable text for a dem_status. */
+const char *dem_status_string(int status);
+
+/* ----------------------------------------------------------------------------------- */
+/* 1. Drop-in: reference-named operator API (particleSystem.cu:13-424).                 */
+/* ----------------------------------------------------------------------------------- */
+
+/* Replaces cudaMemcpyToSymbol(Par) / (ParW) (dempacking.cu:355-356). */
+void setMaterialProperties(const Matproperties *par, const Wallproperties *parw);
+/* Selects the wall set used by calculateforceiw (default: the worldSize box). */
+void setWallGeometry(const dem_walls *walls);
+
+/* particleSystem.cu:27 -- particleHash[i] = (cell of Pos[i], i). */
+void calcHash(double *Pos, uint *particleHash, uint gridSize[3], double cellSize[3],
+              double worldOrigin[3], int numBodies);
+
+/* SDK radix sort (call dempacking.cu:553-555): stable sort of pData0 by key over the low
+ * `bits` key bits, pData1 is scratch of the same size, result in pData0. */
+void RadixSort(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint bits);
+
+/* particleSystem.cu:49 -- cellStart[c] = first sorted slot of cell c, 0xffffffff if empty. */
+void findCellStart(uint *particleHash, uint *cellStart, int numBodies, uint numGridCells);
+
+/* particleSystem.cu:67 -- Anei[i*maxParticlesPerCell + k]: candidates of i, |xj-xi| < cub
+ * (1.55), 27-cell stencil order; njgi / njli = number with j > i / j < i. */
+void neighborarray(double *pos, uint *particleHash, uint *cellStart, uint gridSize[3],
+                   double cellSize[3], double worldOrigin[3], int maxParticlesPerCell,
+                   int numBodies, uint *Anei, uint *njgi, uint *njli);
+
+/* particleSystem.cu:126 -- oldsjgi <- sjgi; sjgi = inclusive scan of njgi;
+ * *totalcontacts (host) = sjgi[numBodies-1]. */
+void prefixsum(uint numBodies, uint *njgi, uint *sjgi, uint *oldsjgi, uint *totalcontacts);
+
+/* particleSystem.cu:143 -- pair list Lp/Ln, reference table Aref, tangential history
+ * remapped from (oldLn, olddispt, oldfricp) by pair identity. */
+void pairlist(uint *sjgi, uint *oldsjgi, uint *oldLn, uint *Anei, uint *njgi, uint *njli,
+              double *olddispt, double *oldfricp, int maxParticlesPerCell,
+              int maxCnPerParticle, uint numBodies, uint *Aref, uint *Lp, uint *Ln,
+              double *dispt, double *fricp);
+
+/* particleSystem.cu:194 -- per-pair contact force and torques (Hertz-Mindlin, damping,
+ * Coulomb limit, rolling friction). */
+void calculateforcepair(double *pos, double *pdis, double *angv, double *rad, double *rmass,
+                        uint *Lp, uint *Ln, uint *oldLn, double *dispt, double *fricp,
+                        double *olddispt, double *oldfricp, double dt, double *forcepair,
+                        double *torqpairi, double *torqpairj, uint *contactpair,
+                        uint totalcontacts);
+
+/* particleSystem.cu:243 -- per-particle sum of pair forces/torques through Aref. */
+void sumallforcei(int maxParticlesPerCell, int numBodies, uint *njgi, uint *njli,
+                  double *forcepair, double *torqpairi, double *torqpairj, uint *contactpair,
+                  uint *Aref, double *forcei, double *torquei, uint *contacti);
+
+/* particleSystem.cu:279 -- adds gravity and particle-wall forces (one history slot). */
+void calculateforceiw(double worldSize[3], double *pos, double *pdis, double *angv,
+                      double *rad, double *rmass, double dt, uint totalcontacts,
+                      int numBodies, double *disptw, double *fricpw, double *forcei,
+                      double *torquei);
+
+/* particleSystem.cu:315 -- integrator; *iall (host) = rebuild flag, *maxhz (host) = max z. */
+void updateposition(double worldSize[3], int numBodies, double *pos, double *pdis,
+                    double *angv, double *rmass, double *inert, double *forcei,
+                    double *torquei, double *qi, double dt, double cdamp, uint *iall,
+                    double *maxhz);
+
+/* particleSystem.cu:386 -- *pcntot (host) = sum of contacti. */
+void outputporositycn(uint numBodies, uint *contacti, double maxhz, double worldSize[3],
+                      uint *pcntot);
+
+/* ----------------------------------------------------------------------------------- */
+/* 2. Context API                                                                       */
+/* ----------------------------------------------------------------------------------- */
+
+typedef struct dem_ctx dem_ctx;
+
+typedef struct dem_stats {
+    unsigned long long steps;      /* steps run */
+    unsigned long long rebuilds;   /* list rebuilds run */
+    uint n;                        /* particles (all instances) */
+    uint npairs;                   /* candidate pairs in the current list */
+    uint pair_capacity;
+    uint ninstances;
+    uint last_rebuild_flag;        /* displacement trigger raised by the last step */
+    uint overflow;
+    double maxhz;                  /* instance 0: max(0, max particle z) after the last step */
+    uint pcntot;                   /* instance 0: sum of per-particle contact counts */
+    uint pad;
+} dem_stats;
+
+/* Options for dem_set_option. */
+enum {
+    DEM_OPT_RECORD_FORCES = 1,     /* 1: keep per-particle force/torque totals for dem_get_forces */
+    DEM_OPT_REBUILD_INTERVAL = 2,  /* 0: displacement trigger (reference); K>0: every K steps */
+    DEM_OPT_DEVICE = 3             /* HIP device ordinal used at create (read-only) */
+};
+
+/* Create a context for `ninstances` independent packings (1 = plain form).  pair_capacity
+ * 0 picks 12 pairs per particle; the plain path grows it on demand. */
+int dem_create(dem_ctx **ctx, int ninstances, const dem_instance *inst, uint pair_capacity);
+int dem_destroy(dem_ctx *ctx);
+int dem_set_option(dem_ctx *ctx, int option, double value);
+
+/* Particle state in index order (instances concatenated), AoS double3 for pos/pdis/angv.
+ * pdis / angv are per-step displacements (reference convention).  `on_device` = 1 means
+ * the pointers are device pointers.  Setting the state clears all contact history and
+ * requests a list rebuild before the next step. */
+int dem_set_state(dem_ctx *ctx, const double *pos, const double *pdis, const double *angv,
+                  const double *rad, const double *rmass, const double *inert, int on_device);
+int dem_get_state(dem_ctx *ctx, double *pos, double *pdis, double *angv, double *qi,
+                  int on_device);
+/* Per-particle wall history (disptw AoS double3, fricpw) in index order. */
+int dem_get_wall_history(dem_ctx *ctx, double *disptw, double *fricpw);
+
+/* Run `nsteps` steps in the reference order: [rebuild if requested] -> pair forces ->
+ * gather + walls + gravity + integration -> rebuild trigger. */
+int dem_step(dem_ctx *ctx, int nsteps);
+/* Run `nsteps` steps as replays of a captured hipGraph (fixed rebuild interval K, set with
+ * DEM_OPT_REBUILD_INTERVAL; nsteps must be a multiple of K). */
+int dem_step_graph(dem_ctx *ctx, int nsteps);
+/* Request a list rebuild before the next step. */
+int dem_request_rebuild(dem_ctx *ctx);
+/* Rebuild now (no force step). */
+int dem_rebuild(dem_ctx *ctx);
+int dem_get_stats(dem_ctx *ctx, dem_stats *out);
+/* Per-instance scalars after the last step: maxhz[ninst], pcntot[ninst] (either may be NULL). */
+int dem_get_instance_stats(dem_ctx *ctx, double *maxhz, uint *pcntot);
+/* Synchronise the context's stream. */
+int dem_sync(dem_ctx *ctx);
+/* Device time (ms) of the last dem_step / dem_step_graph call, from HIP events. */
+double dem_last_ms(dem_ctx *ctx);
+
+/* Introspection (host buffers), for parity checks against the reference layout. */
+/* calcHash output of the last rebuild: 2n uint (cell, index), index order. */
+int dem_get_hash(dem_ctx *ctx, uint *kv);
+/* Sorted (cell, index) of the last rebuild, 2n uint. */
+int dem_get_sorted(dem_ctx *ctx, uint *kv);
+/* cellStart over all instances' cells. */
+int dem_get_cell_start(dem_ctx *ctx, uint *cs, uint ncells);
+/* Pairs in list order: ids[2p] = (index of first, index of second); hist[4p] =
+ * (dispt xyz, fricp) oriented second-relative-to-first; out[10p] = (F on first xyz,
+ * torque first xyz, torque second xyz, contact flag) of the last force step.  Any pointer
+ * may be NULL.  Returns DEM_ERR_CAPACITY if cap < npairs. */
+int dem_get_pairs(dem_ctx *ctx, uint *ids, double *hist, double *out, uint cap);
+/* Last step's per-particle totals (pairs + walls + gravity) in index order; needs
+ * DEM_OPT_RECORD_FORCES. */
+int dem_get_forces(dem_ctx *ctx, double *force, double *torque, uint *contacti);
+
+/* Device properties of the current device: CU count and peak-bandwidth-relevant info. */
+int dem_device_info(int *cu_count, char *name, int name_len);
+
+/* ----------------------------------------------------------------------------------- */
+/* Host utilities: deck, units, restart file, workload generators                       */
+/* ----------------------------------------------------------------------------------- */
+
+/* hopp3d.inp, read positionally as dempacking.cu:112-125. */
+typedef struct dem_deck {
+    int iforcemode;
+    double emod, vpois, denp;
+    double diam;
+    int ntot;
+    double dt;
+    uint tstop;
+    double dmt1, dmt2, thick;
+    double zht1, zht2;
+    double vel0;
+    int nrings;
+    double ufricp, ufricw;
+    double dampnp, damptp, dampnw, damptw, cdamp;
+    double stiffness, dampcoeffn, dampcoefft;
+    double upp, upw;
+} dem_deck;
+
+int dem_read_deck(const char *path, dem_deck *deck);
+/* Reduced material from the deck (dempacking.cu:141-149, 224-225). */
+int dem_deck_material(const dem_deck *deck, dem_material *mat);
+/* Writes restart.dat exactly as dempacking.cu:908-913 (40 bytes). */
+int dem_write_restart(const char *path);
+/* Normal damping coefficient giving restitution e for the Hertz + dashpot law (bisection
+ * on the head-on collision ODE).  Returns the coefficient, or -1 on bad input. */
+double dem_damping_for_restitution(double e);
+/* Restitution produced by damping coefficient g (same ODE). */
+double dem_restitution_for_damping(double g);
+
+/* Diameter generators (reduced units, mean-d = 1): kind 0 monodisperse (p0 = d), kind 1
+ * lognormal (p0 = sigma, clipped to [p1, p2]), kind 2 bimodal (p0 = d_big, p1 = d_small,
+ * p2 = volume fraction of big).  Deterministic for a seed. */
+int dem_gen_diameters(int kind, uint n, double p0, double p1, double p2,
+                      unsigned long long seed, double *diam);
+/* Seeded random sequential insertion without overlap inside `walls` between z0 and z1
+ * (grid-accelerated rejection).  Returns DEM_OK or DEM_ERR_CAPACITY if it cannot place
+ * every sphere within the attempt budget. */
+int dem_gen_insert(const dem_walls *walls, double z0, double z1, uint n, const double *diam,
+                   unsigned long long seed, double *pos);
+/* Jittered lattice (config 0): spacing a, jitter amplitude j (in d), box walls. */
+int dem_gen_lattice(const dem_walls *walls, double z0, double a, double jitter, uint n,
+                    unsigned long long seed, double *pos);
+/* Fill rad/rmass/inert from diameters (rmass = 8 r^3, inert = 0.4 m r^2; dempacking.cu:310-312). */
+int dem_particle_props(uint n, const double *diam, double *rad, double *rmass, double *inert);
+/* Grid covering `walls` up to height h with cell edge c (dempacking.cu:329-344 pattern). */
+int dem_make_grid(const dem_walls *walls, double h, double c, uint gsize[3], double origin[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DEM_H */

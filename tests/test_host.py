@@ -1,0 +1,122 @@
+"""Host-side logic of libdem.so (no GPU calls): deck, restart file, generators, exported ABI."""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(dem):
+    L = ctypes.CDLL(dem.LIB_PATH)
+    hdr = open(os.path.join(ROOT, "include", "dem.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^\s*(?!typedef|enum|struct|#)[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(",
+                         hdr, flags=re.M):
+        names.add(m.group(1))
+    names -= {"if", "while", "for", "sizeof"}
+    assert len(names) >= 40
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(dem.EXPORTS) <= names
+
+
+def test_deck_shipped_layout(dem):
+    """hopp3d.inp as shipped, read positionally as dempacking.cu:112-125."""
+    d = dem.read_deck(os.path.join(GOLDEN, "hopp3d.inp"))
+    assert d.iforcemode == 1
+    assert (d.emod, d.vpois, d.denp, d.diam) == (1e7, 0.29, 2500.0, 0.01)
+    assert (d.ntot, d.dt, d.tstop) == (10000, 0.001, 40000)
+    assert (d.dmt1, d.dmt2, d.thick, d.zht1, d.zht2) == (0.5, 0.5, 0.5, 0.0, 0.6)
+    assert (d.vel0, d.nrings, d.ufricp, d.ufricw) == (4.0, 9, 0.4, 0.4)
+    assert (d.dampnp, d.damptp, d.dampnw, d.damptw, d.cdamp) == (0.3, 0.3, 0.3, 0.3, 0.3)
+    assert (d.stiffness, d.dampcoeffn, d.dampcoefft, d.upp, d.upw) == (10000.0, 100.0, 100.0, 0.0, 0.0)
+
+
+def test_deck_without_iforcemode_line(dem, tmp_path):
+    """A deck lacking the leading iforcemode line (26 values) is accepted with mode 0."""
+    txt = open(os.path.join(GOLDEN, "hopp3d.inp")).read().split("\n", 1)[1]
+    p = tmp_path / "hopp3d.inp"
+    p.write_text(txt)
+    d = dem.read_deck(str(p))
+    assert d.iforcemode == 0 and d.ntot == 10000 and d.emod == 1e7
+
+
+def test_deck_truncated_is_an_error(dem, tmp_path):
+    p = tmp_path / "hopp3d.inp"
+    p.write_text("1\n10000000 0.29 2500.0\n0.01\n")
+    with pytest.raises(dem.DemError) as ei:
+        dem.read_deck(str(p))
+    assert ei.value.code == dem.DEM_ERR_IO
+
+
+def test_deck_missing_file(dem):
+    with pytest.raises(dem.DemError) as ei:
+        dem.read_deck("/nonexistent/hopp3d.inp")
+    assert ei.value.code == dem.DEM_ERR_IO
+
+
+def test_restart_file_bytes(dem, tmp_path):
+    p = tmp_path / "restart.dat"
+    dem.write_restart(str(p))
+    assert p.read_bytes() == open(os.path.join(GOLDEN, "restart.dat"), "rb").read()
+    assert len(p.read_bytes()) == 40
+
+
+def test_lognormal_diameters(dem):
+    d = dem.gen_diameters(1, 200000, 0.1, 0.7, 1.3, seed=2)
+    assert d.min() >= 0.7 and d.max() <= 1.3
+    assert d.mean() == pytest.approx(1.0, abs=3e-3)
+    assert np.log(d).std() == pytest.approx(0.1, rel=0.02)
+    assert np.array_equal(d, dem.gen_diameters(1, 200000, 0.1, 0.7, 1.3, seed=2))
+
+
+def test_bimodal_volume_fraction(dem):
+    d = dem.gen_diameters(2, 100000, 1.0, 0.5, 0.3, seed=3)
+    v = d ** 3
+    assert v[d == 1.0].sum() / v.sum() == pytest.approx(0.3, abs=2e-3)
+    assert set(np.unique(d)) == {0.5, 1.0}
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_insertion_is_non_overlapping_and_inside(dem, kind):
+    n = 3000
+    diam = dem.gen_diameters(1, n, 0.1, 0.7, 1.3, seed=5)
+    if kind == 0:
+        walls = dem.make_walls(dem.WALLS_BOX, (12.0, 10.0, 0.0))
+    else:
+        walls = dem.make_walls(dem.WALLS_CYLINDER, (0, 0, 0), radius=6.0)
+    pos = dem.gen_insert(walls, 0.0, 60.0, diam, seed=7)
+    r = diam / 2
+    for i in range(n):
+        d = np.linalg.norm(pos[i + 1:] - pos[i], axis=1)
+        assert np.all(d >= r[i] + r[i + 1:] - 1e-12)
+    if kind == 0:
+        assert np.all(np.abs(pos[:, 0]) <= 6.0 - r + 1e-12) and np.all(np.abs(pos[:, 1]) <= 5.0 - r + 1e-12)
+    else:
+        assert np.all(np.hypot(pos[:, 0], pos[:, 1]) <= 6.0 - r + 1e-12)
+    assert np.all(pos[:, 2] >= r - 1e-12) and np.all(pos[:, 2] <= 60.0 - r + 1e-12)
+
+
+def test_grid_pattern(dem):
+    """dempacking.cu:329-344: origin (-W/2, -T/2, 0), gridSize = (int)(W / cell) + 1."""
+    w = dem.make_walls(dem.WALLS_BOX, (50.0, 50.0, 60.0))
+    inst = dem.make_instance(10, w, 60.0, 1.55, dem.Material(), 1e-3)
+    assert list(inst.gsize) == [33, 33, 39]
+    assert list(inst.origin) == [-25.0, -25.0, 0.0]
+
+
+def test_config0_workload(dem):
+    w = dem.config0()
+    assert w.n == 2000 and w.rebuild_interval == 20
+    assert w.instances[0].dt == pytest.approx(9.9045e-5, rel=1e-4)
+    p = w.pos
+    for i in range(0, w.n, 7):
+        d = np.linalg.norm(p - p[i], axis=1)
+        d[i] = 9
+        assert d.min() >= 1.0
