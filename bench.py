@@ -1,0 +1,141 @@
+"""Benchmark: particle-steps/s of the plain DEM step at 500K spheres on one MI355X.
+
+Workload: 500,000 lognormal spheres (mean d = 1 mm, sigma 0.1, clipped to [0.7, 1.3] mm) inserted
+at random without overlap into a cylinder of radius 40 mean-d with a flat floor; E = 1e7 Pa,
+nu = 0.29, rho = 2500 kg/m^3, sliding friction 0.3, rolling friction 0.002 d, restitution 0.6,
+dt = 1e-6 s, FP64, reference rebuild policy (displacement trigger).  Inputs are resident in HBM
+before timing.  One step = [list rebuild when triggered] + pair forces + gather/walls/gravity/
+integration.
+
+Prints ONE JSON line.  The CPU baseline is the oracle (the same PCCP algorithm in C with
+OpenMP) on a bounded sample of the same workload.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "discrete-element-method-for-packing-cuda-c-code_amd")
+METRIC = "particle-steps/s at 500K spheres (FP64, plain vs batched); adjoint/forward cost"
+PUBLISHED_500K = 9.8e6      # GPU-PCCP at 500K spheres, reference Figure 2(a)
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "dem_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["dem_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(P, N):
+    """Algorithmic bytes per launch (one touch per array element, reference element sizes:
+    double3 24, double 8, uint 4, uint2 8).
+    pair kernel  : pair (uint2) 8 + history r/w 2*32 + output (F, Ta, Tb, flag) 80 per pair;
+                   gathered particle data (pos, pdis, angv, rad, rmass = 88 B) once per particle.
+    particle kern: pair output 80 + transposed index 4 per pair; per particle record r/w 2*128,
+                   wall history r/w 2*32, two CSR offsets 8, contact count 4."""
+    return {"pair": 152 * P + 88 * N, "particle": 84 * P + 332 * N}
+
+
+def cpu_baseline(dem, w, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=64)
+    t0 = time.perf_counter()
+    assert sim.step(1) == 0          # first step builds the lists
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    assert sim.step(2) == 0
+    per = (time.perf_counter() - t0) / 2
+    k = max(2, min(500, int(budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    assert sim.step(k) == 0
+    el = time.perf_counter() - t0
+    return {"value": w.n * k / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{k} non-rebuild steps of the same {w.n}-sphere workload after its first "
+                      f"step (list build {t_first:.1f} s, excluded), oracle C/OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=500_000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=30)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
+              f"multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    dem = load_pkg()
+    t0 = time.perf_counter()
+    w = dem.config1(n=args.n)
+    log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
+    ctx = dem.Context(w.instances)
+    ctx.load(w)
+    ctx.step(args.warmup)
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    ctx.step(args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    value = w.n * args.steps / el
+    log(f"timed {args.steps} steps in {el:.3f} s: {value:.4e} particle-steps/s, "
+        f"P={s1.npairs} rebuilds={s1.rebuilds - s0.rebuilds}")
+    kt = ctx.time_kernels(args.kernel_steps)
+    P, N = s1.npairs, w.n
+    kb = kernel_bytes(P, N)
+    dom = "pair" if kt["pair_ms"] >= kt["particle_ms"] else "particle"
+    ms = kt[f"{dom}_ms"]
+    achieved = kb[dom] / (ms * 1e-3) / 1e9
+    log(f"kernels: pair {kt['pair_ms'] * 1e3:.1f} us, particle {kt['particle_ms'] * 1e3:.1f} us, "
+        f"rebuild {kt['rebuild_ms']:.3f} ms; {dom}: {achieved:.0f} GB/s")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_pair_force" if dom == "pair" else "k_particle",
+            "bytes_per_launch": kb[dom]}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(dem, w, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_500K,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name, "n": N, "npairs": P, "pairs_per_particle": round(P / N, 3),
+                   "rebuild_policy": "displacement trigger |q| > 0.275",
+                   "rebuilds_timed": int(s1.rebuilds - s0.rebuilds),
+                   "kernel_ms": {k: round(v, 5) for k, v in kt.items() if k.endswith("_ms")}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -125,7 +125,7 @@ EXPORTS = [
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
     "dem_get_state", "dem_get_wall_history", "dem_step", "dem_step_graph",
     "dem_request_rebuild", "dem_rebuild", "dem_get_stats", "dem_get_instance_stats",
-    "dem_sync", "dem_last_ms", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
+    "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
     "dem_get_pairs", "dem_get_forces", "dem_device_info", "dem_read_deck",
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
     "dem_restitution_for_damping", "dem_gen_diameters", "dem_gen_insert", "dem_gen_lattice",
@@ -159,6 +159,7 @@ def lib() -> ctypes.CDLL:
         "dem_get_instance_stats": ([P, P, P], c_int),
         "dem_sync": ([P], c_int),
         "dem_last_ms": ([P], c_double),
+        "dem_time_kernels": ([P, c_int, P, P], c_int),
         "dem_get_hash": ([P, P], c_int),
         "dem_get_sorted": ([P, P], c_int),
         "dem_get_cell_start": ([P, P, c_uint], c_int),
@@ -501,6 +502,15 @@ class Context:
 
     def last_ms(self) -> float:
         return float(self._L.dem_last_ms(self._h))
+
+    def time_kernels(self, nsteps: int):
+        """Per-kernel mean device times (ms) over `nsteps` steps: pair, particle, rebuild."""
+        out = np.zeros(4)
+        nr = c_int(0)
+        _check(self._L.dem_time_kernels(self._h, int(nsteps), _ptr(out), ctypes.byref(nr)),
+               "dem_time_kernels")
+        return {"pair_ms": out[0], "particle_ms": out[1], "rebuild_ms": out[2],
+                "total_ms": out[3], "rebuilds": nr.value}
 
     def sync(self):
         _check(self._L.dem_sync(self._h), "dem_sync")

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
     unsigned cap, uint2 *ab, Hist *hist, unsigned *jt, const unsigned *inv_old,
     const uint2 *ab_old, const unsigned *off_old, const Hist *hist_old, unsigned n_old,
-    DevCtl *ctl, const unsigned *active)
+    unsigned cap_old, DevCtl *ctl, const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned a = blockIdx.x * kBlock + threadIdx.x;
@@ -269,10 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
                 unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
                 if (ob != kEmpty) {
                     if (oa < ob) {
-                        unsigned q = find_second(ab_old, off_old[oa], off_old[oa + 1], ob);
+                        unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
+                        unsigned q = find_second(ab_old, lo, hi, ob);
                         if (q != kEmpty) h = hist_old[q];
                     } else {
-                        unsigned q = find_second(ab_old, off_old[ob], off_old[ob + 1], oa);
+                        unsigned lo = min(off_old[ob], cap_old), hi = min(off_old[ob + 1], cap_old);
+                        unsigned q = find_second(ab_old, lo, hi, oa);
                         if (q != kEmpty) {
                             Hist o = hist_old[q];
                             h = Hist{-o.hx, -o.hy, -o.hz, o.fr};

@@ -305,12 +305,14 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list, c->ctl, active);
+                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
+                           std::min(c->cap[ol], c->out_cap), c->ctl, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list, c->ctl, active);
+                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
+                           std::min(c->cap[ol], c->out_cap), c->ctl, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
                        active);
@@ -350,7 +352,7 @@ int rebuild_host(dem_ctx *c)
     return DEM_OK;
 }
 
-void launch_force_step(dem_ctx *c, unsigned np_grid)
+void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr)
 {
     hipStream_t st = c->stream;
     unsigned gb = cdiv(c->n, kBlock);
@@ -360,6 +362,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid)
         hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(kBlock), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            c->whist[c->rc], (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
@@ -370,6 +373,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid)
         hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(kBlock), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            c->whist[c->rc], (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
@@ -823,6 +827,61 @@ int dem_step_graph(dem_ctx *c, int nsteps)
     const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
     c->last_flag = m->need_rebuild;
     if (w[1]) return DEM_ERR_CAPACITY;
+    return DEM_OK;
+}
+
+// Same loop as dem_step, with HIP events on the context stream around the rebuild, the pair
+// kernel and the per-particle kernel of every step.  out_ms[0] = mean pair-kernel time,
+// out_ms[1] = mean per-particle-kernel time, out_ms[2] = mean rebuild time (0 if none),
+// out_ms[3] = total device time; *nrebuild = rebuilds run.
+int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
+{
+    if (!c || nsteps < 1 || !out_ms) return DEM_ERR_INVALID;
+    hipEvent_t e0, e1, e2, e3, r0, r1;
+    DEM_HIP(hipEventCreate(&e0));
+    DEM_HIP(hipEventCreate(&e1));
+    DEM_HIP(hipEventCreate(&e2));
+    DEM_HIP(hipEventCreate(&e3));
+    DEM_HIP(hipEventCreate(&r0));
+    DEM_HIP(hipEventCreate(&r1));
+    double t_pair = 0.0, t_part = 0.0, t_reb = 0.0, t_all = 0.0;
+    int nreb = 0;
+    for (int s = 0; s < nsteps; s++) {
+        if (c->need_rebuild || (c->interval > 0 && c->since >= (unsigned)c->interval)) {
+            DEM_HIP(hipEventRecord(r0, c->stream));
+            int rc = rebuild_host(c);
+            if (rc) return rc;
+            DEM_HIP(hipEventRecord(r1, c->stream));
+            DEM_HIP(hipEventSynchronize(r1));
+            float ms;
+            DEM_HIP(hipEventElapsedTime(&ms, r0, r1));
+            t_reb += ms;
+            nreb++;
+        }
+        DEM_HIP(hipEventRecord(e0, c->stream));
+        launch_force_step(c, pair_grid(c, c->npairs), e1);
+        DEM_HIP(hipEventRecord(e2, c->stream));
+        DEM_HIP(hipGetLastError());
+        c->steps++;
+        c->since++;
+        int rc = sync_mirror(c);
+        if (rc) return rc;
+        if (c->interval == 0) c->need_rebuild = c->last_flag != 0u;
+        float a, b;
+        DEM_HIP(hipEventElapsedTime(&a, e0, e1));
+        DEM_HIP(hipEventElapsedTime(&b, e1, e2));
+        t_pair += a;
+        t_part += b;
+        t_all += a + b;
+    }
+    (void)e3;
+    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
+    hipEventDestroy(r0); hipEventDestroy(r1);
+    out_ms[0] = t_pair / nsteps;
+    out_ms[1] = t_part / nsteps;
+    out_ms[2] = nreb ? t_reb / nreb : 0.0;
+    out_ms[3] = t_all + t_reb;
+    if (nrebuild) *nrebuild = nreb;
     return DEM_OK;
 }
 

@@ -223,6 +223,10 @@ int dem_get_instance_stats(dem_ctx *ctx, double *maxhz, uint *pcntot);
 int dem_sync(dem_ctx *ctx);
 /* Device time (ms) of the last dem_step / dem_step_graph call, from HIP events. */
 double dem_last_ms(dem_ctx *ctx);
+/* dem_step with HIP events around each kernel (synchronises every step): out_ms[0] mean
+ * pair-force kernel time, out_ms[1] mean per-particle kernel time, out_ms[2] mean rebuild
+ * time, out_ms[3] total; *nrebuild = rebuilds run. */
+int dem_time_kernels(dem_ctx *ctx, int nsteps, double *out_ms, int *nrebuild);
 
 /* Introspection (host buffers), for parity checks against the reference layout. */
 /* calcHash output of the last rebuild: 2n uint (cell, index), index order. */

