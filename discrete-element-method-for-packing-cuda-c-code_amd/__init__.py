@@ -126,10 +126,11 @@ EXPORTS = [
     "dem_get_state", "dem_get_wall_history", "dem_step", "dem_step_graph",
     "dem_request_rebuild", "dem_rebuild", "dem_get_stats", "dem_get_instance_stats",
     "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
-    "dem_get_pairs", "dem_get_forces", "dem_device_info", "dem_read_deck",
+    "dem_get_pairs", "dem_get_contacts", "dem_get_forces", "dem_device_info", "dem_read_deck",
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
     "dem_restitution_for_damping", "dem_gen_diameters", "dem_gen_insert", "dem_gen_lattice",
-    "dem_particle_props", "dem_make_grid",
+    "dem_particle_props", "dem_make_grid", "dem_dev_alloc", "dem_dev_free", "dem_memcpy",
+    "dem_memzero", "dem_device_sync",
 ]
 
 
@@ -165,7 +166,13 @@ def lib() -> ctypes.CDLL:
         "dem_get_cell_start": ([P, P, c_uint], c_int),
         "dem_get_pairs": ([P, P, P, P, c_uint], c_int),
         "dem_get_forces": ([P, P, P, P], c_int),
+        "dem_get_contacts": ([P, P], c_int),
         "dem_device_info": ([P, ctypes.c_char_p, c_int], c_int),
+        "dem_dev_alloc": ([ctypes.c_size_t], P),
+        "dem_dev_free": ([P], c_int),
+        "dem_memcpy": ([P, P, ctypes.c_size_t, c_int], c_int),
+        "dem_memzero": ([P, ctypes.c_size_t], c_int),
+        "dem_device_sync": ([], c_int),
         "dem_read_deck": ([ctypes.c_char_p, P], c_int),
         "dem_deck_material": ([P, P], c_int),
         "dem_write_restart": ([ctypes.c_char_p], c_int),
@@ -539,12 +546,92 @@ class Context:
                "dem_get_pairs")
         return ids, hist, out
 
+    def contacts(self):
+        c = np.empty(self.n, dtype=np.uint32)
+        _check(self._L.dem_get_contacts(self._h, _ptr(c)), "dem_get_contacts")
+        return c
+
     def forces(self):
         f = np.empty((self.n, 3))
         t = np.empty((self.n, 3))
         c = np.empty(self.n, dtype=np.uint32)
         _check(self._L.dem_get_forces(self._h, _ptr(f), _ptr(t), _ptr(c)), "dem_get_forces")
         return f, t, c
+
+
+def packing_stats(pos, rad, contacti, walls, margin: float = 2.0, top_margin: float = 3.0) -> dict:
+    """Packing fraction and mean coordination number of a bed.
+
+    Reference formulas (dempacking.cu:768-770): CN = sum(contacti) / N and
+    phi = sum(pi/6 d^3) / (floor area * maxhz), maxhz = highest centre (the reference has
+    N*pi/6 for unit spheres).  Inner-region estimate: spheres whose centres lie at least
+    `margin` from the walls and floor and `top_margin` below maxhz; phi_inner = their volume /
+    region volume, cn_inner = their mean contact count."""
+    pos = np.asarray(pos)
+    rad = np.asarray(rad)
+    c = np.asarray(contacti, dtype=np.float64)
+    vol = math.pi / 6.0 * (2.0 * rad) ** 3
+    maxhz = max(float(pos[:, 2].max()), 0.0)
+    if walls.kind == WALLS_BOX:
+        area = walls.size[0] * walls.size[1]
+        inside = (np.abs(pos[:, 0]) < walls.size[0] / 2 - margin) & \
+                 (np.abs(pos[:, 1]) < walls.size[1] / 2 - margin)
+        area_in = (walls.size[0] - 2 * margin) * (walls.size[1] - 2 * margin)
+    else:
+        area = math.pi * walls.radius ** 2
+        inside = np.hypot(pos[:, 0], pos[:, 1]) < walls.radius - margin
+        area_in = math.pi * (walls.radius - margin) ** 2
+    z0, z1 = margin, maxhz - top_margin
+    inside &= (pos[:, 2] > z0) & (pos[:, 2] < z1)
+    region = area_in * max(z1 - z0, 0.0)
+    return {
+        "phi_ref": float(vol.sum() / (area * maxhz)) if maxhz > 0 else 0.0,
+        "cn_ref": float(c.sum() / (len(c) + 1e-30)),
+        "phi_inner": float(vol[inside].sum() / region) if region > 0 else 0.0,
+        "cn_inner": float(c[inside].mean()) if inside.any() else 0.0,
+        "n_inner": int(inside.sum()),
+        "maxhz": maxhz,
+    }
+
+
+class DeviceArray:
+    """A device allocation from libdem (for driving the drop-in entry points without torch)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib().dem_dev_alloc(self.nbytes)
+        if not self.ptr:
+            raise DemError(DEM_ERR_HIP, "dem_dev_alloc")
+        self.zero()
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        _check(lib().dem_memcpy(d.ptr, a.ctypes.data_as(c_void_p), d.nbytes, 1), "dem_memcpy")
+        return d
+
+    def zero(self):
+        _check(lib().dem_memzero(self.ptr, self.nbytes), "dem_memzero")
+
+    def host(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().dem_memcpy(out.ctypes.data_as(c_void_p), self.ptr, self.nbytes, 2), "dem_memcpy")
+        return out
+
+    @property
+    def p(self):
+        return c_void_p(self.ptr)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().dem_dev_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
 
 
 def read_deck(path: str) -> Deck:

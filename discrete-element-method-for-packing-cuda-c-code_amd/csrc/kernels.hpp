@@ -17,6 +17,10 @@ namespace dem {
 
 constexpr unsigned kEmpty = 0xffffffffu;
 constexpr int kBlock = 256;
+// The second member of a pair carries, in its top bit, whether the pair's slot order is the
+// reverse of its particle-index order (first member has the larger index).
+constexpr unsigned kFlip = 0x80000000u;
+constexpr unsigned kSlotMask = 0x7fffffffu;
 
 // 128-byte particle record in slot order.  pos/rad, pdis/mass and angv/inert pack the data a
 // pair gathers into the first 96 bytes; q (displacement since the last rebuild), the
@@ -36,7 +40,7 @@ struct __attribute__((aligned(16))) Hist {
 };
 
 // Per-pair force output of the last step (80 bytes): force on first, torque on first,
-// torque on second, contact flag (1.0 / 0.0).
+// torque on second, and flag = contact (0/1) + 2 * index-order-flipped.
 struct __attribute__((aligned(16))) POut {
     double fx, fy, fz, flag;
     double tax, tay, taz, tbx;
@@ -55,18 +59,16 @@ struct InstDev {
     unsigned p_off, n, pad1, pad2;
 };
 
-// Device control block.  Published to a host-mapped mirror by the last block of k_particle.
+// Device control block, copied to a host-mapped mirror by k_publish after each step.  The
+// mirror continues with per-instance max-z bit patterns (u64[ninst]) and contact totals
+// (u32[ninst]) written by k_inst_reduce on demand.
 struct DevCtl {
     unsigned need_rebuild;   // trigger raised by the last step (atomicOr)
-    unsigned steps;          // steps completed (incremented by the last k_particle block)
+    unsigned steps;          // steps completed (incremented by k_publish)
     unsigned npairs;         // written by the pair scan
     unsigned overflow;       // pair capacity exceeded at the last rebuild
-    unsigned done_blocks;    // k_particle last-block ticket
-    unsigned pad[3];
+    unsigned pad[4];
 };
-// Per-instance accumulators follow DevCtl in the same allocation:
-//   unsigned long long maxz_bits[ninst]; unsigned pcntot[ninst];
-// The mirror has the same layout.
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
 __device__ __forceinline__ void st2(double *p, double a, double b)
@@ -229,7 +231,7 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
 {
     while (lo < hi) {
         unsigned mid = (lo + hi) >> 1;
-        unsigned v = ab[mid].y;
+        unsigned v = ab[mid].y & kSlotMask;
         if (v == target) return mid;
         if (v < target) lo = mid + 1;
         else hi = mid;
@@ -262,10 +264,10 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     for_candidates(rec, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
         if (b > a) {
             if (pg >= cap) { over = true; return; }
-            ab[pg] = make_uint2(a, b);
+            unsigned idb = rec[b].id;
+            ab[pg] = make_uint2(a, b | (ida > idb ? kFlip : 0u));
             Hist h = Hist{0.0, 0.0, 0.0, 0.0};
             if (oa != kEmpty) {
-                unsigned idb = rec[b].id;
                 unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
                 if (ob != kEmpty) {
                     if (oa < ob) {
@@ -346,6 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
     const unsigned np = *npairs;
     for (unsigned p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
         uint2 e = ab[p];
+        double flip = (e.y & kFlip) ? 2.0 : 0.0;
+        e.y &= kSlotMask;
         const double *A = reinterpret_cast<const double *>(rec + e.x);
         const double *B = reinterpret_cast<const double *>(rec + e.y);
         double2 a0 = ld2(A), a1 = ld2(A + 2), a2 = ld2(A + 4), a3 = ld2(A + 6), a4 = ld2(A + 8),
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                           fo);
         double *O = reinterpret_cast<double *>(out + p);
         st2(O, fa.x, fa.y);
-        st2(O + 2, fa.z, c ? 1.0 : 0.0);
+        st2(O + 2, fa.z, (c ? 1.0 : 0.0) + flip);
         st2(O + 4, ta.x, ta.y);
         st2(O + 6, ta.z, tb.x);
         st2(O + 8, tb.y, tb.z);
@@ -382,51 +386,58 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
 // Per-particle kernel: gather of the particle's pair forces (own pairs in list order, then
 // pairs where it is second, via the transposed index), then gravity and walls
 // (calculateforceiwD, neiglist.cu:238-306), then the integrator (updatepositionD,
-// neiglist.cu:308-409) with the rebuild trigger |q|^2 > ann2 and per-instance max z and
-// contact totals.  The last block publishes the control block to the host mirror.
+// neiglist.cu:308-409) with the rebuild trigger |q|^2 > ann2.  Max z and contact totals are
+// not reduced here (they are diagnostics): k_inst_reduce computes them on demand.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, Hist *whist, const unsigned *off, const unsigned *toff, const unsigned *jt,
     const POut *out, unsigned n, const InstDev *inst, InstDev one, unsigned *contacti,
-    double *fdbg, DevCtl *ctl, unsigned ninst, unsigned char *mirror)
+    double *fdbg, DevCtl *ctl)
 {
-    unsigned long long *maxz = reinterpret_cast<unsigned long long *>(ctl + 1);
-    unsigned *pcnt = reinterpret_cast<unsigned *>(maxz + ninst);
     const unsigned np = ctl->npairs;
     unsigned a = blockIdx.x * kBlock + threadIdx.x;
     bool live = a < n;
-    unsigned my_inst = 0, flag = 0, cnt = 0;
-    unsigned long long zbits = 0ull;
+    unsigned flag = 0;
     if (live) {
         double *R = reinterpret_cast<double *>(rec + a);
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
         unsigned p0 = off[a], p1 = off[a + 1];
         if (p1 > np) p1 = np;
-        for (unsigned p = p0; p < p1; p++) {
-            const double *O = reinterpret_cast<const double *>(out + p);
-            double2 o0 = ld2(O), o1 = ld2(O + 2), o2 = ld2(O + 4), o3 = ld2(O + 6);
-            fx += o0.x; fy += o0.y; fz += o1.x;
-            tx += o2.x; ty += o2.y; tz += o3.x;
-            c += o1.y != 0.0 ? 1u : 0u;
-        }
         unsigned q0 = toff[a], q1 = toff[a + 1];
         if (q1 > np) q1 = np;
-        for (unsigned k = q0; k < q1; k++) {
-            unsigned q = jt[k];
-            if (q >= np) continue;
-            const double *O = reinterpret_cast<const double *>(out + q);
-            double2 o0 = ld2(O), o1 = ld2(O + 2), o3 = ld2(O + 6), o4 = ld2(O + 8);
-            fx -= o0.x; fy -= o0.y; fz -= o1.x;
-            tx += o3.y; ty += o4.x; tz += o4.y;
-            c += o1.y != 0.0 ? 1u : 0u;
+        // Sum in the reference's order (sumallforceiD's Aref rows): partners with a larger
+        // particle index first, then those with a smaller index, each group in ascending slot
+        // order (partners below this slot come from the transposed list, those above from the
+        // own list).  The terms are the force / torque on this particle either way, so the
+        // result equals the oracle's sum bit for bit.
+        for (int pass = 0; pass < 2; pass++) {
+            for (unsigned k = q0; k < q1; k++) {
+                unsigned q = jt[k];
+                if (q >= np) continue;
+                const double *O = reinterpret_cast<const double *>(out + q);
+                double2 o0 = ld2(O), o1 = ld2(O + 2), o3 = ld2(O + 6), o4 = ld2(O + 8);
+                bool larger = o1.y >= 2.0;   // first member (partner) has the larger index
+                if (larger != (pass == 0)) continue;
+                fx -= o0.x; fy -= o0.y; fz -= o1.x;
+                tx += o3.y; ty += o4.x; tz += o4.y;
+                c += (o1.y == 1.0 || o1.y == 3.0) ? 1u : 0u;
+            }
+            for (unsigned p = p0; p < p1; p++) {
+                const double *O = reinterpret_cast<const double *>(out + p);
+                double2 o0 = ld2(O), o1 = ld2(O + 2), o2 = ld2(O + 4), o3 = ld2(O + 6);
+                bool larger = o1.y < 2.0;    // second member (partner) has the larger index
+                if (larger != (pass == 0)) continue;
+                fx += o0.x; fy += o0.y; fz += o1.x;
+                tx += o2.x; ty += o2.y; tz += o3.x;
+                c += (o1.y == 1.0 || o1.y == 3.0) ? 1u : 0u;
+            }
         }
         double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
                 r5 = ld2(R + 10), r6 = ld2(R + 12);
         double qz = R[14];
         uint2 tail = *reinterpret_cast<const uint2 *>(R + 15);
-        my_inst = tail.x;
-        const InstDev &I = BATCHED ? inst[my_inst] : one;
+        const InstDev &I = BATCHED ? inst[tail.x] : one;
         d3 x = d3{r0.x, r0.y, r1.x};
         double rad = r1.y;
         d3 v = d3{r2.x, r2.y, r3.x};
@@ -479,61 +490,62 @@ __global__ __launch_bounds__(kBlock) void k_particle(
         contacti[a] = c;
         double q2 = qx * qx + qy * qy + qzz * qzz;
         flag = q2 > I.ann2 ? 1u : 0u;
-        cnt = c;
-        double zz = x.z > 0.0 ? x.z : 0.0;
-        zbits = (unsigned long long)__double_as_longlong(zz);
     }
-    // per-instance reductions: wave-uniform fast path, per-lane atomics otherwise
-    unsigned inst0 = __builtin_amdgcn_readfirstlane(my_inst);
-    bool uniform = __all(!live || my_inst == inst0);
-    if (uniform) {
-        unsigned long long zb = zbits;
-        unsigned f = flag, cc = cnt;
+    // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
+    if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
+}
+
+// Publish the control block to the host-mapped mirror (one lane; after k_particle).
+__global__ void k_publish(DevCtl *ctl, unsigned char *mirror)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->steps = ctl->steps + 1u;
+        volatile unsigned *src = reinterpret_cast<volatile unsigned *>(ctl);
+        unsigned *dst = reinterpret_cast<unsigned *>(mirror);
+        for (unsigned k = 0; k < sizeof(DevCtl) / 4; k++) dst[k] = src[k];
+    }
+}
+
+// Per-instance max(0, max z) and contact totals (the reference's host max over
+// updatepositionD's block partials, particleSystem.cu:372-379, and outputporositycn).
+// One block per instance; exact (max of non-negative doubles as ordered bit patterns,
+// integer sums), so the result does not depend on the reduction order.
+__global__ __launch_bounds__(kBlock) void k_inst_reduce(const PRec *rec, const unsigned *contacti,
+                                                        const InstDev *inst, unsigned ninst,
+                                                        const unsigned *slot_inst_begin,
+                                                        unsigned char *mirror)
+{
+    unsigned k = blockIdx.x;
+    if (k >= ninst) return;
+    unsigned s0 = slot_inst_begin[k], s1 = slot_inst_begin[k + 1];
+    unsigned long long zb = 0ull;
+    unsigned cc = 0;
+    for (unsigned s = s0 + threadIdx.x; s < s1; s += kBlock) {
+        double z = rec[s].pz;
+        unsigned long long b = (unsigned long long)__double_as_longlong(z > 0.0 ? z : 0.0);
+        zb = b > zb ? b : zb;
+        cc += contacti[s];
+    }
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            unsigned long long oz = __shfl_xor(zb, d, 64);
-            zb = oz > zb ? oz : zb;
-            f |= __shfl_xor(f, d, 64);
-            cc += __shfl_xor(cc, d, 64);
-        }
-        if (__lane_id() == 0 && __any(live)) {
-            if (zb) atomicMax(&maxz[inst0], zb);
-            if (cc) atomicAdd(&pcnt[inst0], cc);
-            if (f) atomicOr(&ctl->need_rebuild, 1u);
-        }
-    } else if (live) {
-        if (zbits) atomicMax(&maxz[my_inst], zbits);
-        if (cnt) atomicAdd(&pcnt[my_inst], cnt);
-        if (flag) atomicOr(&ctl->need_rebuild, 1u);
+    for (int d = 32; d >= 1; d >>= 1) {
+        unsigned long long oz = __shfl_xor(zb, d, 64);
+        zb = oz > zb ? oz : zb;
+        cc += __shfl_xor(cc, d, 64);
     }
-    // last block publishes the control block and resets the per-step accumulators
-    __shared__ unsigned s_last;
-    __threadfence();
+    __shared__ unsigned long long s_z[kBlock / 64];
+    __shared__ unsigned s_c[kBlock / 64];
+    if (__lane_id() == 0) { s_z[threadIdx.x >> 6] = zb; s_c[threadIdx.x >> 6] = cc; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned t = atomicAdd(&ctl->done_blocks, 1u);
-        s_last = (t == gridDim.x - 1) ? 1u : 0u;
+        zb = s_z[0];
+        cc = s_c[0];
+        for (int w = 1; w < kBlock / 64; w++) { zb = s_z[w] > zb ? s_z[w] : zb; cc += s_c[w]; }
+        unsigned long long *mz = reinterpret_cast<unsigned long long *>(mirror + sizeof(DevCtl));
+        unsigned *mp = reinterpret_cast<unsigned *>(mz + ninst);
+        mz[k] = zb;
+        mp[k] = cc;
     }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        volatile unsigned *vc = reinterpret_cast<volatile unsigned *>(ctl);
-        if (mirror) {
-            unsigned *mc = reinterpret_cast<unsigned *>(mirror);
-            if (threadIdx.x < sizeof(DevCtl) / 4) mc[threadIdx.x] = vc[threadIdx.x];
-            unsigned long long *mz = reinterpret_cast<unsigned long long *>(mirror + sizeof(DevCtl));
-            unsigned *mp = reinterpret_cast<unsigned *>(mz + ninst);
-            volatile unsigned long long *vz = maxz;
-            volatile unsigned *vp = pcnt;
-            for (unsigned k = threadIdx.x; k < ninst; k += kBlock) { mz[k] = vz[k]; mp[k] = vp[k]; }
-        }
-        __syncthreads();
-        for (unsigned k = threadIdx.x; k < ninst; k += kBlock) { maxz[k] = 0ull; pcnt[k] = 0u; }
-        if (threadIdx.x == 0) {
-            ctl->done_blocks = 0u;
-            ctl->steps = ctl->steps + 1u;
-        }
-    }
+    (void)inst;
 }
 
 // ----------------------------------------------------------------------------------------

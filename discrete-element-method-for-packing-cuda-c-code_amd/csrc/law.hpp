@@ -177,6 +177,7 @@ __device__ __forceinline__ int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, do
     int nw = wall_count(g);
     int prim = -1;
     double best = 0.0;
+#pragma unroll 1
     for (int k = 0; k < nw; k++) {
         d3 n;
         double dn = wall_overlap(g, k, x, r, n);
@@ -186,6 +187,7 @@ __device__ __forceinline__ int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, do
     fout = 0.0;
     if (prim < 0) return 0;
     int hits = 0;
+#pragma unroll 1
     for (int k = 0; k < nw; k++) {
         d3 n;
         double dn = wall_overlap(g, k, x, r, n);

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -204,6 +205,7 @@ struct dem_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evs = nullptr;
     float last_ms = 0.f;
 
+    unsigned *inst_begin = nullptr;     // slot range of each instance, [ninst + 1]
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     int graph_k = 0;
@@ -352,7 +354,8 @@ int rebuild_host(dem_ctx *c)
     return DEM_OK;
 }
 
-void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr)
+void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
+                       hipEvent_t end = nullptr)
 {
     hipStream_t st = c->stream;
     unsigned gb = cdiv(c->n, kBlock);
@@ -367,8 +370,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr)
                            c->whist[c->rc], (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
-                           (unsigned)c->ninst, c->mirror_d);
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
     } else {
         hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(kBlock), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
@@ -378,9 +380,10 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr)
                            c->whist[c->rc], (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
-                           (unsigned)c->ninst, c->mirror_d);
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
     }
+    if (end) (void)hipEventRecord(end, st);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d);
 }
 
 InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
@@ -414,7 +417,7 @@ void free_ctx(dem_ctx *c)
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
-                    c->hist[1], c->out, c->jt, c->ctl};
+                    c->hist[1], c->out, c->jt, c->ctl, c->inst_begin};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->S.release();
@@ -494,6 +497,41 @@ const char *dem_status_string(int s)
     }
 }
 
+void *dem_dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+int dem_dev_free(void *p)
+{
+    if (p) DEM_HIP(hipFree(p));
+    return DEM_OK;
+}
+
+int dem_memcpy(void *dst, const void *src, size_t bytes, int kind)
+{
+    static const hipMemcpyKind kinds[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice,
+                                           hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (kind < 0 || kind > 3) return DEM_ERR_INVALID;
+    DEM_HIP(hipMemcpy(dst, src, bytes, kinds[kind]));
+    return DEM_OK;
+}
+
+int dem_memzero(void *dst, size_t bytes)
+{
+    DEM_HIP(hipMemset(dst, 0, bytes));
+    DEM_HIP(hipDeviceSynchronize());
+    return DEM_OK;
+}
+
+int dem_device_sync(void)
+{
+    DEM_HIP(hipDeviceSynchronize());
+    return DEM_OK;
+}
+
 int dem_device_info(int *cu_count, char *name, int name_len)
 {
     int dev = 0;
@@ -527,7 +565,7 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
             return DEM_ERR_INVALID;
         }
         unsigned long long cells = (unsigned long long)d.gsize[0] * d.gsize[1] * d.gsize[2];
-        if (cells + cell_off >= 0xffffffffull || (unsigned long long)p_off + d.n >= 0xffffffffull) {
+        if (cells + cell_off >= 0xffffffffull || (unsigned long long)p_off + d.n >= (unsigned long long)kFlip) {
             free_ctx(c);
             return DEM_ERR_INVALID;
         }
@@ -564,6 +602,13 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->ng, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->toff, sizeof(unsigned) * ((size_t)n + 1)));
+    {
+        std::vector<unsigned> ib(ninst + 1);
+        for (int i = 0; i < ninst; i++) ib[i] = c->hinst[i].p_off;
+        ib[ninst] = c->n;
+        DEM_HIP(hipMalloc(&c->inst_begin, sizeof(unsigned) * (ninst + 1)));
+        DEM_HIP(hipMemcpy(c->inst_begin, ib.data(), sizeof(unsigned) * (ninst + 1), hipMemcpyHostToDevice));
+    }
     c->ctl_bytes = sizeof(DevCtl) + (size_t)ninst * (sizeof(unsigned long long) + sizeof(unsigned));
     DEM_HIP(hipMalloc(&c->ctl, c->ctl_bytes));
     DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
@@ -859,8 +904,7 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
             nreb++;
         }
         DEM_HIP(hipEventRecord(e0, c->stream));
-        launch_force_step(c, pair_grid(c, c->npairs), e1);
-        DEM_HIP(hipEventRecord(e2, c->stream));
+        launch_force_step(c, pair_grid(c, c->npairs), e1, e2);
         DEM_HIP(hipGetLastError());
         c->steps++;
         c->since++;
@@ -885,10 +929,24 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
     return DEM_OK;
 }
 
+static int reduce_instances(dem_ctx *c)
+{
+    // slots of instance k are [p_off, p_off + n) in sorted order too: the sort keys are
+    // offset per instance, so the reorder keeps instances contiguous and in order
+    hipLaunchKernelGGL(k_inst_reduce, dim3(c->ninst), dim3(kBlock), 0, c->stream,
+                       (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti,
+                       (const InstDev *)c->dinst, (unsigned)c->ninst,
+                       (const unsigned *)c->inst_begin, c->mirror_d);
+    DEM_HIP(hipGetLastError());
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    return DEM_OK;
+}
+
 int dem_get_stats(dem_ctx *c, dem_stats *o)
 {
     if (!c || !o) return DEM_ERR_INVALID;
     DEM_HIP(hipStreamSynchronize(c->stream));
+    if (c->steps && reduce_instances(c)) return DEM_ERR_HIP;
     const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
     std::memset(o, 0, sizeof(*o));
     o->steps = c->steps;
@@ -913,6 +971,7 @@ int dem_get_instance_stats(dem_ctx *c, double *maxhz, uint *pcntot)
 {
     if (!c) return DEM_ERR_INVALID;
     DEM_HIP(hipStreamSynchronize(c->stream));
+    if (c->steps && reduce_instances(c)) return DEM_ERR_HIP;
     const volatile unsigned long long *mz = reinterpret_cast<const volatile unsigned long long *>(c->mirror_h + sizeof(DevCtl));
     const volatile unsigned *mp = reinterpret_cast<const volatile unsigned *>(mz + c->ninst);
     for (int k = 0; k < c->ninst; k++) {
@@ -972,11 +1031,26 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
         DEM_HIP(hipMemcpy(ab.data(), c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToHost));
         for (size_t p = 0; p < np; p++) {
             ids[2 * p] = recs[ab[p].x].id;
-            ids[2 * p + 1] = recs[ab[p].y].id;
+            ids[2 * p + 1] = recs[ab[p].y & kSlotMask].id;
         }
     }
     if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
-    if (out) DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
+    if (out) {
+        DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < np; p++) out[10 * p + 3] = std::fmod(out[10 * p + 3], 2.0);   // contact flag
+    }
+    return DEM_OK;
+}
+
+int dem_get_contacts(dem_ctx *c, uint *contacti)
+{
+    if (!c || !contacti) return DEM_ERR_INVALID;
+    std::vector<PRec> recs;
+    int rc = fetch_records(c, recs);
+    if (rc) return rc;
+    std::vector<unsigned> ct(c->n);
+    DEM_HIP(hipMemcpy(ct.data(), c->contacti, sizeof(unsigned) * (size_t)c->n, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < c->n; s++) contacti[recs[s].id] = ct[s];
     return DEM_OK;
 }
 

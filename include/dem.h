@@ -240,12 +240,21 @@ int dem_get_cell_start(dem_ctx *ctx, uint *cs, uint ncells);
  * torque first xyz, torque second xyz, contact flag) of the last force step.  Any pointer
  * may be NULL.  Returns DEM_ERR_CAPACITY if cap < npairs. */
 int dem_get_pairs(dem_ctx *ctx, uint *ids, double *hist, double *out, uint cap);
+/* Last step's per-particle contact counts (contacti) in index order. */
+int dem_get_contacts(dem_ctx *ctx, uint *contacti);
 /* Last step's per-particle totals (pairs + walls + gravity) in index order; needs
  * DEM_OPT_RECORD_FORCES. */
 int dem_get_forces(dem_ctx *ctx, double *force, double *torque, uint *contacti);
 
-/* Device properties of the current device: CU count and peak-bandwidth-relevant info. */
+/* Current device: CU count and gfx architecture name. */
 int dem_device_info(int *cu_count, char *name, int name_len);
+/* Device memory for callers without a HIP or torch binding (e.g. a ctypes or FFI driver of
+ * the drop-in entry points).  dem_memcpy kind: 0 H2H, 1 H2D, 2 D2H, 3 D2D (synchronous). */
+void *dem_dev_alloc(size_t bytes);
+int dem_dev_free(void *ptr);
+int dem_memcpy(void *dst, const void *src, size_t bytes, int kind);
+int dem_memzero(void *dst, size_t bytes);
+int dem_device_sync(void);
 
 /* ----------------------------------------------------------------------------------- */
 /* Host utilities: deck, units, restart file, workload generators                       */

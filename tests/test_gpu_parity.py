@@ -38,8 +38,10 @@ def dense_workload(dem, n=3000, seed=11, box=(14.0, 14.0, 60.0), spacing=0.995, 
         walls = dem.make_walls(dem.WALLS_CYLINDER, (box[0], box[0], box[2]), radius=box[0] / 2)
     else:
         walls = dem.make_walls(dem.WALLS_BOX, box)
+    if poly:
+        spacing = min(spacing, 0.99)
     pos = dem.gen_lattice(walls, 0.498, spacing, 0.01, n, seed)
-    diam = np.ones(n) if not poly else rng.uniform(0.9, 1.0, n)
+    diam = np.ones(n) if not poly else rng.uniform(0.97, 1.03, n)
     rad = diam / 2
     rmass = 8 * rad ** 3
     inert = 0.4 * rmass * rad ** 2
@@ -266,39 +268,31 @@ def test_pair_capacity_grows(dem, orc, gpu_ok):
     assert rel_err(ctx.state()["pdis"], sim.field("pdis")) <= TOL
 
 
-def _dev(torch, a, dtype=None):
-    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
-
-
 def test_dropin_reference_sequence(dem, orc, gpu_ok):
     """The reference-named entry points, called in dempacking.cu:535-739 order on
     caller-owned device buffers, reproduce the oracle (which follows the same layout)."""
     import ctypes
-
-    import torch
     L = dem.lib()
+    D = dem.DeviceArray
     w = dense_workload(dem, n=2000, seed=23, mu_r=0.0)
     inst = w.instances[0]
     n = w.n
     W = 30
     S = n * W
-    f64 = torch.float64
-    i32 = torch.int32
-    pos, pdis, angv = (_dev(torch, getattr(w, k)) for k in ("pos", "pdis", "angv"))
-    rad, rmass, inert = (_dev(torch, getattr(w, k)) for k in ("rad", "rmass", "inert"))
+    f64, u32 = np.float64, np.uint32
+    pos, pdis, angv = (D.from_host(getattr(w, k)) for k in ("pos", "pdis", "angv"))
+    rad, rmass, inert = (D.from_host(getattr(w, k)) for k in ("rad", "rmass", "inert"))
     C = inst.gsize[0] * inst.gsize[1] * inst.gsize[2]
-    Z = lambda cnt, dt: torch.zeros(cnt, dtype=dt, device="cuda")
-    hash0, hash1 = Z(2 * n, i32), Z(2 * n, i32)
-    cstart = Z(C, i32)
-    anei, aref = Z(S, i32), Z(S, i32)
-    njgi, njli, sjgi, oldsjgi = Z(n, i32), Z(n, i32), Z(n, i32), Z(n, i32)
-    lp, ln, oldln, cpair = Z(S, i32), Z(S, i32), Z(S, i32), Z(S, i32)
-    dispt, olddispt, fp, tpi, tpj = Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64), Z(3 * S, f64)
-    fricp, oldfricp = Z(S, f64), Z(S, f64)
-    disptw, fricpw = Z(3 * n, f64), Z(n, f64)
-    forcei, torquei, qi = Z(3 * n, f64), Z(3 * n, f64), Z(3 * n, f64)
-    contacti = Z(n, i32)
-    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    hash0, hash1 = D(2 * n, u32), D(2 * n, u32)
+    cstart = D(C, u32)
+    anei, aref = D(S, u32), D(S, u32)
+    njgi, njli, sjgi, oldsjgi = D(n, u32), D(n, u32), D(n, u32), D(n, u32)
+    lp, ln, oldln, cpair = D(S, u32), D(S, u32), D(S, u32), D(S, u32)
+    dispt, olddispt, fp, tpi, tpj = (D(3 * S, f64) for _ in range(5))
+    fricp, oldfricp = D(S, f64), D(S, f64)
+    disptw, fricpw = D(3 * n, f64), D(n, f64)
+    forcei, torquei, qi = D(3 * n, f64), D(3 * n, f64), D(3 * n, f64)
+    contacti = D(n, u32)
     G = (ctypes.c_uint * 3)(*inst.gsize)
     cell = (ctypes.c_double * 3)(*inst.cell)
     orig = (ctypes.c_double * 3)(*inst.origin)
@@ -312,52 +306,53 @@ def test_dropin_reference_sequence(dem, orc, gpu_ok):
     sim = orc.OracleSim(inst, w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=W)
     nsteps = 25
     for it in range(nsteps):
-        forcei.zero_(); torquei.zero_(); contacti.zero_()
+        forcei.zero(); torquei.zero(); contacti.zero()
         if iall.value:
-            qi.zero_()
-            L.calcHash(p(pos), p(hash0), G, cell, orig, n)
-            L.RadixSort(p(hash0), p(hash1), n, 32)
-            L.findCellStart(p(hash0), p(cstart), n, C)
-            L.neighborarray(p(pos), p(hash0), p(cstart), G, cell, orig, W, n, p(anei), p(njgi), p(njli))
-            L.prefixsum(n, p(njgi), p(sjgi), p(oldsjgi), ctypes.byref(P))
-            L.pairlist(p(sjgi), p(oldsjgi), p(oldln), p(anei), p(njgi), p(njli), p(olddispt),
-                       p(oldfricp), W, W, n, p(aref), p(lp), p(ln), p(dispt), p(fricp))
+            qi.zero()
+            L.calcHash(pos.p, hash0.p, G, cell, orig, n)
+            L.RadixSort(hash0.p, hash1.p, n, 32)
+            L.findCellStart(hash0.p, cstart.p, n, C)
+            if it == 0:
+                sim_h = orc.OracleSim(inst, w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=W)
+                assert sim_h.rebuild() == 0
+                assert np.array_equal(hash0.host().reshape(n, 2), sim_h.field("sorted"))
+                assert np.array_equal(cstart.host(), sim_h.field("cell_start"))
+            L.neighborarray(pos.p, hash0.p, cstart.p, G, cell, orig, W, n, anei.p, njgi.p, njli.p)
+            L.prefixsum(n, njgi.p, sjgi.p, oldsjgi.p, ctypes.byref(P))
+            L.pairlist(sjgi.p, oldsjgi.p, oldln.p, anei.p, njgi.p, njli.p, olddispt.p,
+                       oldfricp.p, W, W, n, aref.p, lp.p, ln.p, dispt.p, fricp.p)
             iall.value = 0
-        L.calculateforcepair(p(pos), p(pdis), p(angv), p(rad), p(rmass), p(lp), p(ln), p(oldln),
-                             p(dispt), p(fricp), p(olddispt), p(oldfricp), inst.dt, p(fp), p(tpi),
-                             p(tpj), p(cpair), P.value)
-        L.sumallforcei(W, n, p(njgi), p(njli), p(fp), p(tpi), p(tpj), p(cpair), p(aref),
-                       p(forcei), p(torquei), p(contacti))
-        L.calculateforceiw(ws, p(pos), p(pdis), p(angv), p(rad), p(rmass), inst.dt, P.value, n,
-                           p(disptw), p(fricpw), p(forcei), p(torquei))
-        L.updateposition(ws, n, p(pos), p(pdis), p(angv), p(rmass), p(inert), p(forcei),
-                         p(torquei), p(qi), inst.dt, 0.3, ctypes.byref(iall), ctypes.byref(maxhz))
+        L.calculateforcepair(pos.p, pdis.p, angv.p, rad.p, rmass.p, lp.p, ln.p, oldln.p,
+                             dispt.p, fricp.p, olddispt.p, oldfricp.p, inst.dt, fp.p, tpi.p,
+                             tpj.p, cpair.p, P.value)
+        L.sumallforcei(W, n, njgi.p, njli.p, fp.p, tpi.p, tpj.p, cpair.p, aref.p,
+                       forcei.p, torquei.p, contacti.p)
+        L.calculateforceiw(ws, pos.p, pdis.p, angv.p, rad.p, rmass.p, inst.dt, P.value, n,
+                           disptw.p, fricpw.p, forcei.p, torquei.p)
+        L.updateposition(ws, n, pos.p, pdis.p, angv.p, rmass.p, inert.p, forcei.p,
+                         torquei.p, qi.p, inst.dt, 0.3, ctypes.byref(iall), ctypes.byref(maxhz))
         assert L.dem_last_error() == 0
-        if it == 0:
-            hh = hash0.cpu().numpy().view(np.uint32).reshape(n, 2)
         assert sim.step(1) == 0
         assert P.value == sim.scalars()["npairs"]
         assert iall.value == sim.scalars()["iall"]
-    torch.cuda.synchronize()
-    L.outputporositycn(n, p(contacti), maxhz.value, ws, ctypes.byref(pcn))
+    L.outputporositycn(n, contacti.p, maxhz.value, ws, ctypes.byref(pcn))
     assert pcn.value == sim.scalars()["pcntot"]
     assert maxhz.value == sim.scalars()["maxhz"]
     Pn = P.value
-    u32 = lambda t: t.cpu().numpy().view(np.uint32)
-    assert np.array_equal(u32(lp)[:Pn], sim.field("lp")[:Pn])
-    assert np.array_equal(u32(ln)[:Pn], sim.field("ln")[:Pn])
-    assert np.array_equal(u32(aref).reshape(n, W), sim.field("aref").reshape(n, W))
-    assert np.array_equal(u32(contacti), sim.field("contacti"))
+    assert np.array_equal(lp.host()[:Pn], sim.field("lp")[:Pn])
+    assert np.array_equal(ln.host()[:Pn], sim.field("ln")[:Pn])
+    assert np.array_equal(aref.host().reshape(n, W), sim.field("aref").reshape(n, W))
+    assert np.array_equal(contacti.host(), sim.field("contacti"))
     for t, k in ((pos, "pos"), (pdis, "pdis"), (angv, "angv")):
-        g = t.cpu().numpy().reshape(n, 3)
+        g = t.host().reshape(n, 3)
         ref = sim.field(k)
         if k == "pos":
             assert rel_err(g - w.pos, ref - w.pos) <= TOL
         else:
             assert rel_err(g, ref) <= TOL, k
-    assert rel_err(fp.cpu().numpy()[:3 * Pn], sim.field("forcepair")[:Pn].ravel()) <= TOL
-    assert rel_err(dispt.cpu().numpy()[:3 * Pn], sim.field("dispt")[:Pn].ravel()) <= TOL
-    assert hh.shape == (n, 2)
+    assert rel_err(fp.host()[:3 * Pn], sim.field("forcepair")[:Pn].ravel()) <= TOL
+    assert rel_err(dispt.host()[:3 * Pn], sim.field("dispt")[:Pn].ravel()) <= TOL
+    assert int(cpair.host()[:Pn].sum()) > n, "drop-in workload must be in contact"
 
 
 def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
@@ -382,3 +377,27 @@ def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
     # internal forces cancel: total force = gravity + walls
     _, _, out = ctx.pairs()
     assert np.all(np.isfinite(out))
+
+
+def test_full_pour_packing_and_coordination(dem, orc, gpu_ok):
+    """Pour of the 2,000-sphere configuration (jittered lattice into a 20d x 20d box, rebuild
+    every 20 steps) run until settled on the GPU (hipGraph path) and on the oracle: packing
+    fraction and mean coordination number agree within 0.5%."""
+    w = dem.config0()
+    K = w.rebuild_interval
+    nsteps = 80000
+    ctx = dem.Context(w.instances)
+    ctx.load(w)
+    ctx.step_graph(nsteps)
+    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+    say(f"[pour] gpu done {ctx.last_ms():.0f} ms; oracle running {nsteps} steps")
+    oracle_steps(sim, nsteps, K, 0)
+    st = ctx.state()
+    walls = w.instances[0].walls
+    g = dem.packing_stats(st["pos"], w.rad, ctx.contacts(), walls)
+    o = dem.packing_stats(sim.field("pos"), w.rad, sim.field("contacti"), walls)
+    say(f"[pour] gpu {g}\n[pour] oracle {o}")
+    assert o["cn_inner"] > 3.0 and o["phi_inner"] > 0.5, "bed must be packed"
+    for k in ("phi_ref", "cn_ref", "phi_inner", "cn_inner"):
+        assert g[k] == pytest.approx(o[k], rel=5e-3), k
+    say(f"[pour] max |dpos| = {np.max(np.abs(st['pos'] - sim.field('pos'))):.3e}")
