@@ -124,7 +124,7 @@ EXPORTS = [
     "calculateforcepair", "sumallforcei", "calculateforceiw", "updateposition",
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
     "dem_get_state", "dem_get_wall_history", "dem_step", "dem_step_graph",
-    "dem_request_rebuild", "dem_rebuild", "dem_get_stats", "dem_get_instance_stats",
+    "dem_request_rebuild", "dem_rebuild", "dem_set_active", "dem_active_count", "dem_get_stats", "dem_get_instance_stats",
     "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
     "dem_get_pairs", "dem_get_contacts", "dem_get_forces", "dem_device_info", "dem_read_deck",
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
@@ -156,6 +156,8 @@ def lib() -> ctypes.CDLL:
         "dem_step_graph": ([P, c_int], c_int),
         "dem_request_rebuild": ([P], c_int),
         "dem_rebuild": ([P], c_int),
+        "dem_set_active": ([P, c_uint], c_int),
+        "dem_active_count": ([P], c_uint),
         "dem_get_stats": ([P, P], c_int),
         "dem_get_instance_stats": ([P, P, P], c_int),
         "dem_sync": ([P], c_int),
@@ -492,6 +494,9 @@ class Context:
 
     def rebuild(self):
         _check(self._L.dem_rebuild(self._h), "dem_rebuild")
+
+    def set_active(self, n: int):
+        _check(self._L.dem_set_active(self._h, int(n)), "dem_set_active")
 
     def request_rebuild(self):
         _check(self._L.dem_request_rebuild(self._h), "dem_request_rebuild")

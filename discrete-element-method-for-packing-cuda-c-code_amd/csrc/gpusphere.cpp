@@ -1,0 +1,223 @@
+// gpusphere.cpp -- DEM packing driver: the reference's main() flow (dempacking.cu:28-918) on
+// top of libdem.so's context API.
+//
+//   gpusphere [hopp3d.inp] [--tstop N] [--quiet]
+//
+// Reads the positional deck, converts to reduced units, builds the layer lattice with the
+// reference's glibc rand() velocities, inserts one layer every tnewpar = 0.5 time units,
+// steps with the displacement-triggered list rebuild, and writes view.dat, ppor.dat (every
+// 1000 steps), timesteps.dat, particle.dat (step 1 and every 50000), jobfinished and
+// restart.dat in the reference's formats.  State stays in HBM; no per-layer reallocation.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "../../include/dem.h"
+
+namespace {
+
+int fail(const char *what, int rc)
+{
+    std::fprintf(stderr, "gpusphere: %s failed: %s\n", what, dem_status_string(rc));
+    return 1;
+}
+
+void write_particles(const char *path, int zone, const std::vector<double> &pos,
+                     const std::vector<double> &rad, unsigned n, double diam)
+{
+    std::FILE *fp = std::fopen(path, "ab+");
+    if (!fp) {
+        std::printf("Cannot open file particle01.dat! %d\n", zone);
+        return;
+    }
+    std::fprintf(fp, "ZONE T=\"ZONE%03d\"\n", zone);   // dempacking.cu:786-790
+    for (unsigned i = 0; i < n; i++)
+        std::fprintf(fp, "%15.6g%15.6g%15.6g%15.6g%8d\n", pos[3 * i] * diam, pos[3 * i + 1] * diam,
+                     pos[3 * i + 2] * diam, rad[i] * diam, (int)i);
+    std::fclose(fp);
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const char *deck_path = "hopp3d.inp";
+    long long tstop_override = -1;
+    bool quiet = false;
+    for (int a = 1; a < argc; a++) {
+        if (!std::strcmp(argv[a], "--tstop") && a + 1 < argc) tstop_override = std::atoll(argv[++a]);
+        else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
+        else deck_path = argv[a];
+    }
+    dem_deck d;
+    int rc = dem_read_deck(deck_path, &d);
+    if (rc) {
+        std::printf("Cannot open file hopp3d.inp !");
+        return 1;
+    }
+    // echo (dempacking.cu:128-139)
+    std::printf("%d\n", d.iforcemode);
+    std::printf("%lf %lf %lf\n", d.emod, d.vpois, d.denp);
+    std::printf("%lf\n", d.diam);
+    std::printf("%d\n", d.ntot);
+    std::printf("%lf %u\n", d.dt, d.tstop);
+    std::printf("%lf %lf %lf\n", d.dmt1, d.dmt2, d.thick);
+    std::printf("%lf %lf\n", d.zht1, d.zht2);
+    std::printf("%lf %d\n", d.vel0, d.nrings);
+    std::printf("%lf %lf\n", d.ufricp, d.ufricw);
+    std::printf("%lf %lf %lf %lf %lf\n", d.dampnp, d.damptp, d.dampnw, d.damptw, d.cdamp);
+    std::printf("%lf %lf %lf\n", d.stiffness, d.dampcoeffn, d.dampcoefft);
+    std::printf("%lf %lf\n", d.upp, d.upw);
+    unsigned long long tstop = tstop_override >= 0 ? (unsigned long long)tstop_override : d.tstop;
+
+    dem_material mat;
+    dem_deck_material(&d, &mat);
+    const double diam = d.diam;
+    const double pi = std::atan(1.0) * 4.0;   // dempacking.cu:160
+    const double dt = d.dt;
+    // geometry in units of d (dempacking.cu:213-223)
+    const double thick = d.thick / diam, thick5 = thick * 0.5;
+    const double dmt2 = d.dmt2 / diam, dmt25 = dmt2 * 0.5;
+    const double dmt1 = d.dmt1 / diam;
+    const double hgt1 = d.zht1 / diam, hgt2 = hgt1 + d.zht2 / diam, hopzt = hgt2;
+    (void)dmt1;
+
+    if (std::FILE *fp6 = std::fopen("view.dat", "wb+")) {   // dempacking.cu:227-234
+        std::fprintf(fp6, "number of particles=%4d\n", d.ntot);
+        std::fprintf(fp6, "diameters =%6.3f\n", diam);
+        std::fprintf(fp6, "time step=%f bed width=%6.3f thickniss=%6.3f\n", dt, d.dmt1 / diam, dmt2);
+        std::fprintf(fp6, "glabal damping=%10.4f bed hbot=%10.4f htop=%10.4f\n", d.cdamp, hgt1, hgt2);
+        std::fclose(fp6);
+    }
+
+    // layer lattice and initial velocities (dempacking.cu:242-316)
+    const unsigned ntot = (unsigned)d.ntot;
+    std::vector<double> xp, yp;
+    long nplayer;
+    double nrings = (unsigned)(dmt2) - 3;
+    double nringsy = (unsigned)(thick) - 3;
+    if (dmt25 > 1.99) {
+        double dringsx = dmt2 / (float)(nrings);
+        std::vector<double> rring((size_t)nrings), rringy((size_t)nringsy);
+        for (long ir = 0; ir < (long)nrings; ir++) rring[ir] = ((float)(ir) + 0.5) * dringsx - 0.5 * dmt2;
+        double dringsy = thick / (float)(nringsy);
+        for (long ir = 0; ir < (long)nringsy; ir++) rringy[ir] = ((float)(ir) + 0.5) * dringsy - 0.5 * thick;
+        for (long ir = 0; ir < (long)nrings; ir++)
+            for (long ij = 0; ij < (long)nringsy; ij++) {
+                xp.push_back(rring[ir]);
+                yp.push_back(rringy[ij]);
+            }
+        nplayer = (long)xp.size();
+    } else {
+        xp.push_back(0.0);
+        yp.push_back(0.0);
+        nplayer = 1;
+    }
+    std::vector<double> pos(3 * (size_t)ntot, 0.0), pdis(3 * (size_t)ntot, 0.0), angv(3 * (size_t)ntot, 0.0);
+    std::vector<double> rad(ntot), rmass(ntot), inert(ntot);
+    {
+        long nlayers = (long)(ntot / nplayer) + 1;
+        double zp0 = hopzt - 5.0;
+        unsigned i = 0;
+        for (long ilay = 0; ilay < nlayers; ilay++)
+            for (long ip = 0; ip < nplayer; ip++) {
+                if (i == ntot) break;
+                pos[3 * i] = xp[ip];
+                pos[3 * i + 1] = yp[ip];
+                pos[3 * i + 2] = zp0;
+                double rno = std::rand() / (double)(RAND_MAX);
+                double ang = pi * (rno * .33333 + .33333);
+                pdis[3 * i + 2] = -d.vel0 * std::sin(ang) * dt;
+                double dxy = -d.vel0 * std::cos(ang) * dt;
+                rno = std::rand() / (double)(RAND_MAX);
+                ang = 2.0 * pi * rno;
+                pdis[3 * i] = dxy * std::cos(ang);
+                pdis[3 * i + 1] = dxy * std::sin(ang);
+                rad[i] = 0.5;
+                rmass[i] = 8 * rad[i] * rad[i] * rad[i];
+                inert[i] = 0.4 * rmass[i] * (rad[i] * rad[i]);
+                i++;
+            }
+    }
+
+    // grid (dempacking.cu:329-344)
+    dem_instance inst;
+    std::memset(&inst, 0, sizeof(inst));
+    inst.n = ntot;
+    double cell = 1.55;
+    double world[3] = {dmt2, thick, hopzt};
+    inst.origin[0] = -dmt25;
+    inst.origin[1] = -thick5;
+    inst.origin[2] = 0.0;
+    for (int k = 0; k < 3; k++) {
+        inst.cell[k] = cell;
+        inst.gsize[k] = (uint)((int)(world[k] / cell) + 1);
+    }
+    inst.walls.kind = DEM_WALLS_BOX;
+    inst.walls.size[0] = world[0];
+    inst.walls.size[1] = world[1];
+    inst.walls.size[2] = world[2];
+    inst.mat = mat;
+    inst.dt = dt;
+    inst.skin = 1.55 - 1.0;   // cub - cua (neiglist.cu:27-28)
+
+    dem_ctx *ctx = nullptr;
+    if ((rc = dem_create(&ctx, 1, &inst, 0))) return fail("dem_create", rc);
+    if ((rc = dem_set_state(ctx, pos.data(), pdis.data(), angv.data(), rad.data(), rmass.data(),
+                            inert.data(), 0)))
+        return fail("dem_set_state", rc);
+
+    unsigned nparticles = 0;
+    int nparticledat = 0;
+    const double tnewpar = 0.50;
+    std::clock_t start = std::clock();
+    for (unsigned long long it = 1; it <= tstop; it++) {
+        double time = dt * (double)it;
+        if (nparticles < ntot && time > (float)nparticles * tnewpar / float(nplayer)) {   // :378-381
+            unsigned nn = nparticles + (unsigned)nplayer;
+            if (nn > ntot) nn = ntot;
+            if ((rc = dem_set_active(ctx, nn))) return fail("dem_set_active", rc);
+            nparticles = nn;
+        }
+        if ((rc = dem_step(ctx, 1))) return fail("dem_step", rc);
+        if (it % 1000 == 0) {   // dempacking.cu:758-777
+            dem_stats st;
+            if ((rc = dem_get_stats(ctx, &st))) return fail("dem_get_stats", rc);
+            double avercn = st.pcntot / (nparticles * 1.0 + 1.0e-30);
+            double xvolumespace = world[0] * world[1] * st.maxhz;
+            double porosity5 = 1.0 - nparticles * pi / 6 / xvolumespace;
+            double bedaver = 1.0 - porosity5;
+            if (std::FILE *fp = std::fopen("ppor.dat", "ab+")) {
+                std::fprintf(fp, "%8d %8d %8d %15.6f %15.6f %15.6f %15.6f\n", (int)it, (int)nparticles,
+                             (int)(st.pcntot / 2), porosity5, bedaver, avercn, st.maxhz);
+                std::fclose(fp);
+            }
+            if (!quiet)
+                std::printf("%8d %8d %8d %15.6f %15.6f %15.6f %15.6f\n", (int)it, (int)nparticles,
+                            (int)(st.pcntot / 2), porosity5, bedaver, avercn, st.maxhz);
+            double tret = (double)(std::clock() - start) / CLOCKS_PER_SEC;   // :796-804
+            if (std::FILE *fp = std::fopen("timesteps.dat", "ab+")) {
+                std::fprintf(fp, "%15.6f %15.6f\n", tret, it * 1.0);
+                std::fclose(fp);
+            }
+        }
+        if (it % 50000 == 0 || it == 1) {   // dempacking.cu:779-792
+            if ((rc = dem_get_state(ctx, pos.data(), nullptr, nullptr, nullptr, 0)))
+                return fail("dem_get_state", rc);
+            nparticledat++;
+            write_particles("particle.dat", nparticledat, pos, rad, nparticles, diam);
+        }
+    }
+    if (std::FILE *fp = std::fopen("jobfinished", "wb")) {   // dempacking.cu:904-907
+        std::fprintf(fp, "job finished\n");
+        std::fclose(fp);
+    }
+    if ((rc = dem_write_restart("restart.dat"))) return fail("dem_write_restart", rc);
+    std::printf("program comes to end!\n");
+    dem_destroy(ctx);
+    return 0;
+}

@@ -165,7 +165,8 @@ struct dem_ctx {
     bool batched = false;
     std::vector<InstDev> hinst;
     InstDev *dinst = nullptr;
-    unsigned n = 0, ncells = 0;
+    unsigned n = 0, ncells = 0;   // n = active particles (<= cap_n, single instance may grow)
+    unsigned cap_n = 0;           // allocated particle slots
     int key_bits = 1;
     int cus = 256;
 
@@ -574,9 +575,10 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
         p_off += d.n;
     }
     c->n = p_off;
+    c->cap_n = p_off;
     c->ncells = cell_off;
     c->key_bits = key_bits_for(c->ncells);
-    unsigned n = c->n;
+    unsigned n = c->cap_n;
     unsigned pc = pair_capacity ? pair_capacity : (unsigned)std::min<unsigned long long>(0xfffffff0ull, 12ull * n + 1024);
 
     DEM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -634,7 +636,7 @@ int dem_set_option(dem_ctx *c, int option, double value)
     switch (option) {
     case DEM_OPT_RECORD_FORCES:
         c->record_forces = value != 0.0;
-        if (c->record_forces && !c->fdbg) DEM_HIP(hipMalloc(&c->fdbg, sizeof(double) * 6 * (size_t)c->n));
+        if (c->record_forces && !c->fdbg) DEM_HIP(hipMalloc(&c->fdbg, sizeof(double) * 6 * (size_t)c->cap_n));
         drop_graph(c);
         return DEM_OK;
     case DEM_OPT_REBUILD_INTERVAL:
@@ -650,7 +652,7 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
                   const double *rad, const double *rmass, const double *inert, int on_device)
 {
     if (!c || !pos || !pdis || !angv || !rad || !rmass || !inert) return DEM_ERR_INVALID;
-    size_t n = c->n;
+    size_t n = c->cap_n;
     std::vector<double> hp, hv, hw, hr, hm, hi;
     if (on_device) {
         hp.resize(3 * n); hv.resize(3 * n); hw.resize(3 * n); hr.resize(n); hm.resize(n); hi.resize(n);
@@ -679,9 +681,13 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
         }
     }
     DEM_HIP(hipStreamSynchronize(c->stream));
-    DEM_HIP(hipMemcpy(c->rec[c->rc], recs.data(), sizeof(PRec) * n, hipMemcpyHostToDevice));
-    DEM_HIP(hipMemset(c->whist[c->rc], 0, sizeof(Hist) * n));
-    DEM_HIP(hipMemcpy(c->inv[c->rc], inv.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    // both buffers: particles not yet active (dem_set_active) keep slot = index in either
+    for (int k = 0; k < 2; k++) {
+        DEM_HIP(hipMemcpy(c->rec[k], recs.data(), sizeof(PRec) * n, hipMemcpyHostToDevice));
+        DEM_HIP(hipMemset(c->whist[k], 0, sizeof(Hist) * n));
+        DEM_HIP(hipMemcpy(c->inv[k], inv.data(), sizeof(unsigned) * n, hipMemcpyHostToDevice));
+    }
+    DEM_HIP(hipMemset(c->contacti, 0, sizeof(unsigned) * n));
     c->have_list = false;
     c->need_rebuild = true;
     c->npairs = 0;
@@ -691,11 +697,28 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
 
 static int fetch_records(dem_ctx *c, std::vector<PRec> &recs)
 {
-    recs.resize(c->n);
+    recs.resize(c->cap_n);
     DEM_HIP(hipStreamSynchronize(c->stream));
-    DEM_HIP(hipMemcpy(recs.data(), c->rec[c->rc], sizeof(PRec) * (size_t)c->n, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(recs.data(), c->rec[c->rc], sizeof(PRec) * (size_t)c->cap_n, hipMemcpyDeviceToHost));
     return DEM_OK;
 }
+
+// Activate particles [n, n_new) of a single-instance context (layer insertion of the pour,
+// dempacking.cu:378-462): they join at the next step, which rebuilds the lists.
+int dem_set_active(dem_ctx *c, uint n_new)
+{
+    if (!c || c->ninst != 1 || n_new < c->n || n_new > c->cap_n || n_new == 0) return DEM_ERR_INVALID;
+    if (n_new == c->n) return DEM_OK;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    c->n = n_new;
+    unsigned ib[2] = {0u, n_new};
+    DEM_HIP(hipMemcpy(c->inst_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
+    c->need_rebuild = true;
+    drop_graph(c);
+    return DEM_OK;
+}
+
+uint dem_active_count(dem_ctx *c) { return c ? c->n : 0u; }
 
 int dem_get_state(dem_ctx *c, double *pos, double *pdis, double *angv, double *qi, int on_device)
 {
@@ -703,7 +726,7 @@ int dem_get_state(dem_ctx *c, double *pos, double *pdis, double *angv, double *q
     std::vector<PRec> recs;
     int rc = fetch_records(c, recs);
     if (rc) return rc;
-    size_t n = c->n;
+    size_t n = c->cap_n;
     std::vector<double> p(3 * n), v(3 * n), w(3 * n), q(3 * n);
     for (size_t s = 0; s < n; s++) {
         const PRec &r = recs[s];
@@ -727,7 +750,7 @@ int dem_get_wall_history(dem_ctx *c, double *disptw, double *fricpw)
     std::vector<PRec> recs;
     int rc = fetch_records(c, recs);
     if (rc) return rc;
-    size_t n = c->n;
+    size_t n = c->cap_n;
     std::vector<Hist> wh(n);
     DEM_HIP(hipMemcpy(wh.data(), c->whist[c->rc], sizeof(Hist) * n, hipMemcpyDeviceToHost));
     for (size_t s = 0; s < n; s++) {
@@ -919,8 +942,8 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
         t_all += a + b;
     }
     (void)e3;
-    hipEventDestroy(e0); hipEventDestroy(e1); hipEventDestroy(e2); hipEventDestroy(e3);
-    hipEventDestroy(r0); hipEventDestroy(r1);
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+    (void)hipEventDestroy(r0); (void)hipEventDestroy(r1);
     out_ms[0] = t_pair / nsteps;
     out_ms[1] = t_part / nsteps;
     out_ms[2] = nreb ? t_reb / nreb : 0.0;
@@ -1048,9 +1071,9 @@ int dem_get_contacts(dem_ctx *c, uint *contacti)
     std::vector<PRec> recs;
     int rc = fetch_records(c, recs);
     if (rc) return rc;
-    std::vector<unsigned> ct(c->n);
-    DEM_HIP(hipMemcpy(ct.data(), c->contacti, sizeof(unsigned) * (size_t)c->n, hipMemcpyDeviceToHost));
-    for (size_t s = 0; s < c->n; s++) contacti[recs[s].id] = ct[s];
+    std::vector<unsigned> ct(c->cap_n);
+    DEM_HIP(hipMemcpy(ct.data(), c->contacti, sizeof(unsigned) * (size_t)c->cap_n, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < c->cap_n; s++) contacti[recs[s].id] = s < c->n ? ct[s] : 0u;
     return DEM_OK;
 }
 
@@ -1060,18 +1083,19 @@ int dem_get_forces(dem_ctx *c, double *force, double *torque, uint *contacti)
     std::vector<PRec> recs;
     int rc = fetch_records(c, recs);
     if (rc) return rc;
-    size_t n = c->n;
+    size_t n = c->cap_n;
     std::vector<double> f(6 * n);
     std::vector<unsigned> ct(n);
     DEM_HIP(hipMemcpy(f.data(), c->fdbg, sizeof(double) * 6 * n, hipMemcpyDeviceToHost));
     DEM_HIP(hipMemcpy(ct.data(), c->contacti, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
     for (size_t s = 0; s < n; s++) {
         size_t i = recs[s].id;
+        bool act = s < c->n;
         for (int k = 0; k < 3; k++) {
-            if (force) force[3 * i + k] = f[6 * s + k];
-            if (torque) torque[3 * i + k] = f[6 * s + 3 + k];
+            if (force) force[3 * i + k] = act ? f[6 * s + k] : 0.0;
+            if (torque) torque[3 * i + k] = act ? f[6 * s + 3 + k] : 0.0;
         }
-        if (contacti) contacti[i] = ct[s];
+        if (contacti) contacti[i] = act ? ct[s] : 0u;
     }
     return DEM_OK;
 }

@@ -772,6 +772,10 @@ void orc_state_scalars(orc_state *st, double *out7)
 
 void orc_state_set_iall(orc_state *st, uint v) { st->iall = v; }
 
+/* Grow the active particle count (layer insertion, dempacking.cu:459-461): the arrays were
+ * sized for the final count at create; the new particles' state must already be in place. */
+void orc_state_set_n(orc_state *st, int n) { st->n = n; st->iall = 1; }
+
 /* Run `steps` steps; returns 0 or the first nonzero orc_step code. */
 int orc_run(const orc_setup *s, orc_state *st, int steps)
 {

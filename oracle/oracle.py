@@ -79,6 +79,7 @@ def lib():
         "orc_state_field": ([P, c_int], P),
         "orc_state_scalars": ([P, P], None),
         "orc_state_set_iall": ([P, c_uint], None),
+        "orc_state_set_n": ([P, c_int], None),
         "orc_brute_pairs": ([c_int, P, P, c_double, P, ctypes.c_longlong], ctypes.c_longlong),
     }
     for name, (args, res) in sig.items():
@@ -184,6 +185,10 @@ class OracleSim:
 
     def set_iall(self, v: int):
         self.L.orc_state_set_iall(self.st, v)
+
+    def set_n(self, n: int):
+        """Activate particles [0, n) (layer insertion); forces a rebuild at the next step."""
+        self.L.orc_state_set_n(self.st, int(n))
 
     def rebuild(self) -> int:
         return self.L.orc_rebuild(ctypes.byref(self.setup), self.st)

@@ -401,3 +401,73 @@ def test_full_pour_packing_and_coordination(dem, orc, gpu_ok):
     for k in ("phi_ref", "cn_ref", "phi_inner", "cn_inner"):
         assert g[k] == pytest.approx(o[k], rel=5e-3), k
     say(f"[pour] max |dpos| = {np.max(np.abs(st['pos'] - sim.field('pos'))):.3e}")
+
+
+def test_layer_insertion_parity(dem, orc, gpu_ok):
+    """Layer-by-layer pour (dempacking.cu:378-462): particles join in index order and force a
+    rebuild; history of existing pairs is kept.  GPU (dem_set_active) vs oracle."""
+    walls = dem.make_walls(dem.WALLS_BOX, (10.0, 10.0, 40.0))
+    per = 64
+    nl = 5
+    n = per * nl
+    rng = np.random.default_rng(7)
+    layer = np.array([[-4.2 + 1.2 * i, -4.2 + 1.2 * j] for i in range(8) for j in range(8)])
+    pos = np.zeros((n, 3))
+    for L in range(nl):
+        pos[L * per:(L + 1) * per, :2] = layer + rng.uniform(-0.05, 0.05, (per, 2))
+        pos[L * per:(L + 1) * per, 2] = 8.0
+    pdis = np.zeros((n, 3))
+    pdis[:, 2] = -2e-3
+    rad = np.full(n, 0.5)
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    mat = dem.reduced_material(E=1e6, nu=0.29, rho=2500.0, diam=1e-3, mu=0.4, mu_w=0.4, e=0.5,
+                               mu_r=0.002)
+    inst = dem.make_instance(n, walls, 40.0, 1.55, mat, 1e-3)
+    ctx = dem.Context([inst])
+    ctx.set_state(pos, pdis, np.zeros((n, 3)), rad, rmass, inert)
+    sim = orc.OracleSim(inst, pos, pdis, np.zeros((n, 3)), rad, rmass, inert)
+    active = 0
+    for it in range(1, 1601):
+        if active < n and it % 300 == 1:
+            active = min(n, active + per)
+            ctx.set_active(active)
+            sim.set_n(active)
+        ctx.step(1)
+        assert sim.step(1) == 0
+    assert ctx.stats().rebuilds >= nl
+    st = ctx.state()
+    assert np.array_equal(ctx.contacts(), sim.field("contacti"))
+    assert sim.field("contacti").sum() > n
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+    assert rel_err(st["pos"] - pos, sim.field("pos") - pos) <= TOL
+
+
+def test_gpusphere_driver_on_shipped_deck(dem, gpu_ok, tmp_path):
+    """The gpusphere driver on the reference's own hopp3d.inp (first 2,500 steps): insertion
+    schedule, ppor.dat / particle.dat formats, restart.dat bytes."""
+    import os
+    import re
+    import shutil
+    import subprocess
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    shutil.copy(os.path.join(golden, "hopp3d.inp"), tmp_path / "hopp3d.inp")
+    exe = os.path.join(dem.HERE, "gpusphere")
+    r = subprocess.run([exe, "--tstop", "2500", "--quiet"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    assert "program comes to end!" in r.stdout
+    lines = (tmp_path / "ppor.dat").read_text().splitlines()
+    assert len(lines) == 2
+    pat = re.compile(r"^\s{0,7}\d{1,8}\s{1,8}\d{1,8}\s{1,8}\d{1,8}(\s+-?\d+\.\d{6}){4}$")
+    for ln in lines:
+        assert pat.match(ln), ln
+    it, n = zip(*[(int(ln.split()[0]), int(ln.split()[1])) for ln in lines])
+    # layers of 47 x 47 = 2209 enter at steps 1, 501, 1001, 1501 (time > N * 0.5 / nplayer)
+    assert it == (1000, 2000) and n == (4418, 8836)
+    assert (tmp_path / "restart.dat").read_bytes() == open(os.path.join(golden, "restart.dat"), "rb").read()
+    assert (tmp_path / "jobfinished").read_text() == "job finished\n"
+    pd = (tmp_path / "particle.dat").read_text().splitlines()
+    assert pd[0] == 'ZONE T="ZONE001"' and len(pd) == 1 + 2209
+    assert (tmp_path / "view.dat").read_text().startswith("number of particles=10000\n")
