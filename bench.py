@@ -67,6 +67,53 @@ def cpu_baseline(dem, w, budget_s):
                       f"step (list build {t_first:.1f} s, excluded), oracle C/OpenMP"}
 
 
+def bench_batched(dem, steps, interval, log_every=64):
+    """Batched form (256 instances, hipGraph-captured) against the same instances stepped as
+    per-instance plain contexts (host loop, reference trigger off, same rebuild interval)."""
+    t0 = time.perf_counter()
+    w = dem.config3(rebuild_interval=interval)
+    log(f"batched workload: {len(w.instances)} instances, n={w.n} ({time.perf_counter() - t0:.1f} s)")
+    big = dem.Context(w.instances)
+    big.load(w)
+    big.step_graph(2 * interval)
+    big.sync()
+    t0 = time.perf_counter()
+    big.step_graph(steps)
+    big.sync()
+    tb = time.perf_counter() - t0
+    nb = w.n
+    del big
+    # per-instance plain launches: one context per instance, run back to back
+    offs = [0]
+    for inst in w.instances:
+        offs.append(offs[-1] + inst.n)
+    ctxs = []
+    for k, inst in enumerate(w.instances):
+        c = dem.Context([inst])
+        sl = slice(offs[k], offs[k + 1])
+        c.set_state(w.pos[sl], w.pdis[sl], w.angv[sl], w.rad[sl], w.rmass[sl], w.inert[sl])
+        c.set_option(dem.OPT_REBUILD_INTERVAL, interval)
+        c.step(2 * interval)
+        ctxs.append(c)
+    for c in ctxs:
+        c.sync()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step(steps)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.step_graph(steps)
+    tg = time.perf_counter() - t0
+    vb, vp, vg = nb * steps / tb, nb * steps / tp, nb * steps / tg
+    log(f"batched {vb:.4e} p-steps/s; per-instance plain {vp:.4e} (graph {vg:.4e}); "
+        f"ratio {vb / vp:.2f}x ({vb / vg:.2f}x vs per-instance graphs)")
+    return {"workload": w.name, "instances": len(w.instances), "n": nb, "steps": steps,
+            "rebuild_interval": interval, "value": vb, "per_instance_plain": vp,
+            "per_instance_graph": vg, "ratio_vs_plain": vb / vp, "ratio_vs_plain_graph": vb / vg,
+            "ms_per_step": tb / steps * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +123,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=30)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batched-interval", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
@@ -110,6 +160,10 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": "k_pair_force" if dom == "pair" else "k_particle",
             "bytes_per_launch": kb[dom]}
+    del ctx
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(dem, args.batched_steps, args.batched_interval)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dem, w, args.cpu_seconds)
@@ -133,6 +187,7 @@ def main():
                    "kernel_ms": {k: round(v, 5) for k, v in kt.items() if k.endswith("_ms")}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

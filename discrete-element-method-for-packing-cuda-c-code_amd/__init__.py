@@ -407,20 +407,25 @@ def config3(ninst: int = 256, seed: int = 4, nmin: int = 1000, nmax: int = 64000
     es = rng.uniform(0.3, 0.9, ninst)
     Es = np.exp(rng.uniform(math.log(1e6), math.log(1e8), ninst))
     dt = reduced_dt(DT0, 1e-3)
-    insts, P = [], []
-    for k in range(ninst):
+
+    def one(k):
         n = int(ns[k])
         phi = 0.2
         side = max(6.0, (n * math.pi / 6.0 / phi / 2.0) ** (1.0 / 3.0))   # H = 2 * side
         h = 2.0 * side
         walls = make_walls(WALLS_BOX, (side, side, h))
-        diam = np.ones(n)
-        pos = gen_insert(walls, 0.0, h, diam, seed * 1000 + k)
+        pos = gen_insert(walls, 0.0, h, np.ones(n), seed * 1000 + k)
         mat = reduced_material(E=float(Es[k]), nu=0.29, rho=2500.0, diam=1e-3, mu=float(mus[k]),
                                mu_w=float(mus[k]), e=float(es[k]), mu_r=0.002)
-        insts.append(make_instance(n, walls, h, 1.0 + SKIN, mat, dt))
-        P.append(pos)
-    pos = np.concatenate(P)
+        return make_instance(n, walls, h, 1.0 + SKIN, mat, dt), pos
+
+    # the C generator releases the GIL, so instances are generated in parallel
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as ex:
+        res = list(ex.map(one, range(ninst)))
+    insts = [r[0] for r in res]
+    pos = np.concatenate([r[1] for r in res])
     n = pos.shape[0]
     rad, rmass, inert = _props(np.ones(n))
     z = np.zeros((n, 3))

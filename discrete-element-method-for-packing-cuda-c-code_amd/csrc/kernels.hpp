@@ -17,10 +17,6 @@ namespace dem {
 
 constexpr unsigned kEmpty = 0xffffffffu;
 constexpr int kBlock = 256;
-// The second member of a pair carries, in its top bit, whether the pair's slot order is the
-// reverse of its particle-index order (first member has the larger index).
-constexpr unsigned kFlip = 0x80000000u;
-constexpr unsigned kSlotMask = 0x7fffffffu;
 
 // 128-byte particle record in slot order.  pos/rad, pdis/mass and angv/inert pack the data a
 // pair gathers into the first 96 bytes; q (displacement since the last rebuild), the
@@ -40,7 +36,7 @@ struct __attribute__((aligned(16))) Hist {
 };
 
 // Per-pair force output of the last step (80 bytes): force on first, torque on first,
-// torque on second, and flag = contact (0/1) + 2 * index-order-flipped.
+// torque on second, contact flag (1.0 / 0.0).
 struct __attribute__((aligned(16))) POut {
     double fx, fy, fz, flag;
     double tax, tay, taz, tbx;
@@ -217,8 +213,9 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const uin
     if (a >= n) return;
     const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
     unsigned cg = 0, cl = 0;
+    unsigned ida = rec[a].id;
     for_candidates(rec, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
-        if (b > a) cg++;
+        if (rec[b].id > ida) cg++;
         else cl++;
     });
     ng[a] = cg;
@@ -231,7 +228,7 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
 {
     while (lo < hi) {
         unsigned mid = (lo + hi) >> 1;
-        unsigned v = ab[mid].y & kSlotMask;
+        unsigned v = ab[mid].y;
         if (v == target) return mid;
         if (v < target) lo = mid + 1;
         else hi = mid;
@@ -239,11 +236,13 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
     return kEmpty;
 }
 
-// Pair list of slot a ([off[a], off[a+1]), partners b > a ascending) with the tangential
-// history carried over by pair identity (the reference's pairlistD remap,
-// particleSystem.cu:143-191): the old list is searched under the particles' old slots, and
-// the stored displacement is negated when the pair's member order flipped.  Partners b < a
-// are written to the transposed list for k_transpose.
+// Pair list of slot a: its partners with a larger particle index, in ascending slot order,
+// at [off[a], off[a+1]) -- each pair is stored once, first member = smaller particle index,
+// exactly the reference's Lp < Ln orientation (dempacking.cu:640-644), while segments follow
+// the cell-sorted slot order.  Tangential history is carried over by pair identity (the
+// reference's pairlistD remap, particleSystem.cu:143-191): the old list is searched under
+// the members' old slots.  Partners with a smaller index go to the transposed list, to be
+// resolved to pair indices by k_transpose.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const uint2 *sorted, unsigned n, const InstDev *inst, InstDev one,
@@ -262,26 +261,17 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     unsigned oa = (inv_old && ida < n_old) ? inv_old[ida] : kEmpty;
     bool over = false;
     for_candidates(rec, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
-        if (b > a) {
+        unsigned idb = rec[b].id;
+        if (idb > ida) {
             if (pg >= cap) { over = true; return; }
-            unsigned idb = rec[b].id;
-            ab[pg] = make_uint2(a, b | (ida > idb ? kFlip : 0u));
+            ab[pg] = make_uint2(a, b);
             Hist h = Hist{0.0, 0.0, 0.0, 0.0};
             if (oa != kEmpty) {
                 unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
                 if (ob != kEmpty) {
-                    if (oa < ob) {
-                        unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
-                        unsigned q = find_second(ab_old, lo, hi, ob);
-                        if (q != kEmpty) h = hist_old[q];
-                    } else {
-                        unsigned lo = min(off_old[ob], cap_old), hi = min(off_old[ob + 1], cap_old);
-                        unsigned q = find_second(ab_old, lo, hi, oa);
-                        if (q != kEmpty) {
-                            Hist o = hist_old[q];
-                            h = Hist{-o.hx, -o.hy, -o.hz, o.fr};
-                        }
-                    }
+                    unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
+                    unsigned q = find_second(ab_old, lo, hi, ob);
+                    if (q != kEmpty) h = hist_old[q];
                 }
             }
             hist[pg] = h;
@@ -294,7 +284,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     if (over && ctl) atomicOr(&ctl->overflow, 1u);
 }
 
-// Replace each transposed entry (partner slot b < a) by the index of pair (b, a).
+// Replace each transposed entry (partner slot b, smaller particle index) by the index of
+// pair (b, a) in b's segment.
 __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const unsigned *off,
                                                       const unsigned *toff, unsigned n,
                                                       unsigned cap, unsigned *jt,
@@ -348,8 +339,6 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
     const unsigned np = *npairs;
     for (unsigned p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
         uint2 e = ab[p];
-        double flip = (e.y & kFlip) ? 2.0 : 0.0;
-        e.y &= kSlotMask;
         const double *A = reinterpret_cast<const double *>(rec + e.x);
         const double *B = reinterpret_cast<const double *>(rec + e.y);
         double2 a0 = ld2(A), a1 = ld2(A + 2), a2 = ld2(A + 4), a3 = ld2(A + 6), a4 = ld2(A + 8),
@@ -373,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                           fo);
         double *O = reinterpret_cast<double *>(out + p);
         st2(O, fa.x, fa.y);
-        st2(O + 2, fa.z, (c ? 1.0 : 0.0) + flip);
+        st2(O + 2, fa.z, c ? 1.0 : 0.0);
         st2(O + 4, ta.x, ta.y);
         st2(O + 6, ta.z, tb.x);
         st2(O + 8, tb.y, tb.z);
@@ -402,36 +391,29 @@ __global__ __launch_bounds__(kBlock) void k_particle(
         double *R = reinterpret_cast<double *>(rec + a);
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
+        // Sum in the reference's order (sumallforceiD over an Aref row): the particle's own
+        // pairs (partners with a larger index) in ascending slot order, then the pairs where
+        // it is the second member, also in ascending slot order.  The same terms in the same
+        // order as the oracle, so the totals agree bit for bit.
         unsigned p0 = off[a], p1 = off[a + 1];
         if (p1 > np) p1 = np;
+        for (unsigned p = p0; p < p1; p++) {
+            const double *O = reinterpret_cast<const double *>(out + p);
+            double2 o0 = ld2(O), o1 = ld2(O + 2), o2 = ld2(O + 4), o3 = ld2(O + 6);
+            fx += o0.x; fy += o0.y; fz += o1.x;
+            tx += o2.x; ty += o2.y; tz += o3.x;
+            c += o1.y != 0.0 ? 1u : 0u;
+        }
         unsigned q0 = toff[a], q1 = toff[a + 1];
         if (q1 > np) q1 = np;
-        // Sum in the reference's order (sumallforceiD's Aref rows): partners with a larger
-        // particle index first, then those with a smaller index, each group in ascending slot
-        // order (partners below this slot come from the transposed list, those above from the
-        // own list).  The terms are the force / torque on this particle either way, so the
-        // result equals the oracle's sum bit for bit.
-        for (int pass = 0; pass < 2; pass++) {
-            for (unsigned k = q0; k < q1; k++) {
-                unsigned q = jt[k];
-                if (q >= np) continue;
-                const double *O = reinterpret_cast<const double *>(out + q);
-                double2 o0 = ld2(O), o1 = ld2(O + 2), o3 = ld2(O + 6), o4 = ld2(O + 8);
-                bool larger = o1.y >= 2.0;   // first member (partner) has the larger index
-                if (larger != (pass == 0)) continue;
-                fx -= o0.x; fy -= o0.y; fz -= o1.x;
-                tx += o3.y; ty += o4.x; tz += o4.y;
-                c += (o1.y == 1.0 || o1.y == 3.0) ? 1u : 0u;
-            }
-            for (unsigned p = p0; p < p1; p++) {
-                const double *O = reinterpret_cast<const double *>(out + p);
-                double2 o0 = ld2(O), o1 = ld2(O + 2), o2 = ld2(O + 4), o3 = ld2(O + 6);
-                bool larger = o1.y < 2.0;    // second member (partner) has the larger index
-                if (larger != (pass == 0)) continue;
-                fx += o0.x; fy += o0.y; fz += o1.x;
-                tx += o2.x; ty += o2.y; tz += o3.x;
-                c += (o1.y == 1.0 || o1.y == 3.0) ? 1u : 0u;
-            }
+        for (unsigned k = q0; k < q1; k++) {
+            unsigned q = jt[k];
+            if (q >= np) continue;
+            const double *O = reinterpret_cast<const double *>(out + q);
+            double2 o0 = ld2(O), o1 = ld2(O + 2), o3 = ld2(O + 6), o4 = ld2(O + 8);
+            fx -= o0.x; fy -= o0.y; fz -= o1.x;
+            tx += o3.y; ty += o4.x; tz += o4.y;
+            c += o1.y != 0.0 ? 1u : 0u;
         }
         double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
                 r5 = ld2(R + 10), r6 = ld2(R + 12);

@@ -566,7 +566,7 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
             return DEM_ERR_INVALID;
         }
         unsigned long long cells = (unsigned long long)d.gsize[0] * d.gsize[1] * d.gsize[2];
-        if (cells + cell_off >= 0xffffffffull || (unsigned long long)p_off + d.n >= (unsigned long long)kFlip) {
+        if (cells + cell_off >= 0xffffffffull || (unsigned long long)p_off + d.n >= 0xffffffffull) {
             free_ctx(c);
             return DEM_ERR_INVALID;
         }
@@ -707,7 +707,10 @@ static int fetch_records(dem_ctx *c, std::vector<PRec> &recs)
 // dempacking.cu:378-462): they join at the next step, which rebuilds the lists.
 int dem_set_active(dem_ctx *c, uint n_new)
 {
-    if (!c || c->ninst != 1 || n_new < c->n || n_new > c->cap_n || n_new == 0) return DEM_ERR_INVALID;
+    // before the first list build any prefix may be chosen; afterwards the count only grows
+    // (active particles have been moved to sorted slots)
+    if (!c || c->ninst != 1 || n_new > c->cap_n || n_new == 0) return DEM_ERR_INVALID;
+    if (c->have_list && n_new < c->n) return DEM_ERR_INVALID;
     if (n_new == c->n) return DEM_OK;
     DEM_HIP(hipStreamSynchronize(c->stream));
     c->n = n_new;
@@ -1054,14 +1057,11 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
         DEM_HIP(hipMemcpy(ab.data(), c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToHost));
         for (size_t p = 0; p < np; p++) {
             ids[2 * p] = recs[ab[p].x].id;
-            ids[2 * p + 1] = recs[ab[p].y & kSlotMask].id;
+            ids[2 * p + 1] = recs[ab[p].y].id;
         }
     }
     if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
-    if (out) {
-        DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
-        for (size_t p = 0; p < np; p++) out[10 * p + 3] = std::fmod(out[10 * p + 3], 2.0);   // contact flag
-    }
+    if (out) DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
     return DEM_OK;
 }
 

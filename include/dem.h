@@ -206,9 +206,10 @@ int dem_get_state(dem_ctx *ctx, double *pos, double *pdis, double *angv, double 
 /* Per-particle wall history (disptw AoS double3, fricpw) in index order. */
 int dem_get_wall_history(dem_ctx *ctx, double *disptw, double *fricpw);
 
-/* Single-instance contexts: make particles [0, n) active (n may only grow; the rest of the
- * state set by dem_set_state waits, as the reference's not-yet-inserted layers do,
- * dempacking.cu:378-462).  The next step rebuilds the lists. */
+/* Single-instance contexts: make particles [0, n) active; the rest of the state set by
+ * dem_set_state waits, as the reference's not-yet-inserted layers do (dempacking.cu:378-462).
+ * Any n before the first step after dem_set_state, only growth afterwards.  The next step
+ * rebuilds the lists. */
 int dem_set_active(dem_ctx *ctx, uint n);
 uint dem_active_count(dem_ctx *ctx);
 /* Run `nsteps` steps in the reference order: [rebuild if requested] -> pair forces ->

@@ -417,7 +417,7 @@ def test_layer_insertion_parity(dem, orc, gpu_ok):
         pos[L * per:(L + 1) * per, :2] = layer + rng.uniform(-0.05, 0.05, (per, 2))
         pos[L * per:(L + 1) * per, 2] = 8.0
     pdis = np.zeros((n, 3))
-    pdis[:, 2] = -2e-3
+    pdis[:, 2] = -4e-3
     rad = np.full(n, 0.5)
     rmass = 8 * rad ** 3
     inert = 0.4 * rmass * rad ** 2
@@ -428,8 +428,8 @@ def test_layer_insertion_parity(dem, orc, gpu_ok):
     ctx.set_state(pos, pdis, np.zeros((n, 3)), rad, rmass, inert)
     sim = orc.OracleSim(inst, pos, pdis, np.zeros((n, 3)), rad, rmass, inert)
     active = 0
-    for it in range(1, 1601):
-        if active < n and it % 300 == 1:
+    for it in range(1, 4001):
+        if active < n and it % 400 == 1:
             active = min(n, active + per)
             ctx.set_active(active)
             sim.set_n(active)
