@@ -3,9 +3,9 @@
 // Context path (cell-sorted slots, CSR pair lists):
 //   rebuild : k_hash -> radix sort -> k_fill / k_cell_range -> k_reorder -> k_nbr_count ->
 //             scans -> k_nbr_fill (pair list + history remap) -> k_transpose
-//   step    : k_pair_force (one lane per candidate pair) -> k_particle (per-particle
-//             gather of its pair forces in fixed order, walls, gravity, integration,
-//             rebuild trigger, per-instance reductions)
+//   step    : k_pair_force (one lane per candidate pair), k_walls (one lane per particle on
+//             the rebuild's wall-candidate list) -> k_particle (per-particle gather of its
+//             pair forces in the reference order, gravity, wall force, integration, trigger)
 // Reference-layout path (drop-in entry points, particle-index order, AoS double3):
 //   c_* kernels with the reference's buffers and semantics.
 #pragma once
@@ -63,7 +63,8 @@ struct DevCtl {
     unsigned steps;          // steps completed (incremented by k_publish)
     unsigned npairs;         // written by the pair scan
     unsigned overflow;       // pair capacity exceeded at the last rebuild
-    unsigned pad[4];
+    unsigned nwall;          // wall-candidate count of the last rebuild
+    unsigned pad[3];
 };
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
@@ -88,7 +89,10 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
 {
     if (!is_active(active)) return;
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
-    if (s == 0 && ctl) ctl->need_rebuild = 0u;
+    if (s == 0 && ctl) {
+        ctl->need_rebuild = 0u;
+        ctl->nwall = 0u;
+    }
     if (s >= n) return;
     const PRec &r = rec[s];
     double px = r.px, py = r.py, pz = r.pz;
@@ -121,15 +125,54 @@ __global__ __launch_bounds__(kBlock) void k_cell_range(const uint2 *sorted, unsi
     if (cend && (s == n - 1 || sorted[s + 1].x != c)) cend[c] = s + 1;
 }
 
-// Move every particle into its sorted slot; q restarts at zero (dempacking.cu:541).
+// Largest overlap the particle could reach with any wall after moving by `slack`: a
+// particle with wall_reach(...) <= 0 cannot touch a wall before the next rebuild, because no
+// particle moves more than half the skin between rebuilds (the rebuild trigger).
+__device__ __forceinline__ double wall_reach(const WallGeom &g, double x, double y, double z,
+                                             double r, double slack)
+{
+    double d = z;   // floor
+    if (g.kind == 0) {
+        d = fmin(d, fmin(x + g.hx, g.hx - x));
+        d = fmin(d, fmin(y + g.hy, g.hy - y));
+    } else {
+        d = fmin(d, g.radius - sqrt(x * x + y * y));
+    }
+    return (r + slack) - d;
+}
+
+// Move every particle into its sorted slot; q restarts at zero (dempacking.cu:541).  Also
+// lists the particles that may touch a wall before the next rebuild (for k_walls).
+template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigned n,
                                                     const PRec *rec_old, PRec *rec_new,
                                                     const Hist *wh_old, Hist *wh_new,
                                                     const unsigned *inv_old, unsigned *inv_new,
-                                                    const unsigned *active)
+                                                    const InstDev *inst, InstDev one,
+                                                    unsigned *wflag, unsigned *wlist,
+                                                    DevCtl *ctl, const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    bool cand = false;
+    if (s < n) {
+        unsigned id = sorted[s].y;
+        unsigned o = inv_old[id];
+        const PRec &r = rec_old[o];
+        const InstDev &I = BATCHED ? inst[r.inst] : one;
+        cand = wall_reach(I.walls, r.px, r.py, r.pz, r.rad, 0.5 * I.skin) > 0.0;
+        wflag[s] = cand ? 1u : 0u;
+    }
+    // wave-aggregated append to the wall-candidate list
+    unsigned long long m = __ballot(cand);
+    if (m) {
+        unsigned base = 0;
+        unsigned lane = __lane_id();
+        unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+        if (lane == leader) base = atomicAdd(&ctl->nwall, (unsigned)__popcll(m));
+        base = __shfl(base, (int)leader, 64);
+        if (cand) wlist[base + __popcll(m & ((1ull << lane) - 1ull))] = s;
+    }
     if (s >= n) return;
     unsigned id = sorted[s].y;
     unsigned o = inv_old[id];
@@ -372,6 +415,44 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
     }
 }
 
+// Wall contact and history for the rebuild's wall candidates (calculateforceiwD's wall part,
+// neiglist.cu:286-301).  wforce[slot] = (F xyz, hits, T xyz, 0); history zeroed when no wall
+// is touched.  Grid-stride over the device-resident candidate count.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
+                                                  const unsigned *wlist, const DevCtl *ctl,
+                                                  const InstDev *inst, InstDev one,
+                                                  double *wforce)
+{
+    const unsigned nw = ctl->nwall;
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < nw; i += gridDim.x * kBlock) {
+        unsigned a = wlist[i];
+        const double *R = reinterpret_cast<const double *>(rec + a);
+        double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
+                r5 = ld2(R + 10);
+        const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
+        double *WH = reinterpret_cast<double *>(whist + a);
+        double2 w0 = ld2(WH), w1 = ld2(WH + 2);
+        d3 fw, tw, hw;
+        double fow;
+        int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
+                             d3{r4.x, r4.y, r5.x}, r1.y, r3.y, d3{w0.x, w0.y, w1.x}, I.dt, I.pw,
+                             fw, tw, hw, fow);
+        if (hits) {
+            st2(WH, hw.x, hw.y);
+            st2(WH + 2, hw.z, fow);
+        } else {
+            st2(WH, 0.0, 0.0);
+            st2(WH + 2, 0.0, 0.0);
+        }
+        double *W = wforce + 8 * (size_t)a;
+        st2(W, fw.x, fw.y);
+        st2(W + 2, fw.z, (double)hits);
+        st2(W + 4, tw.x, tw.y);
+        st2(W + 6, tw.z, 0.0);
+    }
+}
+
 // Per-particle kernel: gather of the particle's pair forces (own pairs in list order, then
 // pairs where it is second, via the transposed index), then gravity and walls
 // (calculateforceiwD, neiglist.cu:238-306), then the integrator (updatepositionD,
@@ -379,9 +460,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
 // not reduced here (they are diagnostics): k_inst_reduce computes them on demand.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
-    PRec *rec, Hist *whist, const unsigned *off, const unsigned *toff, const unsigned *jt,
-    const POut *out, unsigned n, const InstDev *inst, InstDev one, unsigned *contacti,
-    double *fdbg, DevCtl *ctl)
+    PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
+    const unsigned *toff, const unsigned *jt, const POut *out, unsigned n, const InstDev *inst,
+    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl)
 {
     const unsigned np = ctl->npairs;
     unsigned a = blockIdx.x * kBlock + threadIdx.x;
@@ -430,23 +511,16 @@ __global__ __launch_bounds__(kBlock) void k_particle(
         fx = fx + 0.0;
         fy = fy + 0.0;
         fz = fz + (-m);
-        // walls
-        const double *WH = reinterpret_cast<const double *>(whist + a);
-        double2 w0 = ld2(WH), w1 = ld2(WH + 2);
-        d3 fw, tw, hw;
-        double fow;
-        int hits = law_walls(I.walls, x, v, w, rad, m, d3{w0.x, w0.y, w1.x}, I.dt, I.pw, fw, tw,
-                             hw, fow);
-        double *WHW = reinterpret_cast<double *>(whist + a);
-        if (hits) {
-            fx += fw.x; fy += fw.y; fz += fw.z;
-            tx += tw.x; ty += tw.y; tz += tw.z;
-            st2(WHW, hw.x, hw.y);
-            st2(WHW + 2, hw.z, fow);
-        } else {
-            st2(WHW, 0.0, 0.0);
-            st2(WHW + 2, 0.0, 0.0);
+        // walls (computed by k_walls for this rebuild's wall candidates)
+        if (wflag[a]) {
+            const double *W = wforce + 8 * (size_t)a;
+            double2 w0 = ld2(W), w1 = ld2(W + 2), w2 = ld2(W + 4), w3 = ld2(W + 6);
+            if (w1.y != 0.0) {
+                fx += w0.x; fy += w0.y; fz += w1.x;
+                tx += w2.x; ty += w2.y; tz += w3.x;
+            }
         }
+        (void)rad;
         if (fdbg) {
             double *D = fdbg + 6 * (size_t)a;
             D[0] = fx; D[1] = fy; D[2] = fz; D[3] = tx; D[4] = ty; D[5] = tz;

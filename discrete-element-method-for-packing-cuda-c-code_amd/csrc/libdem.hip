@@ -175,6 +175,8 @@ struct dem_ctx {
     unsigned *inv[2] = {nullptr, nullptr};
     int rc = 0;   // index of the current particle buffers
     unsigned *contacti = nullptr;
+    unsigned *wflag = nullptr, *wlist = nullptr;   // wall candidates of the current rebuild
+    double *wforce = nullptr;                      // k_walls output, 8 doubles per slot
     double *fdbg = nullptr;
     bool record_forces = false;
 
@@ -243,6 +245,13 @@ int alloc_out(dem_ctx *c, unsigned cap)
     return DEM_OK;
 }
 
+unsigned wall_grid(const dem_ctx *c)
+{
+    // grid-stride over the device-resident candidate count; at most one lane per particle
+    unsigned g = cdiv(std::max(c->n, 1u), kBlock);
+    return std::max(1u, std::min(g, (unsigned)c->cus * 4u));
+}
+
 unsigned pair_grid(const dem_ctx *c, unsigned np)
 {
     unsigned g = cdiv(std::max(np, 1u), kBlock);
@@ -271,9 +280,16 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
     hipLaunchKernelGGL(k_cell_range, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
                        c->cstart, c->cend, active);
     int nr = 1 - c->rc;
-    hipLaunchKernelGGL(k_reorder, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
-                       c->rec[c->rc], c->rec[nr], c->whist[c->rc], c->whist[nr], c->inv[c->rc],
-                       c->inv[nr], active);
+    if (c->batched)
+        hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
+                           n, c->rec[c->rc], c->rec[nr], c->whist[c->rc], c->whist[nr],
+                           c->inv[c->rc], c->inv[nr], (const InstDev *)c->dinst, one, c->wflag,
+                           c->wlist, c->ctl, active);
+    else
+        hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
+                           (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
+                           c->whist[c->rc], c->whist[nr], c->inv[c->rc], c->inv[nr],
+                           (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl, active);
     c->rc = nr;
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
@@ -366,9 +382,13 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(kBlock), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
+                           (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
+                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
         if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           c->whist[c->rc], (const unsigned *)c->off[c->lc],
+                           (const unsigned *)c->wflag, (const double *)c->wforce,
+                           (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
                            c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
@@ -376,9 +396,13 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(kBlock), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+        hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
+                           (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
+                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
         if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           c->whist[c->rc], (const unsigned *)c->off[c->lc],
+                           (const unsigned *)c->wflag, (const double *)c->wforce,
+                           (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
                            c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
@@ -416,7 +440,7 @@ void free_ctx(dem_ctx *c)
 {
     if (!c) return;
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
-                    c->contacti, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->ctl, c->inst_begin};
     for (void *p : bufs)
@@ -596,6 +620,9 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     }
     if (alloc_out(c, pc)) { free_ctx(c); return DEM_ERR_HIP; }
     DEM_HIP(hipMalloc(&c->contacti, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->wflag, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->wlist, sizeof(unsigned) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->wforce, sizeof(double) * 8 * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_b, sizeof(uint2) * (size_t)n));
