@@ -67,14 +67,25 @@ def cpu_baseline(dem, w, budget_s):
                       f"step (list build {t_first:.1f} s, excluded), oracle C/OpenMP"}
 
 
-def bench_batched(dem, steps, interval, log_every=64):
-    """Batched form (256 instances, hipGraph-captured) against the same instances stepped as
-    per-instance plain contexts (host loop, reference trigger off, same rebuild interval)."""
+def bench_batched(dem, steps, interval, tune_steps=48):
+    """Batched form (256 instances, hipGraph-captured, autotuned skin / launch shapes / rebuild
+    interval) against the same instances stepped as per-instance plain contexts (same skin and
+    fixed rebuild interval): host loop, and hipGraph replays."""
     t0 = time.perf_counter()
     w = dem.config3(rebuild_interval=interval)
     log(f"batched workload: {len(w.instances)} instances, n={w.n} ({time.perf_counter() - t0:.1f} s)")
     big = dem.Context(w.instances)
     big.load(w)
+    tune = None
+    skin = dem.SKIN
+    if tune_steps:
+        big.set_option(dem.OPT_REBUILD_INTERVAL, 0)
+        tune = big.autotune(tune_steps)
+        interval = max(1, min(int(tune["interval"]), 200))
+        skin = tune["skin"]
+        big.set_option(dem.OPT_REBUILD_INTERVAL, interval)
+        log(f"autotune: {tune}")
+    steps = max(interval, (steps // interval) * interval)
     big.step_graph(2 * interval)
     big.sync()
     t0 = time.perf_counter()
@@ -82,6 +93,7 @@ def bench_batched(dem, steps, interval, log_every=64):
     big.sync()
     tb = time.perf_counter() - t0
     nb = w.n
+    flag = big.stats().last_rebuild_flag
     del big
     # per-instance plain launches: one context per instance, run back to back
     offs = [0]
@@ -92,6 +104,7 @@ def bench_batched(dem, steps, interval, log_every=64):
         c = dem.Context([inst])
         sl = slice(offs[k], offs[k + 1])
         c.set_state(w.pos[sl], w.pdis[sl], w.angv[sl], w.rad[sl], w.rmass[sl], w.inert[sl])
+        c.set_option(dem.OPT_SKIN, skin)
         c.set_option(dem.OPT_REBUILD_INTERVAL, interval)
         c.step(2 * interval)
         ctxs.append(c)
@@ -111,7 +124,8 @@ def bench_batched(dem, steps, interval, log_every=64):
     return {"workload": w.name, "instances": len(w.instances), "n": nb, "steps": steps,
             "rebuild_interval": interval, "value": vb, "per_instance_plain": vp,
             "per_instance_graph": vg, "ratio_vs_plain": vb / vp, "ratio_vs_plain_graph": vb / vg,
-            "ms_per_step": tb / steps * 1e3}
+            "ms_per_step": tb / steps * 1e3, "trigger_in_last_group": int(flag),
+            "autotune": tune}
 
 
 def main():

@@ -39,6 +39,10 @@ WALLS_CYLINDER = 1
 
 OPT_RECORD_FORCES = 1
 OPT_REBUILD_INTERVAL = 2
+OPT_SKIN = 4
+OPT_PAIR_BLOCK = 5
+OPT_PAIR_GRID = 6
+OPT_PARTICLE_BLOCK = 7
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0
@@ -91,6 +95,14 @@ class Stats(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("skin", c_double), ("cell", c_double), ("interval", c_int),
+        ("pair_block", c_uint), ("pair_grid_per_cu", c_uint), ("particle_block", c_uint),
+        ("ms_per_step", c_double), ("score_ms", c_double), ("trials", c_int), ("pad", c_int),
+    ]
+
+
 class Deck(ctypes.Structure):
     _fields_ = [
         ("iforcemode", c_int),
@@ -124,7 +136,7 @@ EXPORTS = [
     "calculateforcepair", "sumallforcei", "calculateforceiw", "updateposition",
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
     "dem_get_state", "dem_get_wall_history", "dem_step", "dem_step_graph",
-    "dem_request_rebuild", "dem_rebuild", "dem_set_active", "dem_active_count", "dem_get_stats", "dem_get_instance_stats",
+    "dem_request_rebuild", "dem_rebuild", "dem_autotune", "dem_set_active", "dem_active_count", "dem_get_stats", "dem_get_instance_stats",
     "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
     "dem_get_pairs", "dem_get_contacts", "dem_get_forces", "dem_device_info", "dem_read_deck",
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
@@ -156,6 +168,7 @@ def lib() -> ctypes.CDLL:
         "dem_step_graph": ([P, c_int], c_int),
         "dem_request_rebuild": ([P], c_int),
         "dem_rebuild": ([P], c_int),
+        "dem_autotune": ([P, c_int, P], c_int),
         "dem_set_active": ([P, c_uint], c_int),
         "dem_active_count": ([P], c_uint),
         "dem_get_stats": ([P, P], c_int),
@@ -499,6 +512,12 @@ class Context:
 
     def rebuild(self):
         _check(self._L.dem_rebuild(self._h), "dem_rebuild")
+
+    def autotune(self, steps: int = 64) -> dict:
+        """Run the project autotuner (advances the simulation by its trial steps)."""
+        r = TuneResult()
+        _check(self._L.dem_autotune(self._h, int(steps), ctypes.byref(r)), "dem_autotune")
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_ if k != "pad"}
 
     def set_active(self, n: int):
         _check(self._L.dem_set_active(self._h, int(n)), "dem_set_active")

@@ -380,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                                                        double dt)
 {
     const unsigned np = *npairs;
-    for (unsigned p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+    for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
         uint2 e = ab[p];
         const double *A = reinterpret_cast<const double *>(rec + e.x);
         const double *B = reinterpret_cast<const double *>(rec + e.y);
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
                                                   double *wforce)
 {
     const unsigned nw = ctl->nwall;
-    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < nw; i += gridDim.x * kBlock) {
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
         unsigned a = wlist[i];
         const double *R = reinterpret_cast<const double *>(rec + a);
         double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_particle(
     InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl)
 {
     const unsigned np = ctl->npairs;
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = a < n;
     unsigned flag = 0;
     if (live) {
@@ -563,45 +563,43 @@ __global__ void k_publish(DevCtl *ctl, unsigned char *mirror)
 }
 
 // Per-instance max(0, max z) and contact totals (the reference's host max over
-// updatepositionD's block partials, particleSystem.cu:372-379, and outputporositycn).
-// One block per instance; exact (max of non-negative doubles as ordered bit patterns,
-// integer sums), so the result does not depend on the reduction order.
+// updatepositionD's block partials, particleSystem.cu:372-379, and outputporositycn).  Grid-
+// stride over slots; each wave reduces its lanes of one instance and adds them with one
+// atomic per quantity.  Max of non-negative doubles as ordered bit patterns and integer sums
+// are exact, so the result does not depend on the order.  `acc` (u64[ninst] max-z bits, then
+// u32[ninst] totals) must be zero on entry.
 __global__ __launch_bounds__(kBlock) void k_inst_reduce(const PRec *rec, const unsigned *contacti,
-                                                        const InstDev *inst, unsigned ninst,
-                                                        const unsigned *slot_inst_begin,
-                                                        unsigned char *mirror)
+                                                        unsigned n, unsigned long long *acc_z,
+                                                        unsigned *acc_c)
 {
-    unsigned k = blockIdx.x;
-    if (k >= ninst) return;
-    unsigned s0 = slot_inst_begin[k], s1 = slot_inst_begin[k + 1];
-    unsigned long long zb = 0ull;
-    unsigned cc = 0;
-    for (unsigned s = s0 + threadIdx.x; s < s1; s += kBlock) {
-        double z = rec[s].pz;
-        unsigned long long b = (unsigned long long)__double_as_longlong(z > 0.0 ? z : 0.0);
-        zb = b > zb ? b : zb;
-        cc += contacti[s];
-    }
+    for (unsigned base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        unsigned s = base + threadIdx.x;
+        bool live = s < n;
+        unsigned inst = live ? rec[s].inst : 0u;
+        unsigned long long zb = 0ull;
+        unsigned cc = 0;
+        if (live) {
+            double z = rec[s].pz;
+            zb = (unsigned long long)__double_as_longlong(z > 0.0 ? z : 0.0);
+            cc = contacti[s];
+        }
+        unsigned inst0 = __builtin_amdgcn_readfirstlane(inst);
+        if (__all(!live || inst == inst0)) {
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        unsigned long long oz = __shfl_xor(zb, d, 64);
-        zb = oz > zb ? oz : zb;
-        cc += __shfl_xor(cc, d, 64);
+            for (int d = 32; d >= 1; d >>= 1) {
+                unsigned long long oz = __shfl_xor(zb, d, 64);
+                zb = oz > zb ? oz : zb;
+                cc += __shfl_xor(cc, d, 64);
+            }
+            if (__lane_id() == 0 && __any(live)) {
+                atomicMax(&acc_z[inst0], zb);
+                atomicAdd(&acc_c[inst0], cc);
+            }
+        } else if (live) {
+            atomicMax(&acc_z[inst], zb);
+            atomicAdd(&acc_c[inst], cc);
+        }
     }
-    __shared__ unsigned long long s_z[kBlock / 64];
-    __shared__ unsigned s_c[kBlock / 64];
-    if (__lane_id() == 0) { s_z[threadIdx.x >> 6] = zb; s_c[threadIdx.x >> 6] = cc; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        zb = s_z[0];
-        cc = s_c[0];
-        for (int w = 1; w < kBlock / 64; w++) { zb = s_z[w] > zb ? s_z[w] : zb; cc += s_c[w]; }
-        unsigned long long *mz = reinterpret_cast<unsigned long long *>(mirror + sizeof(DevCtl));
-        unsigned *mp = reinterpret_cast<unsigned *>(mz + ninst);
-        mz[k] = zb;
-        mp[k] = cc;
-    }
-    (void)inst;
 }
 
 // ----------------------------------------------------------------------------------------

@@ -201,6 +201,7 @@ struct dem_ctx {
     unsigned char *mirror_h = nullptr, *mirror_d = nullptr;
 
     int interval = 0;
+    unsigned pair_block = 256, pair_grid_per_cu = 16, particle_block = 256;   // launch shapes
     unsigned long long steps = 0, rebuilds = 0;
     unsigned since = 0;
     bool need_rebuild = true;
@@ -208,7 +209,6 @@ struct dem_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evs = nullptr;
     float last_ms = 0.f;
 
-    unsigned *inst_begin = nullptr;     // slot range of each instance, [ninst + 1]
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     int graph_k = 0;
@@ -254,8 +254,8 @@ unsigned wall_grid(const dem_ctx *c)
 
 unsigned pair_grid(const dem_ctx *c, unsigned np)
 {
-    unsigned g = cdiv(std::max(np, 1u), kBlock);
-    unsigned gmax = (unsigned)c->cus * 16u;
+    unsigned g = cdiv(std::max(np, 1u), c->pair_block);
+    unsigned gmax = (unsigned)c->cus * c->pair_grid_per_cu;
     return std::max(1u, std::min(g, gmax));
 }
 
@@ -375,32 +375,33 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                        hipEvent_t end = nullptr)
 {
     hipStream_t st = c->stream;
-    unsigned gb = cdiv(c->n, kBlock);
+    unsigned pb = c->particle_block;
+    unsigned gb = cdiv(c->n, pb);
     InstDev one = c->hinst[0];
     const unsigned *np = &c->ctl->npairs;
     if (c->batched) {
-        hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
         hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
                            (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
                            (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
         if (mid) (void)hipEventRecord(mid, st);
-        hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+        hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
                            c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
     } else {
-        hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(kBlock), 0, st,
+        hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
                            c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
         hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
                            (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
                            (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
         if (mid) (void)hipEventRecord(mid, st);
-        hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+        hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
@@ -436,13 +437,54 @@ InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
     return I;
 }
 
+// Change the list skin of every instance (the cell edge follows: cell = (cell - skin) + new
+// skin, i.e. the 2 r_max part is kept) and rebuild the grids over the same extents.  The
+// physics does not depend on the skin (pairs outside contact contribute nothing); the
+// rebuild trigger |q| > skin/2 does.
+int regrid(dem_ctx *c, double skin)
+{
+    if (!(skin > 0.0)) return DEM_ERR_INVALID;
+    unsigned cell_off = 0;
+    for (int i = 0; i < c->ninst; i++) {
+        InstDev &I = c->hinst[i];
+        unsigned long long cells = 1;
+        for (int k = 0; k < 3; k++) {
+            double extent = (double)I.gsize[k] * I.cell[k];
+            double cell = (I.cell[k] - I.skin) + skin;
+            I.cell[k] = cell;
+            I.gsize[k] = (unsigned)(int)(extent / cell) + 1u;
+            cells *= I.gsize[k];
+        }
+        if (cells + cell_off >= 0xffffffffull) return DEM_ERR_INVALID;
+        I.skin = skin;
+        double ann = 0.5 * skin;
+        I.ann2 = ann * ann;
+        I.cell_off = cell_off;
+        cell_off += (unsigned)cells;
+    }
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
+    if (cell_off != c->ncells) {
+        DEM_HIP(hipFree(c->cstart));
+        DEM_HIP(hipFree(c->cend));
+        c->cstart = c->cend = nullptr;
+        DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * (size_t)cell_off));
+        DEM_HIP(hipMalloc(&c->cend, sizeof(unsigned) * (size_t)cell_off));
+        c->ncells = cell_off;
+        c->key_bits = key_bits_for(cell_off);
+    }
+    c->need_rebuild = true;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
 void free_ctx(dem_ctx *c)
 {
     if (!c) return;
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
-                    c->hist[1], c->out, c->jt, c->ctl, c->inst_begin};
+                    c->hist[1], c->out, c->jt, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->S.release();
@@ -631,13 +673,6 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->ng, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->toff, sizeof(unsigned) * ((size_t)n + 1)));
-    {
-        std::vector<unsigned> ib(ninst + 1);
-        for (int i = 0; i < ninst; i++) ib[i] = c->hinst[i].p_off;
-        ib[ninst] = c->n;
-        DEM_HIP(hipMalloc(&c->inst_begin, sizeof(unsigned) * (ninst + 1)));
-        DEM_HIP(hipMemcpy(c->inst_begin, ib.data(), sizeof(unsigned) * (ninst + 1), hipMemcpyHostToDevice));
-    }
     c->ctl_bytes = sizeof(DevCtl) + (size_t)ninst * (sizeof(unsigned long long) + sizeof(unsigned));
     DEM_HIP(hipMalloc(&c->ctl, c->ctl_bytes));
     DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
@@ -671,6 +706,25 @@ int dem_set_option(dem_ctx *c, int option, double value)
         c->interval = (int)value;
         drop_graph(c);
         return DEM_OK;
+    case DEM_OPT_SKIN: {
+        drop_graph(c);
+        return regrid(c, value);
+    }
+    case DEM_OPT_PAIR_BLOCK:
+    case DEM_OPT_PARTICLE_BLOCK: {
+        unsigned b = (unsigned)value;
+        if (b != 64 && b != 128 && b != 256) return DEM_ERR_INVALID;
+        if (option == DEM_OPT_PAIR_BLOCK) c->pair_block = b;
+        else c->particle_block = b;
+        drop_graph(c);
+        return DEM_OK;
+    }
+    case DEM_OPT_PAIR_GRID: {
+        if (!(value >= 1.0 && value <= 256.0)) return DEM_ERR_INVALID;
+        c->pair_grid_per_cu = (unsigned)value;
+        drop_graph(c);
+        return DEM_OK;
+    }
     default: return DEM_ERR_INVALID;
     }
 }
@@ -741,8 +795,6 @@ int dem_set_active(dem_ctx *c, uint n_new)
     if (n_new == c->n) return DEM_OK;
     DEM_HIP(hipStreamSynchronize(c->stream));
     c->n = n_new;
-    unsigned ib[2] = {0u, n_new};
-    DEM_HIP(hipMemcpy(c->inst_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
     c->need_rebuild = true;
     drop_graph(c);
     return DEM_OK;
@@ -984,14 +1036,132 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
 
 static int reduce_instances(dem_ctx *c)
 {
-    // slots of instance k are [p_off, p_off + n) in sorted order too: the sort keys are
-    // offset per instance, so the reorder keeps instances contiguous and in order
-    hipLaunchKernelGGL(k_inst_reduce, dim3(c->ninst), dim3(kBlock), 0, c->stream,
-                       (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti,
-                       (const InstDev *)c->dinst, (unsigned)c->ninst,
-                       (const unsigned *)c->inst_begin, c->mirror_d);
+    // per-instance max z / contact totals into the mirror's tail (zeroed host-side first; the
+    // kernel writes device-visible coherent host memory directly)
+    unsigned long long *mz = reinterpret_cast<unsigned long long *>(c->mirror_h + sizeof(DevCtl));
+    unsigned *mp = reinterpret_cast<unsigned *>(mz + c->ninst);
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    std::memset(mz, 0, sizeof(unsigned long long) * c->ninst);
+    std::memset(mp, 0, sizeof(unsigned) * c->ninst);
+    unsigned long long *dz = reinterpret_cast<unsigned long long *>(c->mirror_d + sizeof(DevCtl));
+    unsigned *dp = reinterpret_cast<unsigned *>(dz + c->ninst);
+    unsigned g = std::max(1u, std::min(cdiv(c->n, kBlock), (unsigned)c->cus * 4u));
+    hipLaunchKernelGGL(k_inst_reduce, dim3(g), dim3(kBlock), 0, c->stream,
+                       (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti, c->n, dz, dp);
     DEM_HIP(hipGetLastError());
     DEM_HIP(hipStreamSynchronize(c->stream));
+    return DEM_OK;
+}
+
+// One trial of `steps` steps with the reference trigger, timed with HIP events: mean force-
+// step time (pair + walls + particle kernels), mean rebuild time, and the smallest gap (in
+// steps) between trigger-raised rebuilds (-1 if none fired).
+static int tune_trial(dem_ctx *c, int steps, double *force_ms, double *rebuild_ms, int *min_gap,
+                      hipEvent_t e0, hipEvent_t e1)
+{
+    double tf = 0.0, tr = 0.0;
+    int nf = 0, nr = 0, last = -1, gap = -1;
+    for (int s = 0; s < steps; s++) {
+        if (c->need_rebuild) {
+            bool triggered = c->last_flag != 0u;
+            DEM_HIP(hipEventRecord(e0, c->stream));
+            int rc = rebuild_host(c);
+            if (rc) return rc;
+            DEM_HIP(hipEventRecord(e1, c->stream));
+            DEM_HIP(hipEventSynchronize(e1));
+            float ms;
+            DEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tr += ms;
+            nr++;
+            if (triggered) {
+                if (last >= 0 && (gap < 0 || s - last < gap)) gap = s - last;
+                last = s;
+            } else {
+                last = s;
+            }
+        }
+        DEM_HIP(hipEventRecord(e0, c->stream));
+        launch_force_step(c, pair_grid(c, c->npairs));
+        DEM_HIP(hipEventRecord(e1, c->stream));
+        c->steps++;
+        c->since++;
+        int rc = sync_mirror(c);
+        if (rc) return rc;
+        c->need_rebuild = c->last_flag != 0u;
+        float ms;
+        DEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        tf += ms;
+        nf++;
+    }
+    *force_ms = nf ? tf / nf : 0.0;
+    *rebuild_ms = nr ? tr / nr : 0.0;
+    *min_gap = gap;
+    return DEM_OK;
+}
+
+int dem_autotune(dem_ctx *c, int steps, dem_tune_result *out)
+{
+    if (!c || !out || steps < 8) return DEM_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    hipEvent_t e0, e1;
+    DEM_HIP(hipEventCreate(&e0));
+    DEM_HIP(hipEventCreate(&e1));
+    int saved_interval = c->interval;
+    c->interval = 0;   // trials use the reference trigger, so every trial step is exact
+    static const double skins[] = {0.3, 0.45, 0.55, 0.7, 0.9};
+    double best_score = 1e300, best_skin = c->hinst[0].skin;
+    int best_k = 1;
+    int trials = 0;
+    for (double sk : skins) {
+        int rc = regrid(c, sk);
+        if (rc) return rc;
+        double tf, tr;
+        int gap;
+        if ((rc = tune_trial(c, steps, &tf, &tr, &gap, e0, e1))) return rc;
+        trials++;
+        // safe fixed interval: 3/4 of the shortest observed trigger gap (the trial length if
+        // the trigger never fired)
+        int k = gap > 0 ? std::max(1, (int)(0.75 * gap)) : steps;
+        double score = tf + tr / k;
+        if (score < best_score) { best_score = score; best_skin = sk; best_k = k; }
+    }
+    int rc = regrid(c, best_skin);
+    if (rc) return rc;
+    // launch shapes at the chosen skin (force steps only)
+    static const unsigned blocks[] = {64, 128, 256};
+    static const unsigned grids[] = {4, 8, 16, 32};
+    double best_t = 1e300;
+    unsigned bp = c->pair_block, bg = c->pair_grid_per_cu, bq = c->particle_block;
+    int sub = std::max(4, steps / 4);
+    for (unsigned pb : blocks)
+        for (unsigned pg : grids)
+            for (unsigned qb : blocks) {
+                if (qb != 256 && !(pb == 256 && pg == 16)) continue;   // particle block swept once
+                c->pair_block = pb;
+                c->pair_grid_per_cu = pg;
+                c->particle_block = qb;
+                double tf, tr;
+                int gap;
+                if ((rc = tune_trial(c, sub, &tf, &tr, &gap, e0, e1))) return rc;
+                trials++;
+                if (tf < best_t) { best_t = tf; bp = pb; bg = pg; bq = qb; }
+            }
+    c->pair_block = bp;
+    c->pair_grid_per_cu = bg;
+    c->particle_block = bq;
+    c->interval = saved_interval;
+    drop_graph(c);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out->skin = best_skin;
+    out->cell = c->hinst[0].cell[0];
+    out->interval = best_k;
+    out->pair_block = bp;
+    out->pair_grid_per_cu = bg;
+    out->particle_block = bq;
+    out->ms_per_step = best_t;
+    out->score_ms = best_score;
+    out->trials = trials;
     return DEM_OK;
 }
 

