@@ -142,7 +142,8 @@ EXPORTS = [
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
     "dem_restitution_for_damping", "dem_gen_diameters", "dem_gen_insert", "dem_gen_lattice",
     "dem_particle_props", "dem_make_grid", "dem_dev_alloc", "dem_dev_free", "dem_memcpy",
-    "dem_memzero", "dem_device_sync",
+    "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_law_pair_vjp",
+    "dem_law_walls_eval", "dem_law_walls_vjp",
 ]
 
 
@@ -188,6 +189,10 @@ def lib() -> ctypes.CDLL:
         "dem_memcpy": ([P, P, ctypes.c_size_t, c_int], c_int),
         "dem_memzero": ([P, ctypes.c_size_t], c_int),
         "dem_device_sync": ([], c_int),
+        "dem_law_pair_eval": ([P, P, P], c_int),
+        "dem_law_pair_vjp": ([P, P, P, P, P], c_int),
+        "dem_law_walls_eval": ([P, P, P, P], c_int),
+        "dem_law_walls_vjp": ([P, P, P, P, P, P], c_int),
         "dem_read_deck": ([ctypes.c_char_p, P], c_int),
         "dem_deck_material": ([P, P], c_int),
         "dem_write_restart": ([ctypes.c_char_p], c_int),

@@ -16,6 +16,11 @@
 // pair's result does not depend on which member is "first".
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
+
+// Host + device: the kernels use the laws, and host entry points evaluate the same code for
+// tests of the adjoint.
+#define DEM_HD __host__ __device__ __forceinline__
 
 namespace dem {
 
@@ -23,13 +28,13 @@ struct d3 {
     double x, y, z;
 };
 
-__device__ __forceinline__ d3 mk3(double x, double y, double z) { return d3{x, y, z}; }
-__device__ __forceinline__ double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ d3 cross3(d3 a, d3 b)
+DEM_HD d3 mk3(double x, double y, double z) { return d3{x, y, z}; }
+DEM_HD double dot3(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+DEM_HD d3 cross3(d3 a, d3 b)
 {
     return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-__device__ __forceinline__ d3 scale3(d3 a, double s) { return d3{a.x * s, a.y * s, a.z * s}; }
+DEM_HD d3 scale3(d3 a, double s) { return d3{a.x * s, a.y * s, a.z * s}; }
 
 // Material constants of one contact type (particle-particle or particle-wall); the
 // reference's Matproperties / Wallproperties members in order.
@@ -43,7 +48,7 @@ constexpr double kXSmall = 1e-30;   // neiglist.cu:29
 // rs, ms effective radius and mass; vrel: displacement increment of the other body's
 // contact point relative to i's; wrel = w_i - w_other; hin: incoming tangential history.
 // Outputs: force on i, tangential force on i, rolling moment on i, new history, |Ft|.
-__device__ __forceinline__ void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
+DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
                                          d3 wrel, d3 hin, double dt, const LawParams &P,
                                          d3 &fi, d3 &ft, d3 &mr, d3 &hout, double &fout)
 {
@@ -110,7 +115,7 @@ __device__ __forceinline__ void law_core(d3 n, double dn, double rs, double ms, 
 
 // Pair law between bodies a and b.  Returns true on contact; on no contact every output
 // is zero (history reset), matching the reference's no-contact branch (neiglist.cu:220-227).
-__device__ __forceinline__ bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
+DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
                                          double rb, double ma, double mb, d3 hin, double dt,
                                          const LawParams &P, d3 &fa, d3 &ta, d3 &tb,
                                          d3 &hout, double &fout)
@@ -147,9 +152,9 @@ struct WallGeom {
     double radius;     // cylinder radius
 };
 
-__device__ __forceinline__ int wall_count(const WallGeom &g) { return g.kind == 0 ? 5 : 2; }
+DEM_HD int wall_count(const WallGeom &g) { return g.kind == 0 ? 5 : 2; }
 
-__device__ __forceinline__ double wall_overlap(const WallGeom &g, int k, d3 x, double r, d3 &n)
+DEM_HD double wall_overlap(const WallGeom &g, int k, d3 x, double r, d3 &n)
 {
     if (k == 0) { n = d3{0.0, 0.0, -1.0}; return r - x.z; }
     if (g.kind == 0) {
@@ -170,7 +175,7 @@ __device__ __forceinline__ double wall_overlap(const WallGeom &g, int k, d3 x, d
 // Every overlapped wall contributes; the particle's single history slot belongs to the wall
 // with the largest overlap (lowest index on ties), other walls start from zero history.
 // Returns the number of walls hit.
-__device__ __forceinline__ int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
+DEM_HD int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
                                          double m, d3 hin, double dt, const LawParams &P,
                                          d3 &fw, d3 &tw, d3 &hout, double &fout)
 {
@@ -209,7 +214,7 @@ __device__ __forceinline__ int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, do
 }
 
 // Grid cell coordinate (neiglist.cu:54-74): floor((p - o) / c) clamped to [0, g-1].
-__device__ __forceinline__ unsigned cell_coord(double p, double o, double c, unsigned g)
+DEM_HD unsigned cell_coord(double p, double o, double c, unsigned g)
 {
     double f = floor((p - o) / c);
     if (!(f > 0.0)) return 0u;
@@ -218,7 +223,7 @@ __device__ __forceinline__ unsigned cell_coord(double p, double o, double c, uns
 }
 
 // Candidate test shared by every neighbour kernel: |xb - xa|^2 < (ra + rb + skin)^2.
-__device__ __forceinline__ bool is_candidate(d3 xa, d3 xb, double ra, double rb, double skin)
+DEM_HD bool is_candidate(d3 xa, d3 xb, double ra, double rb, double skin)
 {
     double dx = xb.x - xa.x;
     double dy = xb.y - xa.y;

@@ -11,6 +11,7 @@
 
 #include "../../include/dem.h"
 #include "kernels.hpp"
+#include "law_adj.hpp"
 #include "sort.hpp"
 
 using namespace dem;
@@ -1294,6 +1295,81 @@ int dem_get_forces(dem_ctx *c, double *force, double *torque, uint *contacti)
         }
         if (contacti) contacti[i] = act ? ct[s] : 0u;
     }
+    return DEM_OK;
+}
+
+// =========================================================================================
+// Host evaluation of the contact laws and their vector-Jacobian products (the same inline
+// code the kernels run), for tests and tools.
+// =========================================================================================
+
+namespace {
+LawParams law6(const double *m) { return LawParams{m[0], m[1], m[2], m[3], m[4], m[5]}; }
+d3 v3(const double *p) { return d3{p[0], p[1], p[2]}; }
+void put3(double *p, d3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+WallGeom wall_geom(const dem_walls *w)
+{
+    WallGeom g;
+    g.kind = w->kind;
+    g.hx = 0.5 * w->size[0];
+    g.hy = 0.5 * w->size[1];
+    g.radius = w->radius;
+    return g;
+}
+}  // namespace
+
+int dem_law_pair_eval(const double *in, const double *mat, double *out)
+{
+    if (!in || !mat || !out) return -1;
+    d3 fa, ta, tb, ho;
+    double fo;
+    bool c = law_pair(v3(in), v3(in + 3), v3(in + 6), v3(in + 9), v3(in + 12), v3(in + 15), in[18],
+                      in[19], in[20], in[21], v3(in + 22), in[25], law6(mat), fa, ta, tb, ho, fo);
+    put3(out, fa); put3(out + 3, ta); put3(out + 6, tb); put3(out + 9, ho);
+    out[12] = fo;
+    return c ? 1 : 0;
+}
+
+int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
+                     double *matbar)
+{
+    if (!in || !mat || !outbar || !inbar || !matbar) return -1;
+    d3 g[7];
+    for (int k = 0; k < 7; k++) g[k] = d3{0.0, 0.0, 0.0};
+    LawParamsBar pb = {0.0, 0.0, 0.0, 0.0, 0.0};
+    law_pair_adj(v3(in), v3(in + 3), v3(in + 6), v3(in + 9), v3(in + 12), v3(in + 15), in[18],
+                 in[19], in[20], in[21], v3(in + 22), in[25], law6(mat), v3(outbar),
+                 v3(outbar + 3), v3(outbar + 6), v3(outbar + 9), g[0], g[1], g[2], g[3], g[4],
+                 g[5], g[6], pb);
+    for (int k = 0; k < 7; k++) put3(inbar + 3 * k, g[k]);
+    matbar[0] = pb.estar; matbar[1] = pb.mu; matbar[2] = pb.gn; matbar[3] = pb.gt; matbar[4] = pb.ur;
+    return DEM_OK;
+}
+
+int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out)
+{
+    if (!walls || !in || !mat || !out) return -1;
+    d3 fw, tw, hw;
+    double fo;
+    int hits = law_walls(wall_geom(walls), v3(in), v3(in + 3), v3(in + 6), in[9], in[10],
+                         v3(in + 11), in[14], law6(mat), fw, tw, hw, fo);
+    put3(out, fw); put3(out + 3, tw); put3(out + 6, hw);
+    out[9] = fo;
+    return hits;
+}
+
+int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
+                      const double *outbar, double *inbar, double *matbar)
+{
+    if (!walls || !in || !mat || !outbar || !inbar || !matbar) return -1;
+    d3 g[4];
+    for (int k = 0; k < 4; k++) g[k] = d3{0.0, 0.0, 0.0};
+    LawParamsBar pb = {0.0, 0.0, 0.0, 0.0, 0.0};
+    law_walls_adj(wall_geom(walls), v3(in), v3(in + 3), v3(in + 6), in[9], in[10], v3(in + 11),
+                  in[14], law6(mat), v3(outbar), v3(outbar + 3), v3(outbar + 6), g[0], g[1], g[2],
+                  g[3], pb);
+    for (int k = 0; k < 4; k++) put3(inbar + 3 * k, g[k]);
+    matbar[0] = pb.estar; matbar[1] = pb.mu; matbar[2] = pb.gn; matbar[3] = pb.gt; matbar[4] = pb.ur;
     return DEM_OK;
 }
 

@@ -284,6 +284,23 @@ int dem_memzero(void *dst, size_t bytes);
 int dem_device_sync(void);
 
 /* ----------------------------------------------------------------------------------- */
+/* Contact laws on the host (the kernels' own inline code) and their adjoints            */
+/* ----------------------------------------------------------------------------------- */
+
+/* mat = (E*, mu, nu, gamma_n, gamma_t, mu_r).  Pair: in = xa, xb, va, vb, wa, wb (18), ra, rb,
+ * ma, mb, hin (3), dt (26 values); out = F on a, Ta, Tb, h_out, |Ft| (13).  Returns 1 on
+ * contact.  vjp: outbar = adjoints of (F, Ta, Tb, h_out) (12); inbar = adjoints of xa, xb, va,
+ * vb, wa, wb, hin (21); matbar = adjoints of (E*, mu, gamma_n, gamma_t, mu_r). */
+int dem_law_pair_eval(const double *in, const double *mat, double *out);
+int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
+                     double *matbar);
+/* Walls: in = x, v, w (9), r, m, hin (3), dt (15); out = F, T, h_out, |Ft| (10), returns the
+ * walls hit; vjp outbar = (F, T, h_out) (9), inbar = x, v, w, hin (12). */
+int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out);
+int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
+                      const double *outbar, double *inbar, double *matbar);
+
+/* ----------------------------------------------------------------------------------- */
 /* Host utilities: deck, units, restart file, workload generators                       */
 /* ----------------------------------------------------------------------------------- */
 
