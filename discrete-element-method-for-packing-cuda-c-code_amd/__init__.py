@@ -43,6 +43,7 @@ OPT_SKIN = 4
 OPT_PAIR_BLOCK = 5
 OPT_PAIR_GRID = 6
 OPT_PARTICLE_BLOCK = 7
+OPT_ADJ_ROLL_REG = 8
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0
@@ -121,6 +122,20 @@ class Deck(ctypes.Structure):
     ]
 
 
+class Obs(ctypes.Structure):
+    """Smooth-observable region (dem_obs): side walls moved in by `margin`, zlo < z < zhi,
+    logistic edges of width `eps`; soft contacts ramp over overlaps in [0, eps_c]."""
+    _fields_ = [("margin", c_double), ("zlo", c_double), ("zhi", c_double), ("eps", c_double),
+                ("eps_c", c_double)]
+
+
+def make_obs(zhi: float, margin: float = 2.0, zlo: float = 2.0, eps: float = 0.5,
+             eps_c: float = 1e-4) -> Obs:
+    o = Obs()
+    o.margin, o.zlo, o.zhi, o.eps, o.eps_c = margin, zlo, zhi, eps, eps_c
+    return o
+
+
 class DemError(RuntimeError):
     def __init__(self, code: int, what: str):
         super().__init__(f"{what}: status {code}")
@@ -143,7 +158,9 @@ EXPORTS = [
     "dem_restitution_for_damping", "dem_gen_diameters", "dem_gen_insert", "dem_gen_lattice",
     "dem_particle_props", "dem_make_grid", "dem_dev_alloc", "dem_dev_free", "dem_memcpy",
     "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_law_pair_vjp",
-    "dem_law_walls_eval", "dem_law_walls_vjp",
+    "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
+    "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
+    "dem_adj_release",
 ]
 
 
@@ -193,6 +210,13 @@ def lib() -> ctypes.CDLL:
         "dem_law_pair_vjp": ([P, P, P, P, P], c_int),
         "dem_law_walls_eval": ([P, P, P, P], c_int),
         "dem_law_walls_vjp": ([P, P, P, P, P, P], c_int),
+        "dem_set_material": ([P, c_int, P], c_int),
+        "dem_observe": ([P, P, P, P], c_int),
+        "dem_adj_forward": ([P, c_int, c_int, P, P, P], c_int),
+        "dem_adj_backward": ([P, P, P, P], c_int),
+        "dem_adj_state_grad": ([P, P, P, P], c_int),
+        "dem_adj_info": ([P, P], c_int),
+        "dem_adj_release": ([P], c_int),
         "dem_read_deck": ([ctypes.c_char_p, P], c_int),
         "dem_deck_material": ([P, P], c_int),
         "dem_write_restart": ([ctypes.c_char_p], c_int),
@@ -452,6 +476,49 @@ def config3(ninst: int = 256, seed: int = 4, nmin: int = 1000, nmax: int = 64000
                     meta={"ns": ns.tolist(), "mu": mus.tolist(), "e": es.tolist(), "E": Es.tolist()})
 
 
+def config4(n: int = 20_000, seed: int = 5, phi0: float = 0.3, side: float = 30.0) -> Workload:
+    """Differentiable form: `n` lognormal spheres (as config 1) inserted at random into a
+    `side` x `side` box at loose fill phi0; `settle_bed` pours them into a bed, and the
+    5,000-step window (checkpoints every 100 steps) starts from that bed after a tap."""
+    diam = gen_diameters(1, n, 0.1, 0.7, 1.3, seed)
+    vol = float(np.sum(math.pi / 6.0 * diam ** 3))
+    height = vol / phi0 / (side * side)
+    walls = make_walls(WALLS_BOX, (side, side, height))
+    pos = gen_insert(walls, 0.0, height, diam, seed + 100)
+    rad, rmass, inert = _props(diam)
+    mat = reduced_material(**MAT0)
+    dt = reduced_dt(DT0, MAT0["diam"])
+    cell = float(diam.max()) + SKIN
+    inst = make_instance(n, walls, height, cell, mat, dt)
+    z = np.zeros((n, 3))
+    return Workload("adjoint-20k-lognormal-box", [inst], pos, z.copy(), z.copy(), rad, rmass,
+                    inert, rebuild_interval=0, steps=5000,
+                    meta={"height": height, "phi0": phi0, "every": 100})
+
+
+def settle_bed(w: Workload, steps: int = 100_000, interval: int = 50, tap: float = 0.5,
+               seed: int = 6) -> Workload:
+    """Pour `w` into a bed on the GPU (hipGraph steps, fixed rebuild interval: no particle
+    moves half the skin in `interval` steps of free fall from the fill height), then 'tap'
+    it: every particle gets a random velocity of scale `tap` (in sqrt(g d)) and contact
+    histories restart at zero.  Returns the tapped bed as a new workload."""
+    ctx = Context(w.instances)
+    ctx.set_state(w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+    ctx.set_option(OPT_REBUILD_INTERVAL, interval)
+    ctx.step_graph((steps // interval) * interval)
+    st = ctx.state()
+    ctx.close()
+    dt = w.instances[0].dt
+    rng = np.random.default_rng(seed)
+    pdis = st["pdis"] + rng.normal(scale=tap * dt, size=st["pdis"].shape)
+    maxhz = float(st["pos"][:, 2].max())
+    meta = dict(w.meta)
+    meta.update(settle_steps=steps, tap=tap, bed_top=maxhz)
+    return Workload(w.name + "-settled", w.instances, st["pos"], pdis, st["angv"].copy(),
+                    w.rad, w.rmass, w.inert, rebuild_interval=w.rebuild_interval, steps=w.steps,
+                    meta=meta)
+
+
 # ----------------------------------------------------------------------------------------
 # Context
 # ----------------------------------------------------------------------------------------
@@ -592,6 +659,51 @@ class Context:
         _check(self._L.dem_get_forces(self._h, _ptr(f), _ptr(t), _ptr(c)), "dem_get_forces")
         return f, t, c
 
+    # ---- adjoint form ----
+    def set_material(self, k: int, mat: Material):
+        _check(self._L.dem_set_material(self._h, int(k), ctypes.byref(mat)), "dem_set_material")
+
+    def observe(self, obs: Obs):
+        """Smooth (phi, CN) per instance of the current state."""
+        phi = np.empty(self.ninst)
+        cn = np.empty(self.ninst)
+        _check(self._L.dem_observe(self._h, ctypes.byref(obs), _ptr(phi), _ptr(cn)), "dem_observe")
+        return phi, cn
+
+    def adj_forward(self, nsteps: int, every: int, obs: Obs):
+        """Checkpointed forward of `nsteps` steps; returns (phi, CN) per instance at the end."""
+        phi = np.empty(self.ninst)
+        cn = np.empty(self.ninst)
+        self._obs = obs
+        _check(self._L.dem_adj_forward(self._h, int(nsteps), int(every), ctypes.byref(obs),
+                                       _ptr(phi), _ptr(cn)), "dem_adj_forward")
+        return phi, cn
+
+    def adj_backward(self, gphi, gcn):
+        """Adjoint sweep; returns theta_bar[ninst, 10]: d/d(E*, mu, gamma_n, gamma_t, mu_r) of
+        the pair law, then of the wall law."""
+        gp = np.ascontiguousarray(np.broadcast_to(np.asarray(gphi, dtype=np.float64), (self.ninst,)))
+        gc = np.ascontiguousarray(np.broadcast_to(np.asarray(gcn, dtype=np.float64), (self.ninst,)))
+        th = np.empty((self.ninst, 10))
+        _check(self._L.dem_adj_backward(self._h, _ptr(gp), _ptr(gc), _ptr(th)), "dem_adj_backward")
+        return th
+
+    def adj_state_grad(self):
+        xb, vb, wb = (np.empty((self.n, 3)) for _ in range(3))
+        _check(self._L.dem_adj_state_grad(self._h, _ptr(xb), _ptr(vb), _ptr(wb)),
+               "dem_adj_state_grad")
+        return xb, vb, wb
+
+    def adj_info(self) -> dict:
+        out = np.zeros(6)
+        _check(self._L.dem_adj_info(self._h, _ptr(out)), "dem_adj_info")
+        return {"forward_ms": out[0], "backward_ms": out[1], "segments": int(out[2]),
+                "max_pairs": int(out[3]), "checkpoint_bytes": int(out[4]),
+                "tape_bytes": int(out[5])}
+
+    def adj_release(self):
+        _check(self._L.dem_adj_release(self._h), "dem_adj_release")
+
 
 def packing_stats(pos, rad, contacti, walls, margin: float = 2.0, top_margin: float = 3.0) -> dict:
     """Packing fraction and mean coordination number of a bed.
@@ -682,3 +794,11 @@ def deck_material(d: Deck) -> Material:
 
 def write_restart(path: str):
     _check(lib().dem_write_restart(path.encode()), "dem_write_restart")
+
+
+def __getattr__(name):
+    # `adjoint` (torch.autograd.Function, calibration) is imported on first use
+    if name == "adjoint":
+        import importlib
+        return importlib.import_module(__name__ + ".adjoint")
+    raise AttributeError(name)

@@ -150,7 +150,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
                                                     const unsigned *inv_old, unsigned *inv_new,
                                                     const InstDev *inst, InstDev one,
                                                     unsigned *wflag, unsigned *wlist,
-                                                    DevCtl *ctl, const unsigned *active)
+                                                    DevCtl *ctl, unsigned *perm,
+                                                    const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
@@ -176,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     if (s >= n) return;
     unsigned id = sorted[s].y;
     unsigned o = inv_old[id];
+    if (perm) perm[s] = o;   // recorded for the adjoint of the reorder
     const double *src = reinterpret_cast<const double *>(rec_old + o);
     double *dst = reinterpret_cast<double *>(rec_new + s);
 #pragma unroll
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
     unsigned cap, uint2 *ab, Hist *hist, unsigned *jt, const unsigned *inv_old,
     const uint2 *ab_old, const unsigned *off_old, const Hist *hist_old, unsigned n_old,
-    unsigned cap_old, DevCtl *ctl, const unsigned *active)
+    unsigned cap_old, DevCtl *ctl, unsigned *src, const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned a = blockIdx.x * kBlock + threadIdx.x;
@@ -309,15 +311,17 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
             if (pg >= cap) { over = true; return; }
             ab[pg] = make_uint2(a, b);
             Hist h = Hist{0.0, 0.0, 0.0, 0.0};
+            unsigned q = kEmpty;
             if (oa != kEmpty) {
                 unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
                 if (ob != kEmpty) {
                     unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
-                    unsigned q = find_second(ab_old, lo, hi, ob);
+                    q = find_second(ab_old, lo, hi, ob);
                     if (q != kEmpty) h = hist_old[q];
                 }
             }
             hist[pg] = h;
+            if (src) src[pg] = q;   // recorded for the adjoint of the remap
             pg++;
         } else {
             if (pl < cap) jt[pl] = b;
@@ -377,7 +381,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const unsigned *npairs,
                                                        const PRec *rec, Hist *hist, POut *out,
                                                        const InstDev *inst, LawParams pp,
-                                                       double dt)
+                                                       double dt, Hist *tape_hist)
 {
     const unsigned np = *npairs;
     for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
@@ -390,6 +394,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                 b5 = ld2(B + 10);
         const double *H = reinterpret_cast<const double *>(hist + p);
         double2 h0 = ld2(H), h1 = ld2(H + 2);
+        if (tape_hist) {   // incoming history, for the adjoint
+            double *T = reinterpret_cast<double *>(tape_hist + p);
+            st2(T, h0.x, h0.y);
+            st2(T + 2, h1.x, h1.y);
+        }
         LawParams P = pp;
         double tdt = dt;
         if (BATCHED) {
@@ -422,7 +431,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
                                                   const unsigned *wlist, const DevCtl *ctl,
                                                   const InstDev *inst, InstDev one,
-                                                  double *wforce)
+                                                  double *wforce, Hist *tape_whist)
 {
     const unsigned nw = ctl->nwall;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
@@ -433,6 +442,11 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
         const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
         double *WH = reinterpret_cast<double *>(whist + a);
         double2 w0 = ld2(WH), w1 = ld2(WH + 2);
+        if (tape_whist) {
+            double *T = reinterpret_cast<double *>(tape_whist + a);
+            st2(T, w0.x, w0.y);
+            st2(T + 2, w1.x, w1.y);
+        }
         d3 fw, tw, hw;
         double fow;
         int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
@@ -462,7 +476,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
     const unsigned *toff, const unsigned *jt, const POut *out, unsigned n, const InstDev *inst,
-    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl)
+    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec)
 {
     const unsigned np = ctl->npairs;
     unsigned a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -500,6 +514,14 @@ __global__ __launch_bounds__(kBlock) void k_particle(
                 r5 = ld2(R + 10), r6 = ld2(R + 12);
         double qz = R[14];
         uint2 tail = *reinterpret_cast<const uint2 *>(R + 15);
+        if (tape_rec) {   // the record before the update, for the adjoint
+            double *T = reinterpret_cast<double *>(tape_rec + a);
+            st2(T, r0.x, r0.y); st2(T + 2, r1.x, r1.y); st2(T + 4, r2.x, r2.y);
+            st2(T + 6, r3.x, r3.y); st2(T + 8, r4.x, r4.y); st2(T + 10, r5.x, r5.y);
+            st2(T + 12, r6.x, r6.y);
+            T[14] = qz;
+            *reinterpret_cast<uint2 *>(T + 15) = tail;
+        }
         const InstDev &I = BATCHED ? inst[tail.x] : one;
         d3 x = d3{r0.x, r0.y, r1.x};
         double rad = r1.y;

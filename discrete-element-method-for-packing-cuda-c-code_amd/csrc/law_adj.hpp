@@ -25,9 +25,20 @@ DEM_HD void acc3s(d3 &a, double s, d3 b) { a.x += s * b.x; a.y += s * b.y; a.z +
 // Reverse of law_core.  Output adjoints: fib (force on i), ftb_ext (tangential force, from
 // the caller's torques), mrb (rolling moment), houtb (new history).  Accumulates into nb,
 // dnb, vrelb, wrelb, hinb, Pb.
+//
+// roll_c regularises the rolling moment's linearisation.  The moment -mu_r fcn w/|w| has
+// magnitude fixed by fcn, and its derivative across w is mu_r fcn / |w| (I - w w^T / |w|^2),
+// which grows without bound as w -> 0.  Once |w| is below the change one step of that
+// moment makes, w_c = mu_r fcn dt^2 (1/I_a + 1/I_b), the moment flips sign from step to
+// step (chatter about rolling contact) and the exact linearisation amplifies perturbations
+// by |1 - w_c / |w|| per step.  The caller passes roll_c = k dt^2 (1/I_a + 1/I_b) (one
+// inertia for a wall), and the derivative uses max(|w|, roll_c mu_r fcn) = max(|w|, k w_c):
+// exact away from chatter, at most critically damped (k = 1) within it.  roll_c = 0 is the
+// exact reverse mode.
 DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel, d3 hin,
                          double dt, const LawParams &P, d3 fib, d3 ftb_ext, d3 mrb, d3 houtb,
-                         d3 &nb, double &dnb, d3 &vrelb, d3 &wrelb, d3 &hinb, LawParamsBar &Pb)
+                         d3 &nb, double &dnb, d3 &vrelb, d3 &wrelb, d3 &hinb, LawParamsBar &Pb,
+                         double roll_c = 0.0)
 {
     // ---- forward recompute ----
     double vns = dot3(vrel, n);
@@ -88,20 +99,21 @@ DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel
     }
     double wrm = sqrt(dot3(wrel, wrel));
     bool roll = wrm > kXSmall;
-    double mcoef = roll ? P.ur * fcn / wrm : 0.0;
 
     // ---- reverse ----
     d3 ftb = add3(ftb_ext, fib);   // fi = fn + ft
     d3 fnb = fib;
     double fcnb = 0.0;
     d3 vtb = d3{0.0, 0.0, 0.0}, vnb = d3{0.0, 0.0, 0.0};
-    if (roll) {   // mr = -mcoef * wrel, mcoef = ur fcn / |wrel|
-        double mcoefb = -dot3(mrb, wrel);
-        acc3s(wrelb, -mcoef, mrb);
-        Pb.ur += mcoefb * fcn / wrm;
-        fcnb += mcoefb * P.ur / wrm;
-        double wrmb = -mcoefb * mcoef / wrm;
-        acc3s(wrelb, wrmb / wrm, wrel);
+    if (roll) {   // mr = -ur fcn w_hat, w_hat = wrel / |wrel|
+        d3 wh = scale3(wrel, 1.0 / wrm);
+        double mrw = dot3(mrb, wh);
+        Pb.ur += -mrw * fcn;
+        fcnb += -mrw * P.ur;
+        double wfl = roll_c * P.ur * fcn;
+        double mc = P.ur * fcn / (wrm > wfl ? wrm : wfl);   // = mcoef unless regularised
+        acc3s(wrelb, -mc, mrb);   // d w_hat: (I - w_hat w_hat^T) / |wrel|
+        acc3s(wrelb, mc * mrw, wh);
     }
     if (tang) {
         double dtmaxb = 0.0, omb = 0.0, htm2b = 0.0;
@@ -200,7 +212,7 @@ DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel
 DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, double rb,
                          double ma, double mb, d3 hin, double dt, const LawParams &P, d3 fab,
                          d3 tab, d3 tbb, d3 houtb, d3 &xab, d3 &xbb, d3 &vab, d3 &vbb, d3 &wab,
-                         d3 &wbb, d3 &hinb, LawParamsBar &Pb)
+                         d3 &wbb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0)
 {
     d3 dx = sub3(xb, xa);
     double dist = sqrt(dot3(dx, dx));
@@ -224,7 +236,7 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
     double dnb = 0.0;
     d3 vrelb = d3{0.0, 0.0, 0.0}, wrelb = d3{0.0, 0.0, 0.0};
     law_core_adj(n, dn, rs, ms, vrel, wrel, hin, dt, P, fab, ftb, mrb, houtb, nb, dnb, vrelb,
-                 wrelb, hinb, Pb);
+                 wrelb, hinb, Pb, roll_c);
     acc3(wab, wrelb);   // wrel = wa - wb
     wbb.x -= wrelb.x; wbb.y -= wrelb.y; wbb.z -= wrelb.z;
     acc3(vbb, vrelb);   // vrel = (vb - va) - c
@@ -245,7 +257,7 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
 // torque), houtb (the history slot's new value).  Accumulates into xb, vb, wb, hinb, Pb.
 DEM_HD void law_walls_adj(const WallGeom &g, d3 x, d3 v, d3 w, double r, double m, d3 hin,
                           double dt, const LawParams &P, d3 fwb, d3 twb, d3 houtb, d3 &xb,
-                          d3 &vb, d3 &wb, d3 &hinb, LawParamsBar &Pb)
+                          d3 &vb, d3 &wb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0)
 {
     int nw = wall_count(g);
     int prim = -1;
@@ -276,7 +288,8 @@ DEM_HD void law_walls_adj(const WallGeom &g, d3 x, d3 v, d3 w, double r, double 
         double dnb = 0.0;
         d3 vrelb = d3{0.0, 0.0, 0.0}, wrelb = d3{0.0, 0.0, 0.0}, hkb = d3{0.0, 0.0, 0.0};
         law_core_adj(n, dn, r, m, vrel, w, hk, dt, P, fwb, ftb, twb,
-                     k == prim ? houtb : d3{0.0, 0.0, 0.0}, nb, dnb, vrelb, wrelb, hkb, Pb);
+                     k == prim ? houtb : d3{0.0, 0.0, 0.0}, nb, dnb, vrelb, wrelb, hkb, Pb,
+                     roll_c);
         if (k == prim) acc3(hinb, hkb);
         acc3(wb, wrelb);   // wrel = w
         vb.x -= vrelb.x; vb.y -= vrelb.y; vb.z -= vrelb.z;   // vrel = -v - c

@@ -11,6 +11,7 @@
 
 #include "../../include/dem.h"
 #include "kernels.hpp"
+#include "adjoint.hpp"
 #include "law_adj.hpp"
 #include "sort.hpp"
 
@@ -214,6 +215,13 @@ struct dem_ctx {
     hipGraph_t graph[2] = {nullptr, nullptr};
     int graph_k = 0;
     unsigned graph_np_grid = 0;
+    // adjoint recording (null when not recording): pre-step records, incoming histories,
+    // rebuild permutation and history-source map
+    PRec *tape_rec = nullptr;
+    Hist *tape_hist = nullptr, *tape_whist = nullptr;
+    unsigned *tape_perm = nullptr, *tape_src = nullptr;
+    struct AdjData *adj = nullptr;
+    double adj_roll_k = 1.0;   // rolling-moment regularisation of the reverse pass (0: exact)
     unsigned long long buf_gen = 0;     // bumped whenever a pair buffer is reallocated
     unsigned long long graph_gen = ~0ull;
 };
@@ -285,12 +293,13 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->rc], c->whist[nr],
                            c->inv[c->rc], c->inv[nr], (const InstDev *)c->dinst, one, c->wflag,
-                           c->wlist, c->ctl, active);
+                           c->wlist, c->ctl, c->tape_perm, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
                            c->whist[c->rc], c->whist[nr], c->inv[c->rc], c->inv[nr],
-                           (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl, active);
+                           (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
+                           c->tape_perm, active);
     c->rc = nr;
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
@@ -326,13 +335,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
-                           std::min(c->cap[ol], c->out_cap), c->ctl, active);
+                           std::min(c->cap[ol], c->out_cap), c->ctl, c->tape_src, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
-                           std::min(c->cap[ol], c->out_cap), c->ctl, active);
+                           std::min(c->cap[ol], c->out_cap), c->ctl, c->tape_src, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
                        active);
@@ -343,8 +352,12 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     return DEM_OK;
 }
 
+// Capacity hook of a recorded rebuild: called with the new pair total before the fill, it
+// sizes the history-source tape and points c->tape_src at it.
+typedef int (*pair_hook_fn)(dem_ctx *c, void *arg, unsigned np);
+
 // Host-driven rebuild: count, read the pair total, grow capacities if needed, fill.
-int rebuild_host(dem_ctx *c)
+int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullptr)
 {
     int rc = rebuild_count(c, nullptr);
     if (rc) return rc;
@@ -364,6 +377,7 @@ int rebuild_host(dem_ctx *c)
     // k_check_cap clamped npairs to the capacity seen before growing; store the real total
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->npairs, np);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->overflow, 0u);
+    if (hook && (rc = hook(c, hook_arg, np))) return rc;
     if ((rc = rebuild_fill(c, nullptr))) return rc;
     c->npairs = np;
     c->rebuilds++;
@@ -383,31 +397,37 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
     if (c->batched) {
         hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
-                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt,
+                           c->tape_hist);
         hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
                            (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
-                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
+                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce,
+                           c->tape_whist);
         if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
+                           c->tape_rec);
     } else {
         hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
                            (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
-                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt);
+                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt,
+                           c->tape_hist);
         hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
                            (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
-                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce);
+                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce,
+                           c->tape_whist);
         if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc],
                            (const unsigned *)c->toff, (const unsigned *)c->jt,
                            (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl);
+                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
+                           c->tape_rec);
     }
     if (end) (void)hipEventRecord(end, st);
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d);
@@ -479,9 +499,12 @@ int regrid(dem_ctx *c, double skin)
     return DEM_OK;
 }
 
+void adj_free(dem_ctx *c);
+
 void free_ctx(dem_ctx *c)
 {
     if (!c) return;
+    adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
@@ -720,6 +743,10 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     }
+    case DEM_OPT_ADJ_ROLL_REG:
+        if (!(value >= 0.0)) return DEM_ERR_INVALID;
+        c->adj_roll_k = value;
+        return DEM_OK;
     case DEM_OPT_PAIR_GRID: {
         if (!(value >= 1.0 && value <= 256.0)) return DEM_ERR_INVALID;
         c->pair_grid_per_cu = (unsigned)value;
@@ -1592,6 +1619,626 @@ void outputporositycn(uint numBodies, uint *contacti, double maxhz, double world
                        (const unsigned *)contacti, g_words + 2);
     check_launch("outputporositycn");
     DEM_HIP_V(hipMemcpy(pcntot, g_words + 2, sizeof(uint), hipMemcpyDeviceToHost));
+}
+
+}  // extern "C"
+
+// =========================================================================================
+// Adjoint form: checkpointed forward, segment recompute with tapes, reverse sweep
+// =========================================================================================
+
+namespace {
+
+#define ADJ_FREE(p)                     \
+    do {                                \
+        if (p) (void)hipFree((void *)p); \
+        p = nullptr;                    \
+    } while (0)
+
+// Full dynamic state of a context at a segment boundary (the list included, so the
+// recompute replays the forward bit for bit).
+struct AdjSnap {
+    PRec *rec = nullptr;
+    Hist *whist = nullptr, *hist = nullptr;
+    unsigned *inv = nullptr, *off = nullptr, *toff = nullptr, *jt = nullptr, *wflag = nullptr,
+             *wlist = nullptr;
+    uint2 *ab = nullptr;
+    DevCtl *ctl = nullptr;
+    size_t pcap = 0;
+    unsigned npairs = 0, n_list = 0, since = 0, last_flag = 0;
+    bool have_list = false, need_rebuild = false, valid = false;
+    unsigned long long steps = 0, rebuilds = 0;
+};
+
+// A pair-list version made by a rebuild inside a recomputed segment, with that rebuild's
+// slot permutation and history-source map.
+struct AdjList {
+    uint2 *ab = nullptr;
+    unsigned *off = nullptr, *toff = nullptr, *jt = nullptr, *wflag = nullptr, *perm = nullptr,
+             *src = nullptr;
+    size_t pcap = 0;
+    unsigned np = 0;
+};
+
+struct ListView {
+    const uint2 *ab;
+    const unsigned *off, *toff, *jt, *wflag, *perm, *src;
+    unsigned np;
+};
+
+}  // namespace
+
+struct AdjData {
+    int nsteps = 0, K = 0, nseg = 0;
+    unsigned n = 0, max_np = 0;
+    std::vector<AdjSnap> ck;          // nseg + 1 boundaries
+    PRec *t_rec = nullptr;            // K records per slot
+    Hist *t_hist = nullptr;           // K x max_np incoming pair histories
+    Hist *t_whist = nullptr;          // K incoming wall histories per slot
+    std::vector<AdjList> pool;        // list versions of the segment being reversed
+    SAdj *lam[2] = {nullptr, nullptr};
+    double *hb[2] = {nullptr, nullptr};   // 4 doubles per pair
+    double *ftb = nullptr, *thacc = nullptr;
+    PAdj *pa = nullptr;
+    double *pobs = nullptr, *sobs = nullptr, *part = nullptr, *sums = nullptr, *coef = nullptr;
+    size_t pobs_cap = 0;
+    unsigned *range = nullptr;
+    ObsDev obs{};
+    std::vector<double> vreg, wsum, csum;
+    int cur = 0, tape_K = 0;
+    bool have_grad = false;
+    float fwd_ms = 0.f, bwd_ms = 0.f;
+};
+
+namespace {
+
+void snap_free(AdjSnap &S)
+{
+    ADJ_FREE(S.rec); ADJ_FREE(S.whist); ADJ_FREE(S.hist); ADJ_FREE(S.inv); ADJ_FREE(S.off);
+    ADJ_FREE(S.toff); ADJ_FREE(S.jt); ADJ_FREE(S.wflag); ADJ_FREE(S.wlist); ADJ_FREE(S.ab);
+    ADJ_FREE(S.ctl);
+    S.pcap = 0;
+    S.valid = false;
+}
+
+void list_free(AdjList &L)
+{
+    ADJ_FREE(L.ab); ADJ_FREE(L.off); ADJ_FREE(L.toff); ADJ_FREE(L.jt); ADJ_FREE(L.wflag);
+    ADJ_FREE(L.perm); ADJ_FREE(L.src);
+    L.pcap = 0;
+}
+
+void adj_release(AdjData *A)
+{
+    if (!A) return;
+    for (AdjSnap &S : A->ck) snap_free(S);
+    for (AdjList &L : A->pool) list_free(L);
+    ADJ_FREE(A->t_rec); ADJ_FREE(A->t_hist); ADJ_FREE(A->t_whist);
+    ADJ_FREE(A->lam[0]); ADJ_FREE(A->lam[1]); ADJ_FREE(A->hb[0]); ADJ_FREE(A->hb[1]);
+    ADJ_FREE(A->ftb); ADJ_FREE(A->thacc); ADJ_FREE(A->pa);
+    ADJ_FREE(A->pobs); ADJ_FREE(A->sobs); ADJ_FREE(A->part); ADJ_FREE(A->sums); ADJ_FREE(A->coef);
+    ADJ_FREE(A->range);
+    delete A;
+}
+
+void adj_free(dem_ctx *c)
+{
+    if (c && c->adj) {
+        (void)hipStreamSynchronize(c->stream);
+        adj_release(c->adj);
+        c->adj = nullptr;
+    }
+}
+
+int snap_save(dem_ctx *c, AdjSnap &S)
+{
+    hipStream_t st = c->stream;
+    size_t n = c->n, np = c->have_list ? c->npairs : 0;
+    if (!S.rec) {
+        DEM_HIP(hipMalloc(&S.rec, sizeof(PRec) * n));
+        DEM_HIP(hipMalloc(&S.whist, sizeof(Hist) * n));
+        DEM_HIP(hipMalloc(&S.inv, sizeof(unsigned) * n));
+        DEM_HIP(hipMalloc(&S.off, sizeof(unsigned) * (n + 1)));
+        DEM_HIP(hipMalloc(&S.toff, sizeof(unsigned) * (n + 1)));
+        DEM_HIP(hipMalloc(&S.wflag, sizeof(unsigned) * n));
+        DEM_HIP(hipMalloc(&S.wlist, sizeof(unsigned) * n));
+        DEM_HIP(hipMalloc(&S.ctl, sizeof(DevCtl)));
+    }
+    if (S.pcap < std::max<size_t>(np, 1)) {
+        ADJ_FREE(S.ab); ADJ_FREE(S.hist); ADJ_FREE(S.jt);
+        S.pcap = std::max<size_t>(np, 1);
+        DEM_HIP(hipMalloc(&S.ab, sizeof(uint2) * S.pcap));
+        DEM_HIP(hipMalloc(&S.hist, sizeof(Hist) * S.pcap));
+        DEM_HIP(hipMalloc(&S.jt, sizeof(unsigned) * S.pcap));
+    }
+    DEM_HIP(hipMemcpyAsync(S.rec, c->rec[c->rc], sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(S.whist, c->whist[c->rc], sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(S.inv, c->inv[c->rc], sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(S.ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
+    if (c->have_list) {
+        DEM_HIP(hipMemcpyAsync(S.off, c->off[c->lc], sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(S.toff, c->toff, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(S.wflag, c->wflag, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(S.wlist, c->wlist, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+        if (np) {
+            DEM_HIP(hipMemcpyAsync(S.ab, c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(S.hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(S.jt, c->jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    S.npairs = (unsigned)np;
+    S.n_list = c->n_list;
+    S.since = c->since;
+    S.last_flag = c->last_flag;
+    S.have_list = c->have_list;
+    S.need_rebuild = c->need_rebuild;
+    S.steps = c->steps;
+    S.rebuilds = c->rebuilds;
+    S.valid = true;
+    return DEM_OK;
+}
+
+int snap_restore(dem_ctx *c, const AdjSnap &S)
+{
+    hipStream_t st = c->stream;
+    size_t n = c->n, np = S.npairs;
+    int rc;
+    DEM_HIP(hipStreamSynchronize(st));
+    if (np > c->cap[c->lc] && (rc = alloc_pairs(c, c->lc, (unsigned)(np * 3 / 2 + 1024)))) return rc;
+    if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
+    DEM_HIP(hipMemcpyAsync(c->rec[c->rc], S.rec, sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(c->whist[c->rc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(c->inv[c->rc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
+    if (S.have_list) {
+        DEM_HIP(hipMemcpyAsync(c->off[c->lc], S.off, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(c->toff, S.toff, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(c->wflag, S.wflag, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+        DEM_HIP(hipMemcpyAsync(c->wlist, S.wlist, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+        if (np) {
+            DEM_HIP(hipMemcpyAsync(c->ab[c->lc], S.ab, sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(c->hist[c->lc], S.hist, sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    c->npairs = S.npairs;
+    c->n_list = S.n_list;
+    c->since = S.since;
+    c->last_flag = S.last_flag;
+    c->have_list = S.have_list;
+    c->need_rebuild = S.need_rebuild;
+    c->steps = S.steps;
+    c->rebuilds = S.rebuilds;
+    drop_graph(c);
+    return DEM_OK;
+}
+
+int list_reserve(AdjList &L, size_t n, size_t np)
+{
+    if (!L.off) {
+        DEM_HIP(hipMalloc(&L.off, sizeof(unsigned) * (n + 1)));
+        DEM_HIP(hipMalloc(&L.toff, sizeof(unsigned) * (n + 1)));
+        DEM_HIP(hipMalloc(&L.wflag, sizeof(unsigned) * n));
+        DEM_HIP(hipMalloc(&L.perm, sizeof(unsigned) * n));
+    }
+    if (L.pcap < std::max<size_t>(np, 1)) {
+        ADJ_FREE(L.ab); ADJ_FREE(L.jt); ADJ_FREE(L.src);
+        L.pcap = std::max<size_t>(np, 1);
+        DEM_HIP(hipMalloc(&L.ab, sizeof(uint2) * L.pcap));
+        DEM_HIP(hipMalloc(&L.jt, sizeof(unsigned) * L.pcap));
+        DEM_HIP(hipMalloc(&L.src, sizeof(unsigned) * L.pcap));
+    }
+    return DEM_OK;
+}
+
+struct HookArg {
+    AdjList *L;
+    size_t n;
+};
+
+int record_hook(dem_ctx *c, void *arg, unsigned np)
+{
+    HookArg *h = static_cast<HookArg *>(arg);
+    int rc = list_reserve(*h->L, h->n, np);
+    if (rc) return rc;
+    c->tape_src = h->L->src;
+    return DEM_OK;
+}
+
+// One step of dem_step's schedule.  With `L` non-null a rebuild records its permutation,
+// history sources and resulting list into L (*rebuilt set); with `tape_step` >= 0 the force
+// step writes the tapes of that step.
+int adj_step(dem_ctx *c, AdjData *A, int tape_step, AdjList *L, bool *rebuilt)
+{
+    int rc;
+    if (rebuilt) *rebuilt = false;
+    if (c->need_rebuild || (c->interval > 0 && c->since >= (unsigned)c->interval)) {
+        if (L) {
+            if ((rc = list_reserve(*L, c->n, 1))) return rc;
+            c->tape_perm = L->perm;
+            HookArg h{L, c->n};
+            rc = rebuild_host(c, record_hook, &h);
+            c->tape_perm = nullptr;
+            c->tape_src = nullptr;
+            if (rc) return rc;
+            hipStream_t st = c->stream;
+            size_t n = c->n, np = c->npairs;
+            DEM_HIP(hipMemcpyAsync(L->off, c->off[c->lc], sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(L->toff, c->toff, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(L->wflag, c->wflag, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+            if (np) {
+                DEM_HIP(hipMemcpyAsync(L->ab, c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
+                DEM_HIP(hipMemcpyAsync(L->jt, c->jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
+            }
+            L->np = (unsigned)np;
+            if (rebuilt) *rebuilt = true;
+        } else if ((rc = rebuild_host(c))) {
+            return rc;
+        }
+    }
+    if (A && c->npairs > A->max_np && tape_step >= 0) return DEM_ERR_CAPACITY;
+    if (tape_step >= 0) {
+        c->tape_rec = A->t_rec + (size_t)tape_step * c->n;
+        c->tape_hist = A->t_hist + (size_t)tape_step * A->max_np;
+        c->tape_whist = A->t_whist + (size_t)tape_step * c->n;
+    }
+    launch_force_step(c, pair_grid(c, c->npairs));
+    c->tape_rec = nullptr;
+    c->tape_hist = nullptr;
+    c->tape_whist = nullptr;
+    DEM_HIP(hipGetLastError());
+    c->steps++;
+    c->since++;
+    if (c->interval == 0) {
+        if ((rc = sync_mirror(c))) return rc;
+        c->need_rebuild = c->last_flag != 0u;
+    }
+    return DEM_OK;
+}
+
+int obs_alloc(dem_ctx *c, AdjData *A)
+{
+    size_t n = c->n;
+    if (!A->sobs) {
+        DEM_HIP(hipMalloc(&A->sobs, sizeof(double) * 8 * n));
+        DEM_HIP(hipMalloc(&A->part, sizeof(double) * 3 * n));
+        DEM_HIP(hipMalloc(&A->sums, sizeof(double) * 3 * c->ninst));
+        DEM_HIP(hipMalloc(&A->coef, sizeof(double) * 3 * c->ninst));
+        DEM_HIP(hipMalloc(&A->range, sizeof(unsigned) * (c->ninst + 1)));
+        std::vector<unsigned> r(c->ninst + 1);
+        for (int k = 0; k < c->ninst; k++) r[k] = c->hinst[k].p_off;
+        r[c->ninst] = (unsigned)n;
+        DEM_HIP(hipMemcpy(A->range, r.data(), sizeof(unsigned) * r.size(), hipMemcpyHostToDevice));
+    }
+    size_t np = std::max<size_t>(c->npairs, 1);
+    if (A->pobs_cap < np) {
+        ADJ_FREE(A->pobs);
+        A->pobs_cap = np * 3 / 2 + 1024;
+        DEM_HIP(hipMalloc(&A->pobs, sizeof(double) * 4 * A->pobs_cap));
+    }
+    return DEM_OK;
+}
+
+double region_volume(const InstDev &I, const ObsDev &o)
+{
+    double h = std::max(o.zhi - o.zlo, 0.0);
+    if (I.walls.kind == 0)
+        return std::max(2.0 * (I.walls.hx - o.margin), 0.0) * std::max(2.0 * (I.walls.hy - o.margin), 0.0) * h;
+    double r = std::max(I.walls.radius - o.margin, 0.0);
+    return 3.14159265358979323846 * r * r * h;
+}
+
+// Smooth observables of the current state (per instance): fills A's per-slot terms and the
+// sums; phi/cn may be null.
+int observe(dem_ctx *c, AdjData *A, const ObsDev &o, double *phi, double *cn)
+{
+    int rc;
+    if (!c->have_list && (rc = rebuild_host(c))) return rc;
+    if ((rc = obs_alloc(c, A))) return rc;
+    hipStream_t st = c->stream;
+    unsigned n = c->n, np = c->npairs;
+    InstDev one = c->hinst[0];
+    const PRec *rec = c->rec[c->rc];
+    unsigned pg = std::max(1u, std::min(cdiv(np, kBlock), (unsigned)c->cus * 16u));
+    if (c->batched)
+        hipLaunchKernelGGL(k_obs_pair<true>, dim3(pg), dim3(kBlock), 0, st, (const uint2 *)c->ab[c->lc],
+                           np, rec, (const InstDev *)c->dinst, one, o, A->pobs);
+    else
+        hipLaunchKernelGGL(k_obs_pair<false>, dim3(pg), dim3(kBlock), 0, st, (const uint2 *)c->ab[c->lc],
+                           np, rec, (const InstDev *)c->dinst, one, o, A->pobs);
+    if (c->batched)
+        hipLaunchKernelGGL(k_obs_slot<true>, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, rec,
+                           (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
+                           (const unsigned *)c->jt, np, (const double *)A->pobs,
+                           (const InstDev *)c->dinst, one, o, n, A->sobs, A->part);
+    else
+        hipLaunchKernelGGL(k_obs_slot<false>, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, st, rec,
+                           (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
+                           (const unsigned *)c->jt, np, (const double *)A->pobs,
+                           (const InstDev *)c->dinst, one, o, n, A->sobs, A->part);
+    hipLaunchKernelGGL(k_obs_reduce, dim3(c->ninst), dim3(kBlock), 0, st, (const double *)A->part,
+                       (const unsigned *)A->range, A->sums);
+    DEM_HIP(hipGetLastError());
+    std::vector<double> sums(3 * (size_t)c->ninst);
+    DEM_HIP(hipMemcpyAsync(sums.data(), A->sums, sizeof(double) * sums.size(), hipMemcpyDeviceToHost, st));
+    DEM_HIP(hipStreamSynchronize(st));
+    A->obs = o;
+    A->vreg.assign(c->ninst, 0.0);
+    A->wsum.assign(c->ninst, 0.0);
+    A->csum.assign(c->ninst, 0.0);
+    for (int k = 0; k < c->ninst; k++) {
+        double v = region_volume(c->hinst[k], o);
+        A->vreg[k] = v;
+        A->wsum[k] = sums[3 * k + 1];
+        A->csum[k] = sums[3 * k + 2];
+        if (phi) phi[k] = v > 0.0 ? sums[3 * k] / v : 0.0;
+        if (cn) cn[k] = sums[3 * k + 1] > 0.0 ? sums[3 * k + 2] / sums[3 * k + 1] : 0.0;
+    }
+    return DEM_OK;
+}
+
+bool obs_valid(const dem_obs *o)
+{
+    return o && o->eps > 0.0 && o->eps_c > 0.0 && o->zhi > o->zlo && o->margin >= 0.0;
+}
+
+ObsDev obs_dev(const dem_obs *o)
+{
+    return ObsDev{o->margin, o->zlo, o->zhi, o->eps, o->eps_c};
+}
+
+AdjData *adj_get(dem_ctx *c)
+{
+    if (!c->adj) c->adj = new AdjData();
+    return c->adj;
+}
+
+// Reverse sweep over the recomputed segment [ck[seg], ck[seg+1]): `ks` steps.
+int reverse_segment(dem_ctx *c, AdjData *A, int ks, const std::vector<ListView> &views,
+                    const std::vector<int> &step_ver, const std::vector<char> &step_reb, int &hc)
+{
+    hipStream_t st = c->stream;
+    unsigned n = c->n;
+    InstDev one = c->hinst[0];
+    unsigned gn = cdiv(n, kBlock);
+    for (int j = ks - 1; j >= 0; j--) {
+        const ListView &V = views[step_ver[j]];
+        const PRec *tr = A->t_rec + (size_t)j * n;
+        const Hist *th = A->t_hist + (size_t)j * A->max_np;
+        const Hist *twh = A->t_whist + (size_t)j * n;
+        if (c->batched)
+            hipLaunchKernelGGL(k_adj_particle<true>, dim3(gn), dim3(kBlock), 0, st, tr, twh, V.wflag,
+                               A->lam[A->cur], A->ftb, A->thacc, n, (const InstDev *)c->dinst, one,
+                               c->adj_roll_k);
+        else
+            hipLaunchKernelGGL(k_adj_particle<false>, dim3(gn), dim3(kBlock), 0, st, tr, twh, V.wflag,
+                               A->lam[A->cur], A->ftb, A->thacc, n, (const InstDev *)c->dinst, one,
+                               c->adj_roll_k);
+        unsigned pg = std::max(1u, std::min(cdiv(V.np, kBlock), (unsigned)c->cus * 16u));
+        if (c->batched)
+            hipLaunchKernelGGL(k_adj_pair<true>, dim3(pg), dim3(kBlock), 0, st, V.ab, V.np, tr, th,
+                               (const double *)A->ftb, A->hb[hc], A->pa, (const InstDev *)c->dinst,
+                               one.pp, one.dt, c->adj_roll_k);
+        else
+            hipLaunchKernelGGL(k_adj_pair<false>, dim3(pg), dim3(kBlock), 0, st, V.ab, V.np, tr, th,
+                               (const double *)A->ftb, A->hb[hc], A->pa, (const InstDev *)c->dinst,
+                               one.pp, one.dt, c->adj_roll_k);
+        hipLaunchKernelGGL(k_adj_gather, dim3(gn), dim3(kBlock), 0, st, A->lam[A->cur],
+                           (const PAdj *)A->pa, V.off, V.toff, V.jt, V.np, tr, A->thacc, n);
+        if (step_reb[j]) {
+            const ListView &P = views[step_ver[j] - 1];
+            hipLaunchKernelGGL(k_adj_unperm, dim3(gn), dim3(kBlock), 0, st,
+                               (const SAdj *)A->lam[A->cur], A->lam[1 - A->cur], V.perm, n);
+            A->cur = 1 - A->cur;
+            size_t zn = 4 * (size_t)std::max(P.np, 1u);
+            hipLaunchKernelGGL(k_zero_d, dim3(std::max(1u, std::min(cdiv(zn, kBlock), 4096u))), dim3(kBlock),
+                               0, st, A->hb[1 - hc], zn);
+            hipLaunchKernelGGL(k_adj_unremap, dim3(std::max(1u, cdiv(V.np, kBlock))), dim3(kBlock), 0, st,
+                               (const double *)A->hb[hc], A->hb[1 - hc], V.src, V.np, P.np);
+            hc = 1 - hc;
+        }
+        DEM_HIP(hipGetLastError());
+    }
+    return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dem_set_material(dem_ctx *c, int instance, const dem_material *mat)
+{
+    if (!c || !mat || instance < 0 || instance >= c->ninst) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    InstDev &I = c->hinst[instance];
+    I.pp = law_from(mat->p);
+    I.pw = law_from(mat->w);
+    DEM_HIP(hipMemcpy(c->dinst + instance, &I, sizeof(InstDev), hipMemcpyHostToDevice));
+    drop_graph(c);
+    return DEM_OK;
+}
+
+int dem_observe(dem_ctx *c, const dem_obs *obs, double *phi, double *cn)
+{
+    if (!c || !obs_valid(obs) || c->n != c->cap_n) return DEM_ERR_INVALID;
+    return observe(c, adj_get(c), obs_dev(obs), phi, cn);
+}
+
+int dem_adj_forward(dem_ctx *c, int nsteps, int every, const dem_obs *obs, double *phi, double *cn)
+{
+    if (!c || nsteps < 1 || every < 1 || !obs_valid(obs) || c->n != c->cap_n) return DEM_ERR_INVALID;
+    int rc;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    drop_graph(c);
+    AdjData *A = adj_get(c);
+    for (AdjSnap &S : A->ck) snap_free(S);
+    A->ck.clear();
+    A->have_grad = false;
+    A->nsteps = nsteps;
+    A->K = every;
+    A->nseg = (nsteps + every - 1) / every;
+    A->n = c->n;
+    A->ck.resize(A->nseg + 1);
+    unsigned max_np = c->have_list ? c->npairs : 0;
+    DEM_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < A->nseg; s++) {
+        if ((rc = snap_save(c, A->ck[s]))) return rc;
+        int ks = std::min(every, nsteps - s * every);
+        for (int j = 0; j < ks; j++) {
+            if ((rc = adj_step(c, nullptr, -1, nullptr, nullptr))) return rc;
+            max_np = std::max(max_np, c->npairs);
+        }
+    }
+    if ((rc = snap_save(c, A->ck[A->nseg]))) return rc;
+    A->max_np = max_np;
+    DEM_HIP(hipEventRecord(c->ev1, c->stream));
+    DEM_HIP(hipEventSynchronize(c->ev1));
+    DEM_HIP(hipEventElapsedTime(&A->fwd_ms, c->ev0, c->ev1));
+    c->last_ms = A->fwd_ms;
+    return observe(c, A, obs_dev(obs), phi, cn);
+}
+
+int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *theta_bar)
+{
+    if (!c || !c->adj || c->adj->ck.empty() || !gphi || !gcn) return DEM_ERR_INVALID;
+    AdjData *A = c->adj;
+    if (c->n != A->n) return DEM_ERR_INVALID;
+    int rc;
+    hipStream_t st = c->stream;
+    size_t n = c->n, K = (size_t)A->K, mnp = std::max<size_t>(A->max_np, 1);
+    if (A->tape_K != A->K) {
+        ADJ_FREE(A->t_rec); ADJ_FREE(A->t_whist);
+        DEM_HIP(hipMalloc(&A->t_rec, sizeof(PRec) * K * n));
+        DEM_HIP(hipMalloc(&A->t_whist, sizeof(Hist) * K * n));
+        A->tape_K = A->K;
+    }
+    if (!A->lam[0]) {
+        DEM_HIP(hipMalloc(&A->lam[0], sizeof(SAdj) * n));
+        DEM_HIP(hipMalloc(&A->lam[1], sizeof(SAdj) * n));
+        DEM_HIP(hipMalloc(&A->ftb, sizeof(double) * 6 * n));
+        DEM_HIP(hipMalloc(&A->thacc, sizeof(double) * kThetaPer * n));
+    }
+    // pair-sized buffers follow the largest list of the forward
+    ADJ_FREE(A->t_hist); ADJ_FREE(A->hb[0]); ADJ_FREE(A->hb[1]); ADJ_FREE(A->pa);
+    DEM_HIP(hipMalloc(&A->t_hist, sizeof(Hist) * K * mnp));
+    DEM_HIP(hipMalloc(&A->hb[0], sizeof(double) * 4 * mnp));
+    DEM_HIP(hipMalloc(&A->hb[1], sizeof(double) * 4 * mnp));
+    DEM_HIP(hipMalloc(&A->pa, sizeof(PAdj) * mnp));
+
+    DEM_HIP(hipEventRecord(c->ev0, st));
+    // seed at the final state
+    if ((rc = snap_restore(c, A->ck[A->nseg]))) return rc;
+    if ((rc = observe(c, A, A->obs, nullptr, nullptr))) return rc;
+    std::vector<double> coef(3 * (size_t)c->ninst);
+    for (int k = 0; k < c->ninst; k++) {
+        double W = A->wsum[k];
+        coef[3 * k] = A->vreg[k] > 0.0 ? gphi[k] / A->vreg[k] : 0.0;
+        coef[3 * k + 1] = W > 0.0 ? gcn[k] / W : 0.0;
+        coef[3 * k + 2] = W > 0.0 ? gcn[k] * A->csum[k] / (W * W) : 0.0;
+    }
+    DEM_HIP(hipMemcpyAsync(A->coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, st));
+    unsigned gn = cdiv(n, kBlock);
+    A->cur = 0;
+    hipLaunchKernelGGL(k_obs_seed, dim3(gn), dim3(kBlock), 0, st, (const PRec *)c->rec[c->rc],
+                       (const double *)A->sobs, (const double *)A->coef, A->lam[0], (unsigned)n);
+    int hc = 0;
+    size_t z = 4 * mnp;
+    hipLaunchKernelGGL(k_zero_d, dim3(std::max(1u, std::min(cdiv(z, kBlock), 4096u))), dim3(kBlock), 0, st,
+                       A->hb[0], z);
+    z = (size_t)kThetaPer * n;
+    hipLaunchKernelGGL(k_zero_d, dim3(std::max(1u, std::min(cdiv(z, kBlock), 4096u))), dim3(kBlock), 0, st,
+                       A->thacc, z);
+    DEM_HIP(hipGetLastError());
+
+    for (int s = A->nseg - 1; s >= 0; s--) {
+        const AdjSnap &S = A->ck[s];
+        if ((rc = snap_restore(c, S))) return rc;
+        int ks = std::min(A->K, A->nsteps - s * A->K);
+        std::vector<ListView> views;
+        views.push_back(ListView{S.ab, S.off, S.toff, S.jt, S.wflag, nullptr, nullptr, S.npairs});
+        std::vector<int> step_ver(ks);
+        std::vector<char> step_reb(ks, 0);
+        int used = 0;
+        for (int j = 0; j < ks; j++) {
+            if ((size_t)used >= A->pool.size()) A->pool.emplace_back();
+            bool reb = false;
+            AdjList &L = A->pool[used];
+            if ((rc = adj_step(c, A, j, &L, &reb))) return rc;
+            if (reb) {
+                views.push_back(ListView{L.ab, L.off, L.toff, L.jt, L.wflag, L.perm, L.src, L.np});
+                used++;
+                step_reb[j] = 1;
+            }
+            step_ver[j] = (int)views.size() - 1;
+        }
+        if ((rc = reverse_segment(c, A, ks, views, step_ver, step_reb, hc))) return rc;
+    }
+    DEM_HIP(hipEventRecord(c->ev1, st));
+    DEM_HIP(hipEventSynchronize(c->ev1));
+    DEM_HIP(hipEventElapsedTime(&A->bwd_ms, c->ev0, c->ev1));
+    c->last_ms = A->bwd_ms;
+    A->have_grad = true;
+    if (theta_bar) {
+        std::vector<double> th((size_t)kThetaPer * n);
+        DEM_HIP(hipMemcpy(th.data(), A->thacc, sizeof(double) * th.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < c->ninst; k++) {
+            const InstDev &I = c->hinst[k];
+            double acc[kThetaPer] = {0.0};
+            for (unsigned i = I.p_off; i < I.p_off + I.n; i++)
+                for (int j = 0; j < kThetaPer; j++) acc[j] += th[(size_t)kThetaPer * i + j];
+            for (int j = 0; j < kThetaPer; j++) theta_bar[(size_t)kThetaPer * k + j] = acc[j];
+        }
+    }
+    // leave the context at the end of the forward
+    return snap_restore(c, A->ck[A->nseg]);
+}
+
+int dem_adj_state_grad(dem_ctx *c, double *xbar, double *vbar, double *wbar)
+{
+    if (!c || !c->adj || !c->adj->have_grad) return DEM_ERR_INVALID;
+    AdjData *A = c->adj;
+    size_t n = c->n;
+    std::vector<SAdj> lam(n);
+    std::vector<unsigned> inv(n);
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipMemcpy(lam.data(), A->lam[A->cur], sizeof(SAdj) * n, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(inv.data(), A->ck[0].inv, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        const SAdj &L = lam[inv[i]];
+        for (int k = 0; k < 3; k++) {
+            if (xbar) xbar[3 * i + k] = L.x[k];
+            if (vbar) vbar[3 * i + k] = L.v[k];
+            if (wbar) wbar[3 * i + k] = L.w[k];
+        }
+    }
+    return DEM_OK;
+}
+
+int dem_adj_info(dem_ctx *c, double *out)
+{
+    if (!c || !c->adj || !out) return DEM_ERR_INVALID;
+    AdjData *A = c->adj;
+    out[0] = A->fwd_ms;
+    out[1] = A->bwd_ms;
+    out[2] = A->nseg;
+    out[3] = A->max_np;
+    // device bytes held by the checkpoints and by the tapes of one segment
+    size_t snaps = 0;
+    for (const AdjSnap &S : A->ck)
+        if (S.valid)
+            snaps += (sizeof(PRec) + sizeof(Hist) + 6 * sizeof(unsigned)) * (size_t)A->n +
+                     S.pcap * (sizeof(uint2) + sizeof(Hist) + sizeof(unsigned));
+    size_t tapes = (size_t)A->tape_K * (A->n * (sizeof(PRec) + sizeof(Hist)) + (size_t)A->max_np * sizeof(Hist));
+    out[4] = (double)snaps;
+    out[5] = (double)tapes;
+    return DEM_OK;
+}
+
+int dem_adj_release(dem_ctx *c)
+{
+    if (!c) return DEM_ERR_INVALID;
+    adj_free(c);
+    return DEM_OK;
 }
 
 }  // extern "C"

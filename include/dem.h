@@ -1,7 +1,7 @@
 /*
  * dem.h -- C ABI of the MI355X (gfx950) DEM packing step, libdem.so.
  *
- * Two layers, one set of HIP kernels:
+ * Three layers, one set of HIP kernels:
  *
  *  1. Reference-named entry points (section "drop-in").  Same names, argument order and
  *     meaning as the reference's extern "C" operator API in particleSystem.cu, plus the
@@ -18,6 +18,10 @@
  *     per-particle gather, wall force, gravity and integration into one kernel, and runs
  *     one or many independent instances (batched form) per launch.  Functions return a
  *     dem_status.
+ *
+ *  3. Adjoint form (dem_adj_*).  Reverse mode of the context step with checkpoints and
+ *     segment recompute, for gradients of smooth packing observables with respect to the
+ *     material parameters and the initial state.
  *
  * All arithmetic is IEEE FP64.  Units are the reference's reduced units: length = particle
  * diameter d, mass = rho*pi*d^3/6, force = m*g, time = sqrt(d/g) (dempacking.cu:146-153,
@@ -189,7 +193,10 @@ enum {
     DEM_OPT_SKIN = 4,              /* list skin (cell edge follows); requests a rebuild */
     DEM_OPT_PAIR_BLOCK = 5,        /* threads per block of the pair kernel: 64, 128, 256 */
     DEM_OPT_PAIR_GRID = 6,         /* pair kernel blocks per CU (grid-stride) */
-    DEM_OPT_PARTICLE_BLOCK = 7     /* threads per block of the per-particle kernel */
+    DEM_OPT_PARTICLE_BLOCK = 7,    /* threads per block of the per-particle kernel */
+    DEM_OPT_ADJ_ROLL_REG = 8       /* adjoint: floor of |w_rel| in the rolling moment's
+                                      linearisation, in units of the one-step chatter scale
+                                      mu_r fcn dt^2 (1/I_a + 1/I_b); 0 = exact, default 1 */
 };
 
 /* Result of dem_autotune. */
@@ -299,6 +306,49 @@ int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, 
 int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out);
 int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
                       const double *outbar, double *inbar, double *matbar);
+
+/* ----------------------------------------------------------------------------------- */
+/* 3. Adjoint (differentiable) form -- no counterpart in the reference                  */
+/* ----------------------------------------------------------------------------------- */
+
+/* Smooth packing observables, per instance.  Region: side walls moved in by `margin`,
+ * zlo < z < zhi, with logistic edges of width `eps`:
+ *   phi = sum_i w(x_i) V_i / V_region,   CN = sum_i w_i c_i / sum_i w_i,
+ *   c_i = sum_j S(overlap_ij / eps_c), S the C1 smoothstep (0 at no overlap, 1 past eps_c). */
+typedef struct dem_obs {
+    double margin, zlo, zhi, eps, eps_c;
+} dem_obs;
+
+/* Replace the material of one instance (E*, mu, damping, rolling friction of the pair and
+ * wall laws) without touching the state. */
+int dem_set_material(dem_ctx *ctx, int instance, const dem_material *mat);
+
+/* phi[ninst], cn[ninst] of the current state (builds the list first if there is none). */
+int dem_observe(dem_ctx *ctx, const dem_obs *obs, double *phi, double *cn);
+
+/* Run nsteps steps (dem_step's schedule) keeping a full-state checkpoint every
+ * `checkpoint_every` steps, then evaluate the observables at the end.  Requires every
+ * particle active. */
+int dem_adj_forward(dem_ctx *ctx, int nsteps, int checkpoint_every, const dem_obs *obs,
+                    double *phi, double *cn);
+
+/* Reverse sweep of the last dem_adj_forward: for each segment, last first, restore its
+ * checkpoint, recompute it with tapes, and run the adjoint kernels back over it.  gphi,
+ * gcn[ninst] are the adjoints of the observables; theta_bar[10 * ninst] receives, per
+ * instance, d(sum gphi phi + gcn CN) / d(E*, mu, gamma_n, gamma_t, mu_r) of the pair law and
+ * then of the wall law.  The context is left at the forward's end state. */
+int dem_adj_backward(dem_ctx *ctx, const double *gphi, const double *gcn, double *theta_bar);
+
+/* Adjoints of the initial positions, displacements and rotations (per particle index,
+ * 3 per particle each; any may be null) from the last dem_adj_backward. */
+int dem_adj_state_grad(dem_ctx *ctx, double *xbar, double *vbar, double *wbar);
+
+/* out[6] = forward ms, backward ms, segments, largest pair list, checkpoint bytes, tape
+ * bytes of the last forward/backward. */
+int dem_adj_info(dem_ctx *ctx, double *out);
+
+/* Free the checkpoints and tapes. */
+int dem_adj_release(dem_ctx *ctx);
 
 /* ----------------------------------------------------------------------------------- */
 /* Host utilities: deck, units, restart file, workload generators                       */

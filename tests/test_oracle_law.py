@@ -140,6 +140,20 @@ def test_restitution_map_against_scipy(dem, e):
     assert got == pytest.approx(e, abs=2e-4)
 
 
+@pytest.mark.parametrize("e", [0.3, 0.6, 0.9])
+def test_restitution_map_derivative_against_scipy(dem, e):
+    """d gamma / d e of the library's map (central difference, relative step 1e-4, as the
+    adjoint's chain rule takes it) against the inverse slope of an independent tight-tolerance
+    solve of the same collision in physical form."""
+    g = dem.damping_for_restitution(e)
+    dg = 1e-4 * g
+    de_dg = (_collision_restitution_ode(g + dg, 0.05, 5e4, 0.25, 0.5)
+             - _collision_restitution_ode(g - dg, 0.05, 5e4, 0.25, 0.5)) / (2 * dg)
+    h = 1e-4 * e
+    ours = (dem.damping_for_restitution(e + h) - dem.damping_for_restitution(e - h)) / (2 * h)
+    assert ours == pytest.approx(1.0 / de_dg, rel=1e-3)
+
+
 @pytest.mark.parametrize("e", [0.3, 0.6])
 def test_discrete_collision_restitution(dem, orc, e):
     """Head-on collision integrated with the oracle law and the reference integrator update
