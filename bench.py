@@ -13,9 +13,12 @@ OpenMP) on a bounded sample of the same workload.
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "discrete-element-method-for-packing-cuda-c-code_amd")
@@ -128,6 +131,126 @@ def bench_batched(dem, steps, interval, tune_steps=48):
             "autotune": tune}
 
 
+def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4):
+    """Differentiable form on the 20K-sphere bed: checkpointed forward + adjoint backward
+    against the plain forward (cost ratio), and the adjoint gradient of phi and CN in
+    (mu, log E, e, mu_r) against central differences of the plain forward."""
+    import copy
+    A = dem.adjoint
+    t0 = time.perf_counter()
+    w = dem.settle_bed(dem.config4(), tap=0.0)
+    log(f"adjoint workload: n={w.n}, bed top {w.meta['bed_top']:.2f} ({time.perf_counter() - t0:.1f} s)")
+    obs = dem.make_obs(zhi=w.meta["bed_top"] - 3.0, margin=2.0, zlo=2.0, eps=0.5, eps_c=1e-3)
+    prob = A.PackingProblem(w, nsteps, every, obs)
+    th0 = [0.3, math.log(1e7), 0.6, 0.002]
+    prob.evaluate([th0])          # warm-up
+    phi_p, cn_p = prob.evaluate([th0])
+    t_plain = prob.timing["plain_s"]
+    ms_plain = prob.ctx.last_ms()
+    phi, cn = prob.forward([th0])
+    g_phi = prob.backward([1.0], [0.0])[0]
+    t_fwd, t_bwd = prob.timing["forward_s"], prob.timing["backward_s"]
+    info = prob.ctx.adj_info()
+    g_cn = prob.backward([0.0], [1.0])[0]
+    assert phi[0] == phi_p[0] and cn[0] == cn_p[0]
+    r = A.fd_check(prob, [th0], rel=fd_rel, which=(0, 1, 3))
+    base = prob.mmap.material(th0)
+    gam = base.p.dampnp
+    vals = []
+    for sgn in (1, -1):
+        m = copy.deepcopy(base)
+        m.p.dampnp = m.p.damptp = m.w.dampnw = m.w.damptw = gam * (1 + sgn * fd_rel)
+        prob.ctx.set_material(0, m)
+        prob.ctx.load(w)
+        prob.ctx.step(nsteps)
+        vals.append(prob.ctx.observe(obs))
+    dg = prob.mmap.dgamma_de(0.6)
+    r["fd_phi"][2] = (vals[0][0][0] - vals[1][0][0]) / (2 * fd_rel * gam) * dg
+    r["fd_cn"][2] = (vals[0][1][0] - vals[1][1][0]) / (2 * fd_rel * gam) * dg
+    rel = lambda a, b: abs(a - b) / max(abs(a), abs(b), 1e-300)
+    out = {"workload": w.name, "n": w.n, "steps": nsteps, "checkpoint_every": every,
+           "phi": float(phi[0]), "cn": float(cn[0]),
+           "plain_s": t_plain, "forward_s": t_fwd, "backward_s": t_bwd,
+           "cost_ratio": (t_fwd + t_bwd) / t_plain, "backward_over_forward": t_bwd / t_fwd,
+           "device_ms": {"plain": ms_plain, "forward": info["forward_ms"],
+                         "backward": info["backward_ms"]},
+           "checkpoint_bytes": info["checkpoint_bytes"], "tape_bytes": info["tape_bytes"],
+           "params": list(A.PARAMS), "fd_rel_step": fd_rel,
+           "grad_phi": g_phi.tolist(), "fd_phi": r["fd_phi"].tolist(),
+           "grad_cn": g_cn.tolist(), "fd_cn": r["fd_cn"].tolist(),
+           "rel_err_phi": [rel(a, b) for a, b in zip(g_phi, r["fd_phi"])],
+           "rel_err_cn": [rel(a, b) for a, b in zip(g_cn, r["fd_cn"])],
+           # error against the dominant sensitivity: |adj - fd| x natural scale of the
+           # parameter / max over parameters of |fd| x its natural scale
+           "norm_err_phi": norm_err(g_phi, r["fd_phi"], th0),
+           "norm_err_cn": norm_err(g_cn, r["fd_cn"], th0)}
+    log(f"adjoint: cost ratio {out['cost_ratio']:.2f}x (plain {t_plain:.3f} s, fwd {t_fwd:.3f} s, "
+        f"bwd {t_bwd:.3f} s); rel err phi {np.round(out['rel_err_phi'], 4)} cn "
+        f"{np.round(out['rel_err_cn'], 4)}; normalised phi {np.round(out['norm_err_phi'], 4)} "
+        f"cn {np.round(out['norm_err_cn'], 4)}")
+    return out
+
+
+def norm_err(adj, fd, theta):
+    nat = np.array([theta[0], 1.0, theta[2], theta[3]])
+    fd = np.asarray(fd)
+    top = np.max(np.abs(fd * nat))
+    return (np.abs(np.asarray(adj) - fd) * nat / top).tolist()
+
+
+def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, iters=16,
+                      seed=7):
+    """Calibration of (mu, log E, e, mu_r) to the observables of a bed run by the plain method
+    at seeded hidden parameters: `ninst` seeded starts in one batched context (replicas of one
+    settled bed), one forward + adjoint backward per iteration."""
+    A = dem.adjoint
+    t0 = time.perf_counter()
+    w1 = dem.settle_bed(dem.config4(n=n, side=side, seed=seed), tap=0.0)
+    obs = dem.make_obs(zhi=w1.meta["bed_top"] - 2.0, margin=1.5, zlo=1.5, eps=0.5, eps_c=1e-3)
+    rng = np.random.default_rng(seed)
+
+    def draw(k):
+        return np.stack([rng.uniform(0.2, 0.6, k), np.log(rng.uniform(3e6, 3e7, k)),
+                         rng.uniform(0.4, 0.8, k), rng.uniform(0.001, 0.005, k)], axis=1)
+    hidden = draw(1)[0]
+    tgt = A.PackingProblem(w1, nsteps, every, obs)
+    tphi, tcn = tgt.evaluate([hidden])
+    del tgt
+    insts = [copy_instance(w1.instances[0]) for _ in range(ninst)]
+    rep = lambda a: np.concatenate([a] * ninst)
+    big = dem.Workload(f"calibration-{ninst}x{n}", insts, rep(w1.pos), rep(w1.pdis), rep(w1.angv),
+                       rep(w1.rad), rep(w1.rmass), rep(w1.inert), rebuild_interval=0)
+    prob = A.PackingProblem(big, nsteps, every, obs)
+    starts = draw(ninst)
+    log(f"calibration: {ninst} x {n} spheres, target phi {tphi[0]:.5f} cn {tcn[0]:.4f} "
+        f"({time.perf_counter() - t0:.1f} s setup)")
+    t0 = time.perf_counter()
+    res = A.calibrate(prob, float(tphi[0]), float(tcn[0]), starts, iters=iters,
+                      lr=(0.03, 0.1, 0.03, 0.0005), scale=(1e-3, 0.05),
+                      progress=lambda h: log(f"  calibration iter {h['iter']}: loss {h['loss']:.4f} "
+                                             f"(fwd {h['forward_s']:.2f} s, bwd {h['backward_s']:.2f} s)"))
+    el = time.perf_counter() - t0
+    h0, h1 = res["history"][0], res["history"][-1]
+    best = [min(h["loss_per_instance"][k] for h in res["history"]) for k in range(ninst)]
+    out = {"instances": ninst, "n_per_instance": n, "steps": nsteps, "iters": iters,
+           "hidden": hidden.tolist(), "target": [float(tphi[0]), float(tcn[0])],
+           "loss_first": h0["loss"], "loss_last": h1["loss"],
+           "median_instance_loss_first": float(np.median(h0["loss_per_instance"])),
+           "median_instance_loss_best": float(np.median(best)),
+           "seconds": el, "seconds_per_iter": el / iters,
+           "particle_steps_per_s_fwd_bwd": big.n * nsteps * iters / el}
+    log(f"calibration: loss {h0['loss']:.3f} -> {h1['loss']:.3f} in {iters} iterations "
+        f"({el:.1f} s)")
+    return out
+
+
+def copy_instance(inst):
+    import ctypes
+    c = type(inst)()
+    ctypes.pointer(c)[0] = inst
+    return c
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +263,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--batched-interval", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-calibration", action="store_true")
+    ap.add_argument("--calib-instances", type=int, default=64)
+    ap.add_argument("--calib-n", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
@@ -178,6 +305,17 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(dem, args.batched_steps, args.batched_interval)
+    adjoint = None
+    if not args.no_adjoint:
+        log("importing torch for the autograd layer (the first import on a fresh machine can "
+            "take a minute or two)")
+        t0 = time.perf_counter()
+        import torch  # noqa: F401
+        log(f"torch imported ({time.perf_counter() - t0:.1f} s)")
+        adjoint = bench_adjoint(dem)
+        if not args.no_calibration:
+            adjoint["calibration"] = bench_calibration(dem, ninst=args.calib_instances,
+                                                       n=args.calib_n)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dem, w, args.cpu_seconds)
@@ -202,6 +340,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

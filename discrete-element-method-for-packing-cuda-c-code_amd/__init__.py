@@ -12,6 +12,7 @@ displacements, as in the reference integrator (neiglist.cu:356-360).
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 import os
 from dataclasses import dataclass, field
@@ -281,6 +282,7 @@ def reduced_material(E: float, nu: float, rho: float, diam: float, mu: float, mu
     return m
 
 
+@functools.lru_cache(maxsize=4096)
 def damping_for_restitution(e: float) -> float:
     g = lib().dem_damping_for_restitution(float(e))
     if g < 0:

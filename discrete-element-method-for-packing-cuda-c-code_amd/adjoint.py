@@ -72,10 +72,20 @@ class PackingProblem:
         self.theta = None
         self.timing = {}
 
+    def _map(self, f, rows):
+        # the restitution inversion (an ODE solve per call) runs in the C library, which
+        # releases the GIL: instances map in parallel
+        if len(rows) == 1:
+            return [f(rows[0])]
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(min(16, len(rows))) as ex:
+            return list(ex.map(f, rows))
+
     def _set(self, theta):
         theta = np.asarray(theta, dtype=np.float64).reshape(self.ninst, 4)
-        for k in range(self.ninst):
-            self.ctx.set_material(k, self.mmap.material(theta[k]))
+        mats = self._map(self.mmap.material, list(theta))
+        for k, m in enumerate(mats):
+            self.ctx.set_material(k, m)
         self.ctx.load(self.w)
         return theta
 
@@ -93,7 +103,8 @@ class PackingProblem:
         tb = self.ctx.adj_backward(gphi, gcn)
         self.timing["backward_s"] = time.perf_counter() - t0
         self.theta_bar = tb
-        return np.stack([self.mmap.chain(self.theta[k], tb[k]) for k in range(self.ninst)])
+        return np.stack(self._map(lambda k: self.mmap.chain(self.theta[k], tb[k]),
+                                  list(range(self.ninst))))
 
     def evaluate(self, theta):
         """Plain forward (dem_step, no checkpoints) and the observables: the FD reference."""
@@ -166,11 +177,12 @@ BOUNDS = np.array([[0.05, 1.0], [math.log(1e6), math.log(1e9)], [0.1, 0.95], [0.
 
 
 def calibrate(problem: PackingProblem, target_phi: float, target_cn: float, theta0,
-              iters: int = 10, lr=(0.05, 0.2, 0.05, 0.002), scale=(0.01, 0.5)) -> dict:
+              iters: int = 10, lr=(0.05, 0.2, 0.05, 0.002), scale=(0.01, 0.5),
+              progress=None) -> dict:
     """Gradient-based calibration of theta (one row per instance, each its own start) to the
     target observables: Adam on the scaled squared misfit with per-parameter step sizes `lr`,
     projected onto BOUNDS after each step.  Every iteration is one batched forward + backward
-    through the autograd function."""
+    through the autograd function; `progress(record)` is called after each."""
     torch = _torch()
     step = torch.tensor(lr, dtype=torch.float64)
     # Adam's steps are about lr in the optimised variable whatever the gradient's scale, so
@@ -192,6 +204,8 @@ def calibrate(problem: PackingProblem, target_phi: float, target_cn: float, thet
         hist.append({"iter": it, "loss": float(loss.detach()), "loss_per_instance": loss_k.detach().numpy().copy(),
                      "theta": theta.detach().numpy().copy(), "phi": phi.detach().numpy().copy(),
                      "cn": cn.detach().numpy().copy(), **problem.timing})
+        if progress:
+            progress(hist[-1])
         opt.step()
         with torch.no_grad():
             z.copy_(torch.max(torch.min(z * step, hi), lo) / step)

@@ -188,12 +188,34 @@ double dem_damping_for_restitution(double e)
 {
     if (!(e > 0.0) || !(e <= 1.0)) return -1.0;
     if (e >= 1.0) return 0.0;
+    // bracket, then the Illinois variant of regula falsi on f(g) = e(g) - e (e decreases
+    // with g), to the last bits of g
     double lo = 0.0, hi = 0.1;
-    while (restitution_ode(hi) > e && hi < 100.0) hi *= 2.0;
-    for (int it = 0; it < 60; it++) {
-        double mid = 0.5 * (lo + hi);
-        if (restitution_ode(mid) > e) lo = mid;
-        else hi = mid;
+    double flo = 1.0 - e, fhi = restitution_ode(hi) - e;
+    while (fhi > 0.0 && hi < 100.0) {
+        lo = hi;
+        flo = fhi;
+        hi *= 2.0;
+        fhi = restitution_ode(hi) - e;
+    }
+    if (fhi > 0.0) return hi;
+    int side = 0;
+    for (int it = 0; it < 200 && hi - lo > 1e-13 * hi; it++) {
+        double g = (lo * fhi - hi * flo) / (fhi - flo);
+        if (!(g > lo && g < hi)) g = 0.5 * (lo + hi);
+        double fg = restitution_ode(g) - e;
+        if (std::fabs(fg) < 1e-15) return g;
+        if (fg > 0.0) {
+            lo = g;
+            flo = fg;
+            if (side == -1) fhi *= 0.5;
+            side = -1;
+        } else {
+            hi = g;
+            fhi = fg;
+            if (side == 1) flo *= 0.5;
+            side = 1;
+        }
     }
     return 0.5 * (lo + hi);
 }
