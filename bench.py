@@ -7,6 +7,8 @@ dt = 1e-6 s, FP64, reference rebuild policy (displacement trigger).  Inputs are 
 before timing.  One step = [list rebuild when triggered] + pair forces + gather/walls/gravity/
 integration.
 
+The step runs the fused per-particle kernel (k_step) unless --split is given.
+
 Prints ONE JSON line.  The CPU baseline is the oracle (the same PCCP algorithm in C with
 OpenMP) on a bounded sample of the same workload.
 """
@@ -41,13 +43,36 @@ def log(msg):
 
 
 def kernel_bytes(P, N):
-    """Algorithmic bytes per launch (one touch per array element, reference element sizes:
-    double3 24, double 8, uint 4, uint2 8).
-    pair kernel  : pair (uint2) 8 + history r/w 2*32 + output (F, Ta, Tb, flag) 80 per pair;
-                   gathered particle data (pos, pdis, angv, rad, rmass = 88 B) once per particle.
-    particle kern: pair output 80 + transposed index 4 per pair; per particle record r/w 2*128,
-                   wall history r/w 2*32, two CSR offsets 8, contact count 4."""
-    return {"pair": 152 * P + 88 * N, "particle": 84 * P + 332 * N}
+    """Algorithmic bytes per launch: each array element the kernel must read or write,
+    touched once (records and histories as stored: 128-byte particle record, 32-byte
+    history, 8-byte pair, 4-byte index).
+    fused step (k_step): per particle record read + write 2*128, wall-candidate flag 4, CSR
+                   offsets 2*4 + 2*4, contact count 4 (= 280); per pair, first side: pair 8,
+                   history read + write 2*32; second side: transposed index 4, pair 8,
+                   history 32 (= 116).  Neighbour records are the same records (counted
+                   once per particle).
+    split path, pair kernel: pair 8 + history r/w 2*32 + output (F, Ta, Tb, flag) 80 per
+                   pair; gathered particle data (pos, pdis, angv, rad, rmass = 88 B) once
+                   per particle.
+    split path, particle kernel: pair output 80 + transposed index 4 per pair; per particle
+                   record r/w 2*128, wall history r/w 2*32, two CSR offsets 8, contact
+                   count 4."""
+    return {"step": 116 * P + 280 * N, "pair": 152 * P + 88 * N, "particle": 84 * P + 332 * N}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_step_500k_lognormal_cylinder.md")
+
+
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a known-size copy), for the 500K workload only."""
+    if n != 500_000 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    key = f"traffic_bytes_per_launch dem::{kernel}<false> = "
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith(key):
+            return float(line[len(key):])
+    return None
 
 
 def cpu_baseline(dem, w, budget_s):
@@ -267,16 +292,34 @@ def main():
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--calib-instances", type=int, default=64)
     ap.add_argument("--calib-n", type=int, default=2000)
+    ap.add_argument("--split", action="store_true",
+                    help="time the split path (pair kernel + gather) instead of the fused step")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run 3 device-to-device copies of this many bytes (the known-size "
+                         "kernel that scales the HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no "
               f"multi-GPU path", file=sys.stderr)
         sys.exit(2)
     dem = load_pkg()
+    if args.copy_probe:
+        L = dem.lib()
+        nb = int(args.copy_probe)
+        a, b = L.dem_dev_alloc(nb), L.dem_dev_alloc(nb)
+        assert a and b, "copy probe allocation failed"
+        dem._check(L.dem_memzero(a, nb), "dem_memzero")
+        for _ in range(3):
+            dem._check(L.dem_memcpy(b, a, nb, 3), "dem_memcpy")
+        dem._check(L.dem_device_sync(), "dem_device_sync")
+        L.dem_dev_free(a)
+        L.dem_dev_free(b)
+        log(f"copy probe: 3 device-to-device copies of {nb} bytes")
     t0 = time.perf_counter()
     w = dem.config1(n=args.n)
     log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
     ctx = dem.Context(w.instances)
+    ctx.set_option(dem.OPT_FUSED_STEP, 0 if args.split else 1)
     ctx.load(w)
     ctx.step(args.warmup)
     ctx.sync()
@@ -292,15 +335,21 @@ def main():
     kt = ctx.time_kernels(args.kernel_steps)
     P, N = s1.npairs, w.n
     kb = kernel_bytes(P, N)
-    dom = "pair" if kt["pair_ms"] >= kt["particle_ms"] else "particle"
-    ms = kt[f"{dom}_ms"]
+    if args.split:
+        dom = "pair" if kt["pair_ms"] >= kt["particle_ms"] else "particle"
+        ms = kt[f"{dom}_ms"]
+        kname = "k_pair_force" if dom == "pair" else "k_particle"
+        log(f"kernels: pair+walls {kt['pair_ms'] * 1e3:.1f} us, particle "
+            f"{kt['particle_ms'] * 1e3:.1f} us, rebuild {kt['rebuild_ms']:.3f} ms")
+    else:
+        dom, ms, kname = "step", kt["particle_ms"], "k_step"
+        log(f"kernels: walls {kt['pair_ms'] * 1e3:.1f} us, fused step "
+            f"{kt['particle_ms'] * 1e3:.1f} us, rebuild {kt['rebuild_ms']:.3f} ms")
     achieved = kb[dom] / (ms * 1e-3) / 1e9
-    log(f"kernels: pair {kt['pair_ms'] * 1e3:.1f} us, particle {kt['particle_ms'] * 1e3:.1f} us, "
-        f"rebuild {kt['rebuild_ms']:.3f} ms; {dom}: {achieved:.0f} GB/s")
+    log(f"{kname}: {achieved:.0f} GB/s algorithmic")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_pair_force" if dom == "pair" else "k_particle",
-            "bytes_per_launch": kb[dom]}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, N),
+            "kernel": kname, "bytes_per_launch": kb[dom], "avg_us": ms * 1e3}
     del ctx
     batched = None
     if not args.no_batched:
@@ -334,6 +383,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "n": N, "npairs": P, "pairs_per_particle": round(P / N, 3),
+                   "step_path": "split" if args.split else "fused",
                    "rebuild_policy": "displacement trigger |q| > 0.275",
                    "rebuilds_timed": int(s1.rebuilds - s0.rebuilds),
                    "kernel_ms": {k: round(v, 5) for k, v in kt.items() if k.endswith("_ms")}},

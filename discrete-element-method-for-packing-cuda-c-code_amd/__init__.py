@@ -45,6 +45,7 @@ OPT_PAIR_BLOCK = 5
 OPT_PAIR_GRID = 6
 OPT_PARTICLE_BLOCK = 7
 OPT_ADJ_ROLL_REG = 8
+OPT_FUSED_STEP = 9
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_particle(const PRec *tr, const H
                                                          const InstDev *inst, InstDev one,
                                                          double roll_k)
 {
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     const PRec R = tr[a];
     const InstDev &I = BATCHED ? inst[R.inst] : one;
@@ -92,7 +92,10 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(const uint2 *ab, unsigned n
                                                      PAdj *pa, const InstDev *inst, LawParams pp,
                                                      double dt, double roll_k)
 {
-    for (unsigned p = blockIdx.x * kBlock + threadIdx.x; p < np; p += gridDim.x * kBlock) {
+    const unsigned per = (np + gridDim.x - 1) / gridDim.x;   // contiguous, XCD-aware ranges
+    const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned p_end = min(p_begin + per, np);
+    for (unsigned p = p_begin + threadIdx.x; p < p_end; p += kBlock) {
         uint2 e = ab[p];
         const PRec &A = tr[e.x];
         const PRec &B = tr[e.y];
@@ -137,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_gather(SAdj *lam, const PAdj *pa
                                                        unsigned np, const PRec *tr,
                                                        double *thacc, unsigned n)
 {
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double t[5] = {0, 0, 0, 0, 0};

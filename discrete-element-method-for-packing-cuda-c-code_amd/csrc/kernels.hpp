@@ -55,9 +55,9 @@ struct InstDev {
     unsigned p_off, n, pad1, pad2;
 };
 
-// Device control block, copied to a host-mapped mirror by k_publish after each step.  The
-// mirror continues with per-instance max-z bit patterns (u64[ninst]) and contact totals
-// (u32[ninst]) written by k_inst_reduce on demand.
+// Device control block, copied to a host-mapped mirror by k_publish after each step.  Its
+// allocation continues with per-instance max-z bit patterns (u64[ninst]) and contact totals
+// (u32[ninst]), reduced there by k_inst_reduce on demand and copied to the mirror's tail.
 struct DevCtl {
     unsigned need_rebuild;   // trigger raised by the last step (atomicOr)
     unsigned steps;          // steps completed (incremented by k_publish)
@@ -74,6 +74,19 @@ __device__ __forceinline__ void st2(double *p, double a, double b)
 }
 
 __device__ __forceinline__ bool is_active(const unsigned *active) { return !active || *active != 0u; }
+
+// XCD-aware block order.  Workgroups are dispatched to the 8 XCDs round-robin (block b to
+// XCD b mod 8), each XCD with its own L2.  Renumbering so that XCD x runs the contiguous
+// range of logical blocks [base_x, base_x + count_x), in order, keeps a particle's
+// neighbours (nearby slots: slots are cell-sorted) and the pair data its partners read on
+// the same L2.  A bijection on [0, nb) for any nb.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+    unsigned q = nb / kXcds, r = nb % kXcds;
+    unsigned x = b % kXcds, i = b / kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // ----------------------------------------------------------------------------------------
 // Rebuild
@@ -254,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const uin
                                                       unsigned *nl, const unsigned *active)
 {
     if (!is_active(active)) return;
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
     unsigned cg = 0, cl = 0;
@@ -297,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     unsigned cap_old, DevCtl *ctl, unsigned *src, const unsigned *active)
 {
     if (!is_active(active)) return;
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
     unsigned pg = off[a];
@@ -339,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const uns
                                                       const unsigned *active)
 {
     if (!is_active(active)) return;
-    unsigned a = blockIdx.x * kBlock + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     unsigned e = toff[a + 1];
     if (e > cap) e = cap;
@@ -379,12 +392,19 @@ __global__ void k_set_u32(unsigned *p, unsigned v)
 // Grid-stride over npairs read from the device, so a fixed grid serves any list size.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const unsigned *npairs,
-                                                       const PRec *rec, Hist *hist, POut *out,
+                                                       const PRec *rec, const Hist *hist,
+                                                       Hist *hist_out, POut *out,
                                                        const InstDev *inst, LawParams pp,
                                                        double dt, Hist *tape_hist)
 {
     const unsigned np = *npairs;
-    for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    // each block takes a contiguous range of pairs, XCD-aware (pairs are ordered by their
+    // first member's slot, so a range touches nearby records)
+    const unsigned bs = blockDim.x;
+    const unsigned per = (np + gridDim.x - 1) / gridDim.x;
+    const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned p_end = min(p_begin + per, np);
+    for (unsigned p = p_begin + threadIdx.x; p < p_end; p += bs) {
         uint2 e = ab[p];
         const double *A = reinterpret_cast<const double *>(rec + e.x);
         const double *B = reinterpret_cast<const double *>(rec + e.y);
@@ -418,9 +438,11 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
         st2(O + 4, ta.x, ta.y);
         st2(O + 6, ta.z, tb.x);
         st2(O + 8, tb.y, tb.z);
-        double *HW = reinterpret_cast<double *>(hist + p);
-        st2(HW, ho.x, ho.y);
-        st2(HW + 2, ho.z, fo);
+        if (hist_out) {   // in place (hist_out == hist) on the split path
+            double *HW = reinterpret_cast<double *>(hist_out + p);
+            st2(HW, ho.x, ho.y);
+            st2(HW + 2, ho.z, fo);
+        }
     }
 }
 
@@ -467,11 +489,83 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
     }
 }
 
-// Per-particle kernel: gather of the particle's pair forces (own pairs in list order, then
-// pairs where it is second, via the transposed index), then gravity and walls
-// (calculateforceiwD, neiglist.cu:238-306), then the integrator (updatepositionD,
-// neiglist.cu:308-409) with the rebuild trigger |q|^2 > ann2.  Max z and contact totals are
-// not reduced here (they are diagnostics): k_inst_reduce computes them on demand.
+// Gravity, wall force, force record, integrator and rebuild trigger of slot a, after its
+// pair forces have been summed (calculateforceiwD, neiglist.cu:238-306, and
+// updatepositionD, neiglist.cu:308-409).  R is the record before the step, W receives the
+// record after it (the same record on the split path, the other buffer on the fused one).
+// Returns 1 if the particle moved more than half the skin since the last rebuild.
+template <bool BATCHED>
+__device__ __forceinline__ unsigned finish_particle(const double *R, double *W, unsigned a,
+                                                    double fx, double fy, double fz, double tx,
+                                                    double ty, double tz, unsigned c,
+                                                    const unsigned *wflag, const double *wforce,
+                                                    const InstDev *inst, const InstDev &one,
+                                                    unsigned *contacti, double *fdbg,
+                                                    PRec *tape_rec)
+{
+    double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
+            r5 = ld2(R + 10), r6 = ld2(R + 12);
+    double qz = R[14];
+    uint2 tail = *reinterpret_cast<const uint2 *>(R + 15);
+    if (tape_rec) {   // the record before the update, for the adjoint
+        double *T = reinterpret_cast<double *>(tape_rec + a);
+        st2(T, r0.x, r0.y); st2(T + 2, r1.x, r1.y); st2(T + 4, r2.x, r2.y);
+        st2(T + 6, r3.x, r3.y); st2(T + 8, r4.x, r4.y); st2(T + 10, r5.x, r5.y);
+        st2(T + 12, r6.x, r6.y);
+        T[14] = qz;
+        *reinterpret_cast<uint2 *>(T + 15) = tail;
+    }
+    const InstDev &I = BATCHED ? inst[tail.x] : one;
+    d3 x = d3{r0.x, r0.y, r1.x};
+    d3 v = d3{r2.x, r2.y, r3.x};
+    double m = r3.y;
+    d3 w = d3{r4.x, r4.y, r5.x};
+    double inert = r5.y;
+    // gravity (neiglist.cu:272, 284)
+    fx = fx + 0.0;
+    fy = fy + 0.0;
+    fz = fz + (-m);
+    // walls (computed by k_walls for this rebuild's wall candidates)
+    if (wflag[a]) {
+        const double *WF = wforce + 8 * (size_t)a;
+        double2 w0 = ld2(WF), w1 = ld2(WF + 2), w2 = ld2(WF + 4), w3 = ld2(WF + 6);
+        if (w1.y != 0.0) {
+            fx += w0.x; fy += w0.y; fz += w1.x;
+            tx += w2.x; ty += w2.y; tz += w3.x;
+        }
+    }
+    if (fdbg) {
+        double *D = fdbg + 6 * (size_t)a;
+        D[0] = fx; D[1] = fy; D[2] = fz; D[3] = tx; D[4] = ty; D[5] = tz;
+    }
+    // integrator
+    double dt = I.dt;
+    v.x = v.x + fx * dt * dt / m;
+    v.y = v.y + fy * dt * dt / m;
+    v.z = v.z + fz * dt * dt / m;
+    x.x = x.x + v.x; x.y = x.y + v.y; x.z = x.z + v.z;
+    double qx = r6.x + v.x, qy = r6.y + v.y, qzz = qz + v.z;
+    w.x = w.x + tx * dt * dt / inert;
+    w.y = w.y + ty * dt * dt / inert;
+    w.z = w.z + tz * dt * dt / inert;
+    st2(W, x.x, x.y);
+    st2(W + 2, x.z, r1.y);
+    st2(W + 4, v.x, v.y);
+    st2(W + 6, v.z, m);
+    st2(W + 8, w.x, w.y);
+    st2(W + 10, w.z, inert);
+    st2(W + 12, qx, qy);
+    W[14] = qzz;
+    *reinterpret_cast<uint2 *>(W + 15) = tail;
+    contacti[a] = c;
+    double q2 = qx * qx + qy * qy + qzz * qzz;
+    return q2 > I.ann2 ? 1u : 0u;
+}
+
+// Per-particle kernel of the split path: gather of the particle's pair forces (own pairs in
+// list order, then pairs where it is second, via the transposed index) from k_pair_force's
+// output, then finish_particle.  Max z and contact totals are not reduced here (they are
+// diagnostics): k_inst_reduce computes them on demand.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
@@ -479,11 +573,10 @@ __global__ __launch_bounds__(kBlock) void k_particle(
     InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec)
 {
     const unsigned np = ctl->npairs;
-    unsigned a = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool live = a < n;
     unsigned flag = 0;
     if (live) {
-        double *R = reinterpret_cast<double *>(rec + a);
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
         // Sum in the reference's order (sumallforceiD over an Aref row): the particle's own
@@ -510,66 +603,104 @@ __global__ __launch_bounds__(kBlock) void k_particle(
             tx += o3.y; ty += o4.x; tz += o4.y;
             c += o1.y != 0.0 ? 1u : 0u;
         }
-        double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
-                r5 = ld2(R + 10), r6 = ld2(R + 12);
-        double qz = R[14];
-        uint2 tail = *reinterpret_cast<const uint2 *>(R + 15);
-        if (tape_rec) {   // the record before the update, for the adjoint
-            double *T = reinterpret_cast<double *>(tape_rec + a);
-            st2(T, r0.x, r0.y); st2(T + 2, r1.x, r1.y); st2(T + 4, r2.x, r2.y);
-            st2(T + 6, r3.x, r3.y); st2(T + 8, r4.x, r4.y); st2(T + 10, r5.x, r5.y);
-            st2(T + 12, r6.x, r6.y);
-            T[14] = qz;
-            *reinterpret_cast<uint2 *>(T + 15) = tail;
-        }
-        const InstDev &I = BATCHED ? inst[tail.x] : one;
-        d3 x = d3{r0.x, r0.y, r1.x};
-        double rad = r1.y;
-        d3 v = d3{r2.x, r2.y, r3.x};
-        double m = r3.y;
-        d3 w = d3{r4.x, r4.y, r5.x};
-        double inert = r5.y;
-        // gravity (neiglist.cu:272, 284)
-        fx = fx + 0.0;
-        fy = fy + 0.0;
-        fz = fz + (-m);
-        // walls (computed by k_walls for this rebuild's wall candidates)
-        if (wflag[a]) {
-            const double *W = wforce + 8 * (size_t)a;
-            double2 w0 = ld2(W), w1 = ld2(W + 2), w2 = ld2(W + 4), w3 = ld2(W + 6);
-            if (w1.y != 0.0) {
-                fx += w0.x; fy += w0.y; fz += w1.x;
-                tx += w2.x; ty += w2.y; tz += w3.x;
-            }
-        }
-        (void)rad;
-        if (fdbg) {
-            double *D = fdbg + 6 * (size_t)a;
-            D[0] = fx; D[1] = fy; D[2] = fz; D[3] = tx; D[4] = ty; D[5] = tz;
-        }
-        // integrator
-        double dt = I.dt;
-        v.x = v.x + fx * dt * dt / m;
-        v.y = v.y + fy * dt * dt / m;
-        v.z = v.z + fz * dt * dt / m;
-        x.x = x.x + v.x; x.y = x.y + v.y; x.z = x.z + v.z;
-        double qx = r6.x + v.x, qy = r6.y + v.y, qzz = qz + v.z;
-        w.x = w.x + tx * dt * dt / inert;
-        w.y = w.y + ty * dt * dt / inert;
-        w.z = w.z + tz * dt * dt / inert;
-        st2(R, x.x, x.y);
-        st2(R + 4, v.x, v.y);
-        R[6] = v.z;
-        R[2] = x.z;
-        st2(R + 8, w.x, w.y);
-        R[10] = w.z;
-        st2(R + 12, qx, qy);
-        R[14] = qzz;
-        contacti[a] = c;
-        double q2 = qx * qx + qy * qy + qzz * qzz;
-        flag = q2 > I.ann2 ? 1u : 0u;
+        double *R = reinterpret_cast<double *>(rec + a);
+        flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wflag, wforce, inst,
+                                        one, contacti, fdbg, tape_rec);
     }
     // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
+    if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
+}
+
+// Law of pair (A first, B second) from the records' first 96 bytes.
+template <bool BATCHED>
+__device__ __forceinline__ bool pair_from_records(const double *A, const double *B, d3 h,
+                                                  const InstDev *inst, const LawParams &pp,
+                                                  double dt, unsigned inst_a, d3 &fa, d3 &ta,
+                                                  d3 &tb, d3 &ho, double &fo)
+{
+    double2 a0 = ld2(A), a1 = ld2(A + 2), a2 = ld2(A + 4), a3 = ld2(A + 6), a4 = ld2(A + 8),
+            a5 = ld2(A + 10);
+    double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6), b4 = ld2(B + 8),
+            b5 = ld2(B + 10);
+    LawParams P = pp;
+    double tdt = dt;
+    if (BATCHED) {
+        const InstDev &I = inst[inst_a];
+        P = I.pp;
+        tdt = I.dt;
+    }
+    return law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
+                    d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x}, a1.y, b1.y,
+                    a3.y, b3.y, h, tdt, P, fa, ta, tb, ho, fo);
+}
+
+// Fused step (one lane per particle): the lane evaluates the law of each of its own pairs
+// (it is first; it writes the new history) and of each pair where it is second (the same
+// call with the same arguments, so the same bits as the owner's), summing in the reference
+// order -- own pairs in slot order, then the others in slot order -- and then runs
+// finish_particle.  No per-pair output buffer: each law is evaluated twice instead of its
+// 80-byte result being written once and gathered twice.  Records and histories are read
+// from rin / hin and written to rout / hout, so no lane sees a neighbour's updated state.
+template <bool BATCHED>
+__global__ __launch_bounds__(kBlock) void k_step(
+    const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
+    const unsigned *jt, const Hist *hin, Hist *hout, const unsigned *wflag, const double *wforce,
+    unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
+    PRec *tape_rec, Hist *tape_hist)
+{
+    const unsigned np = ctl->npairs;
+    unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    bool live = a < n;
+    unsigned flag = 0;
+    if (live) {
+        const double *RA = reinterpret_cast<const double *>(rin + a);
+        unsigned inst_a = BATCHED ? rin[a].inst : 0u;
+        double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+        unsigned c = 0;
+        unsigned p0 = off[a], p1 = off[a + 1];
+        if (p1 > np) p1 = np;
+        unsigned q0 = toff[a], q1 = toff[a + 1];
+        if (q1 > np) q1 = np;
+        for (unsigned p = p0; p < p1; p++) {
+            unsigned b = ab[p].y;
+            const double *H = reinterpret_cast<const double *>(hin + p);
+            double2 h0 = ld2(H), h1 = ld2(H + 2);
+            if (tape_hist) {
+                double *T = reinterpret_cast<double *>(tape_hist + p);
+                st2(T, h0.x, h0.y);
+                st2(T + 2, h1.x, h1.y);
+            }
+            d3 fa, ta, tb, ho;
+            double fo;
+            bool con = pair_from_records<BATCHED>(RA, reinterpret_cast<const double *>(rin + b),
+                                                  d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
+                                                  inst_a, fa, ta, tb, ho, fo);
+            double *HW = reinterpret_cast<double *>(hout + p);
+            st2(HW, ho.x, ho.y);
+            st2(HW + 2, ho.z, fo);
+            fx += fa.x; fy += fa.y; fz += fa.z;
+            tx += ta.x; ty += ta.y; tz += ta.z;
+            c += con ? 1u : 0u;
+        }
+        for (unsigned k = q0; k < q1; k++) {
+            unsigned q = jt[k];
+            if (q >= np) continue;
+            unsigned f = ab[q].x;
+            const double *H = reinterpret_cast<const double *>(hin + q);
+            double2 h0 = ld2(H), h1 = ld2(H + 2);
+            d3 fa, ta, tb, ho;
+            double fo;
+            bool con = pair_from_records<BATCHED>(reinterpret_cast<const double *>(rin + f), RA,
+                                                  d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
+                                                  inst_a, fa, ta, tb, ho, fo);
+            fx -= fa.x; fy -= fa.y; fz -= fa.z;
+            tx += tb.x; ty += tb.y; tz += tb.z;
+            c += con ? 1u : 0u;
+        }
+        flag = finish_particle<BATCHED>(RA, reinterpret_cast<double *>(rout + a), a, fx, fy, fz,
+                                        tx, ty, tz, c, wflag, wforce, inst, one, contacti, fdbg,
+                                        tape_rec);
+    }
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 

@@ -175,7 +175,8 @@ struct dem_ctx {
     PRec *rec[2] = {nullptr, nullptr};
     Hist *whist[2] = {nullptr, nullptr};
     unsigned *inv[2] = {nullptr, nullptr};
-    int rc = 0;   // index of the current particle buffers
+    int rc = 0;   // index of the current record buffer (flips at rebuilds and fused steps)
+    int pc = 0;   // index of the current wall histories and inverse slot map (flips at rebuilds)
     unsigned *contacti = nullptr;
     unsigned *wflag = nullptr, *wlist = nullptr;   // wall candidates of the current rebuild
     double *wforce = nullptr;                      // k_walls output, 8 doubles per slot
@@ -187,9 +188,16 @@ struct dem_ctx {
     unsigned *ng = nullptr, *nl = nullptr, *toff = nullptr;
     unsigned *off[2] = {nullptr, nullptr};
     uint2 *ab[2] = {nullptr, nullptr};
-    Hist *hist[2] = {nullptr, nullptr};
     unsigned cap[2] = {0, 0};
     int lc = 0;   // index of the current pair list
+    // Tangential histories of the current list: hist[hc] holds them, hist[1-hc] is the
+    // other buffer (the fused step writes it and flips hc; a rebuild fills it for the new
+    // list and flips hc).  Both always hold at least the current list.
+    Hist *hist[2] = {nullptr, nullptr};
+    unsigned hcap[2] = {0, 0};
+    int hc = 0;
+    bool fused = true;        // fused per-particle step (k_step) vs pair kernel + gather
+    bool out_valid = false;   // fused: rec[1-rc] / hist[1-hc] hold the last step's inputs
     POut *out = nullptr;
     unsigned *jt = nullptr;
     unsigned out_cap = 0;
@@ -215,6 +223,8 @@ struct dem_ctx {
     hipGraph_t graph[2] = {nullptr, nullptr};
     int graph_k = 0;
     unsigned graph_np_grid = 0;
+    bool graph_fused = false;
+    int gpar[4] = {0, 0, 0, 0};   // (rc, lc, hc, pc) after one captured group
     // adjoint recording (null when not recording): pre-step records, incoming histories,
     // rebuild permutation and history-source map
     PRec *tape_rec = nullptr;
@@ -228,15 +238,22 @@ struct dem_ctx {
 
 namespace {
 
-int alloc_pairs(dem_ctx *c, int side, unsigned cap)
+int alloc_list(dem_ctx *c, int side, unsigned cap)
 {
     if (c->ab[side]) DEM_HIP(hipFree(c->ab[side]));
-    if (c->hist[side]) DEM_HIP(hipFree(c->hist[side]));
     c->ab[side] = nullptr;
-    c->hist[side] = nullptr;
     DEM_HIP(hipMalloc(&c->ab[side], (size_t)std::max(cap, 1u) * sizeof(uint2)));
-    DEM_HIP(hipMalloc(&c->hist[side], (size_t)std::max(cap, 1u) * sizeof(Hist)));
     c->cap[side] = cap;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
+int alloc_hist(dem_ctx *c, int side, unsigned cap)
+{
+    if (c->hist[side]) DEM_HIP(hipFree(c->hist[side]));
+    c->hist[side] = nullptr;
+    DEM_HIP(hipMalloc(&c->hist[side], (size_t)std::max(cap, 1u) * sizeof(Hist)));
+    c->hcap[side] = cap;
     c->buf_gen++;
     return DEM_OK;
 }
@@ -291,16 +308,17 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
     int nr = 1 - c->rc;
     if (c->batched)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
-                           n, c->rec[c->rc], c->rec[nr], c->whist[c->rc], c->whist[nr],
-                           c->inv[c->rc], c->inv[nr], (const InstDev *)c->dinst, one, c->wflag,
+                           n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
+                           c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
                            c->wlist, c->ctl, c->tape_perm, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
-                           c->whist[c->rc], c->whist[nr], c->inv[c->rc], c->inv[nr],
+                           c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
                            c->tape_perm, active);
     c->rc = nr;
+    c->pc = 1 - c->pc;
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
@@ -313,8 +331,8 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
     if (scan_excl(c->S, c->ng, n, c->off[nl], c->S.total, st, active)) return DEM_ERR_HIP;
     if (scan_excl(c->S, c->nl, n, c->toff, c->S.total + 1, st, active)) return DEM_ERR_HIP;
     hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, c->ctl, (const unsigned *)c->S.total,
-                       (const unsigned *)(c->S.total + 1), std::min(c->cap[nl], c->out_cap),
-                       active);
+                       (const unsigned *)(c->S.total + 1),
+                       std::min(std::min(c->cap[nl], c->out_cap), c->hcap[1 - c->hc]), active);
     DEM_HIP(hipGetLastError());
     return DEM_OK;
 }
@@ -328,27 +346,31 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     InstDev one = c->hinst[0];
     int nl = 1 - c->lc;
     int ol = c->lc;
-    const unsigned *inv_old = c->have_list ? c->inv[1 - c->rc] : nullptr;
-    unsigned capn = std::min(c->cap[nl], c->out_cap);
+    const unsigned *inv_old = c->have_list ? c->inv[1 - c->pc] : nullptr;
+    int nh = 1 - c->hc, oh = c->hc;
+    unsigned capn = std::min(std::min(c->cap[nl], c->out_cap), c->hcap[nh]);
+    unsigned capo = std::min(std::min(c->cap[ol], c->out_cap), c->hcap[oh]);
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
-                           std::min(c->cap[ol], c->out_cap), c->ctl, c->tape_src, active);
+                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
+                           c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
+                           c->tape_src, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nl], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[ol], c->n_list,
-                           std::min(c->cap[ol], c->out_cap), c->ctl, c->tape_src, active);
+                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
+                           c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
+                           c->tape_src, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
                        active);
     DEM_HIP(hipGetLastError());
     c->lc = nl;
+    c->hc = nh;
     c->have_list = true;
     c->n_list = n;
+    c->out_valid = false;
     return DEM_OK;
 }
 
@@ -366,19 +388,21 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
     DEM_HIP(hipStreamSynchronize(c->stream));
     unsigned np = tot[0];
     int nl = 1 - c->lc;
-    if (np > c->cap[nl]) {
-        unsigned nc = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)np * 3 / 2 + 1024);
-        if ((rc = alloc_pairs(c, nl, nc))) return rc;
-    }
-    if (np > c->out_cap) {
-        unsigned nc = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)np * 3 / 2 + 1024);
-        if ((rc = alloc_out(c, nc))) return rc;
-    }
+    unsigned grow = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)np * 3 / 2 + 1024);
+    if (np > c->cap[nl] && (rc = alloc_list(c, nl, grow))) return rc;
+    if (np > c->hcap[1 - c->hc] && (rc = alloc_hist(c, 1 - c->hc, grow))) return rc;
+    if (np > c->out_cap && (rc = alloc_out(c, grow))) return rc;
     // k_check_cap clamped npairs to the capacity seen before growing; store the real total
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->npairs, np);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->overflow, 0u);
     if (hook && (rc = hook(c, hook_arg, np))) return rc;
     if ((rc = rebuild_fill(c, nullptr))) return rc;
+    if (np > c->hcap[1 - c->hc]) {
+        // the buffer the fused step writes next must hold the new list too (its contents
+        // were the old list's histories, read by the fill just launched)
+        DEM_HIP(hipStreamSynchronize(c->stream));
+        if ((rc = alloc_hist(c, 1 - c->hc, grow))) return rc;
+    }
     c->npairs = np;
     c->rebuilds++;
     c->since = 0;
@@ -386,6 +410,11 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
     return DEM_OK;
 }
 
+// One step.  Split path: k_pair_force (one lane per pair, history in place, 80-byte output
+// per pair) -> k_walls -> k_particle (gather, integrate in place).  Fused path: k_walls ->
+// k_step (one lane per particle, both sides of each pair evaluated, records and histories
+// from the current buffers into the other ones; the buffer indices flip).  `mid` is
+// recorded before the per-particle kernel, `end` after it.
 void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                        hipEvent_t end = nullptr)
 {
@@ -394,40 +423,62 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
     unsigned gb = cdiv(c->n, pb);
     InstDev one = c->hinst[0];
     const unsigned *np = &c->ctl->npairs;
-    if (c->batched) {
-        hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
-                           (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
-                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                           c->tape_hist);
-        hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
-                           (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
-                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce,
-                           c->tape_whist);
-        if (mid) (void)hipEventRecord(mid, st);
+    const PRec *rin = c->rec[c->rc];
+    const InstDev *dinst = (const InstDev *)c->dinst;
+    double *fdbg = c->record_forces ? c->fdbg : nullptr;
+    if (!c->fused) {
+        Hist *h = c->hist[c->hc];
+        if (c->batched)
+            hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
+                               (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
+                               dinst, one.pp, one.dt, c->tape_hist);
+        else
+            hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
+                               (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
+                               dinst, one.pp, one.dt, c->tape_hist);
+    }
+    if (c->batched)
+        hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+                           c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
+                           dinst, one, c->wforce, c->tape_whist);
+    else
+        hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+                           c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
+                           dinst, one, c->wforce, c->tape_whist);
+    if (mid) (void)hipEventRecord(mid, st);
+    if (c->fused) {
+        int nr = 1 - c->rc, nh = 1 - c->hc;
+        // the other record buffer takes the whole updated record; wall histories are per
+        // slot and updated in place by k_walls, so they move with the records
+        if (c->batched)
+            hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
+                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
+                               (const unsigned *)c->toff, (const unsigned *)c->jt,
+                               (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
+                               (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
+                               c->ctl, c->tape_rec, c->tape_hist);
+        else
+            hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
+                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
+                               (const unsigned *)c->toff, (const unsigned *)c->jt,
+                               (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
+                               (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
+                               c->ctl, c->tape_rec, c->tape_hist);
+        c->rc = nr;
+        c->hc = nh;
+        c->out_valid = true;
+    } else if (c->batched) {
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
-                           (const unsigned *)c->off[c->lc],
-                           (const unsigned *)c->toff, (const unsigned *)c->jt,
-                           (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
-                           c->tape_rec);
+                           (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
+                           (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
+                           c->contacti, fdbg, c->ctl, c->tape_rec);
     } else {
-        hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
-                           (const uint2 *)c->ab[c->lc], np, (const PRec *)c->rec[c->rc],
-                           c->hist[c->lc], c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                           c->tape_hist);
-        hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st,
-                           (const PRec *)c->rec[c->rc], c->whist[c->rc], (const unsigned *)c->wlist,
-                           (const DevCtl *)c->ctl, (const InstDev *)c->dinst, one, c->wforce,
-                           c->tape_whist);
-        if (mid) (void)hipEventRecord(mid, st);
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
-                           (const unsigned *)c->off[c->lc],
-                           (const unsigned *)c->toff, (const unsigned *)c->jt,
-                           (const POut *)c->out, c->n, (const InstDev *)c->dinst, one,
-                           c->contacti, c->record_forces ? c->fdbg : nullptr, c->ctl,
-                           c->tape_rec);
+                           (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
+                           (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
+                           c->contacti, fdbg, c->ctl, c->tape_rec);
     }
     if (end) (void)hipEventRecord(end, st);
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d);
@@ -544,22 +595,31 @@ void drop_graph(dem_ctx *c)
     c->graph_k = 0;
 }
 
-// Grow one side of the pair buffers keeping its contents (the live list and history).
-int grow_pairs_keep(dem_ctx *c, int side, unsigned cap)
+// Grow a list / history buffer keeping its contents (the live list and history).
+int grow_list_keep(dem_ctx *c, int side, unsigned cap)
 {
     if (c->cap[side] >= cap) return DEM_OK;
     uint2 *nab = nullptr;
-    Hist *nh = nullptr;
     DEM_HIP(hipMalloc(&nab, (size_t)cap * sizeof(uint2)));
-    DEM_HIP(hipMalloc(&nh, (size_t)cap * sizeof(Hist)));
     DEM_HIP(hipMemcpyAsync(nab, c->ab[side], (size_t)c->cap[side] * sizeof(uint2), hipMemcpyDeviceToDevice, c->stream));
-    DEM_HIP(hipMemcpyAsync(nh, c->hist[side], (size_t)c->cap[side] * sizeof(Hist), hipMemcpyDeviceToDevice, c->stream));
     DEM_HIP(hipStreamSynchronize(c->stream));
     DEM_HIP(hipFree(c->ab[side]));
-    DEM_HIP(hipFree(c->hist[side]));
     c->ab[side] = nab;
-    c->hist[side] = nh;
     c->cap[side] = cap;
+    c->buf_gen++;
+    return DEM_OK;
+}
+
+int grow_hist_keep(dem_ctx *c, int side, unsigned cap)
+{
+    if (c->hcap[side] >= cap) return DEM_OK;
+    Hist *nh = nullptr;
+    DEM_HIP(hipMalloc(&nh, (size_t)cap * sizeof(Hist)));
+    DEM_HIP(hipMemcpyAsync(nh, c->hist[side], (size_t)c->hcap[side] * sizeof(Hist), hipMemcpyDeviceToDevice, c->stream));
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    DEM_HIP(hipFree(c->hist[side]));
+    c->hist[side] = nh;
+    c->hcap[side] = cap;
     c->buf_gen++;
     return DEM_OK;
 }
@@ -682,7 +742,7 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
         DEM_HIP(hipMalloc(&c->whist[k], sizeof(Hist) * (size_t)n));
         DEM_HIP(hipMalloc(&c->inv[k], sizeof(unsigned) * (size_t)n));
         DEM_HIP(hipMalloc(&c->off[k], sizeof(unsigned) * ((size_t)n + 1)));
-        if (alloc_pairs(c, k, pc)) { free_ctx(c); return DEM_ERR_HIP; }
+        if (alloc_list(c, k, pc) || alloc_hist(c, k, pc)) { free_ctx(c); return DEM_ERR_HIP; }
     }
     if (alloc_out(c, pc)) { free_ctx(c); return DEM_ERR_HIP; }
     DEM_HIP(hipMalloc(&c->contacti, sizeof(unsigned) * (size_t)n));
@@ -743,6 +803,11 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     }
+    case DEM_OPT_FUSED_STEP:
+        c->fused = value != 0.0;
+        c->out_valid = false;
+        drop_graph(c);
+        return DEM_OK;
     case DEM_OPT_ADJ_ROLL_REG:
         if (!(value >= 0.0)) return DEM_ERR_INVALID;
         c->adj_roll_k = value;
@@ -798,6 +863,7 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
     }
     DEM_HIP(hipMemset(c->contacti, 0, sizeof(unsigned) * n));
     c->have_list = false;
+    c->out_valid = false;
     c->need_rebuild = true;
     c->npairs = 0;
     DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
@@ -862,7 +928,7 @@ int dem_get_wall_history(dem_ctx *c, double *disptw, double *fricpw)
     if (rc) return rc;
     size_t n = c->cap_n;
     std::vector<Hist> wh(n);
-    DEM_HIP(hipMemcpy(wh.data(), c->whist[c->rc], sizeof(Hist) * n, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(wh.data(), c->whist[c->pc], sizeof(Hist) * n, hipMemcpyDeviceToHost));
     for (size_t s = 0; s < n; s++) {
         size_t i = recs[s].id;
         if (disptw) { disptw[3 * i] = wh[s].hx; disptw[3 * i + 1] = wh[s].hy; disptw[3 * i + 2] = wh[s].hz; }
@@ -930,9 +996,15 @@ int dem_step_graph(dem_ctx *c, int nsteps)
     if (c->need_rebuild || !c->have_list || c->since != 0) {
         if ((rc = rebuild_host(c))) return rc;
     }
+    // one capacity for every pair buffer (lists, histories, output and transposed index), so
+    // the kernels captured for either parity agree on it
     unsigned want = (unsigned)std::min<unsigned long long>(0xfffffff0ull, (unsigned long long)c->npairs * 3 / 2 + 4096);
-    if (c->cap[1 - c->lc] < want && (rc = alloc_pairs(c, 1 - c->lc, want))) return rc;
-    if ((rc = grow_pairs_keep(c, c->lc, want))) return rc;
+    want = std::max(want, std::max(std::max(c->cap[0], c->cap[1]), std::max(c->hcap[0], c->hcap[1])));
+    want = std::max(want, c->out_cap);
+    if (c->cap[1 - c->lc] < want && (rc = alloc_list(c, 1 - c->lc, want))) return rc;
+    if ((rc = grow_list_keep(c, c->lc, want))) return rc;
+    if (c->hcap[1 - c->hc] < want && (rc = alloc_hist(c, 1 - c->hc, want))) return rc;
+    if ((rc = grow_hist_keep(c, c->hc, want))) return rc;
     if (c->out_cap < want) {
         // out / jt hold the live transposed index: keep jt's contents
         POut *no = nullptr;
@@ -948,17 +1020,9 @@ int dem_step_graph(dem_ctx *c, int nsteps)
         c->out_cap = want;
         c->buf_gen++;
     }
-    unsigned capmin = std::min(std::min(c->cap[0], c->cap[1]), c->out_cap);
-    unsigned np_grid = pair_grid(c, capmin);
-    if (c->cap[0] != c->cap[1]) {
-        // both sides share one capacity so the two graphs agree on it
-        unsigned cm = std::max(c->cap[0], c->cap[1]);
-        if ((rc = alloc_pairs(c, 1 - c->lc, cm))) return rc;
-        if ((rc = grow_pairs_keep(c, c->lc, cm))) return rc;
-        capmin = std::min(cm, c->out_cap);
-        np_grid = pair_grid(c, capmin);
-    }
-    if (!c->gexec[0] || c->graph_k != K || c->graph_np_grid != np_grid || c->graph_gen != c->buf_gen) {
+    unsigned np_grid = pair_grid(c, want);
+    if (!c->gexec[0] || c->graph_k != K || c->graph_np_grid != np_grid || c->graph_gen != c->buf_gen ||
+        c->graph_fused != c->fused) {
         drop_graph(c);
         hipStream_t st = c->stream;
         for (int g = 0; g < 2; g++) {
@@ -975,11 +1039,17 @@ int dem_step_graph(dem_ctx *c, int nsteps)
             }
             c->graph[g] = gr;
             DEM_HIP(hipGraphInstantiate(&c->gexec[g], gr, nullptr, nullptr, 0));
+            // buffer parities after one group: the second graph starts from them
+            if (g == 0) {
+                c->gpar[0] = c->rc; c->gpar[1] = c->lc; c->gpar[2] = c->hc; c->gpar[3] = c->pc;
+            }
         }
-        // two captures flipped the buffer indices twice: back to the live parity
+        // a group flips each parity the same way every time, so two groups restore them:
+        // the host indices are back at the live state
         c->graph_k = K;
         c->graph_np_grid = np_grid;
         c->graph_gen = c->buf_gen;
+        c->graph_fused = c->fused;
     }
     int groups = nsteps / K;
     DEM_HIP(hipEventRecord(c->ev0, c->stream));
@@ -988,13 +1058,15 @@ int dem_step_graph(dem_ctx *c, int nsteps)
     DEM_HIP(hipEventSynchronize(c->ev1));
     DEM_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
     if (groups & 1) {
-        // odd number of groups: the live buffers are the other parity, and the graphs
+        // odd number of groups: the live buffers are those after one group, and the graphs
         // must start from the other one next time
-        c->rc = 1 - c->rc;
-        c->lc = 1 - c->lc;
+        int r0 = c->rc, l0 = c->lc, h0 = c->hc, p0 = c->pc;
+        c->rc = c->gpar[0]; c->lc = c->gpar[1]; c->hc = c->gpar[2]; c->pc = c->gpar[3];
+        c->gpar[0] = r0; c->gpar[1] = l0; c->gpar[2] = h0; c->gpar[3] = p0;
         std::swap(c->gexec[0], c->gexec[1]);
         std::swap(c->graph[0], c->graph[1]);
     }
+    c->out_valid = false;
     unsigned w[2];
     DEM_HIP(hipMemcpy(w, &c->ctl->npairs, 2 * sizeof(unsigned), hipMemcpyDeviceToHost));
     c->npairs = w[0];
@@ -1064,19 +1136,18 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
 
 static int reduce_instances(dem_ctx *c)
 {
-    // per-instance max z / contact totals into the mirror's tail (zeroed host-side first; the
-    // kernel writes device-visible coherent host memory directly)
-    unsigned long long *mz = reinterpret_cast<unsigned long long *>(c->mirror_h + sizeof(DevCtl));
-    unsigned *mp = reinterpret_cast<unsigned *>(mz + c->ninst);
-    DEM_HIP(hipStreamSynchronize(c->stream));
-    std::memset(mz, 0, sizeof(unsigned long long) * c->ninst);
-    std::memset(mp, 0, sizeof(unsigned) * c->ninst);
-    unsigned long long *dz = reinterpret_cast<unsigned long long *>(c->mirror_d + sizeof(DevCtl));
+    // per-instance max z / contact totals, reduced into the device control block's tail (the
+    // per-wave atomics stay in HBM) and copied to the mirror's tail
+    size_t tail = c->ctl_bytes - sizeof(DevCtl);
+    unsigned char *dt = reinterpret_cast<unsigned char *>(c->ctl) + sizeof(DevCtl);
+    unsigned long long *dz = reinterpret_cast<unsigned long long *>(dt);
     unsigned *dp = reinterpret_cast<unsigned *>(dz + c->ninst);
+    DEM_HIP(hipMemsetAsync(dt, 0, tail, c->stream));
     unsigned g = std::max(1u, std::min(cdiv(c->n, kBlock), (unsigned)c->cus * 4u));
     hipLaunchKernelGGL(k_inst_reduce, dim3(g), dim3(kBlock), 0, c->stream,
                        (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti, c->n, dz, dp);
     DEM_HIP(hipGetLastError());
+    DEM_HIP(hipMemcpyAsync(c->mirror_h + sizeof(DevCtl), dt, tail, hipMemcpyDeviceToHost, c->stream));
     DEM_HIP(hipStreamSynchronize(c->stream));
     return DEM_OK;
 }
@@ -1285,8 +1356,35 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
             ids[2 * p + 1] = recs[ab[p].y].id;
         }
     }
-    if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
-    if (out) DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
+    if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->hc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
+    if (out) {
+        if (c->fused) {
+            // the fused step keeps no per-pair output: evaluate the last step's pair laws
+            // again from its inputs, which stay in the other buffers until the next step
+            // (zeros when no step has run on the current list)
+            if (!c->out_valid) {
+                std::memset(out, 0, sizeof(POut) * np);
+                return DEM_OK;
+            }
+            InstDev one = c->hinst[0];
+            unsigned g = pair_grid(c, (unsigned)np);
+            if (c->batched)
+                hipLaunchKernelGGL(k_pair_force<true>, dim3(g), dim3(c->pair_block), 0, c->stream,
+                                   (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
+                                   (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
+                                   (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
+                                   (Hist *)nullptr);
+            else
+                hipLaunchKernelGGL(k_pair_force<false>, dim3(g), dim3(c->pair_block), 0, c->stream,
+                                   (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
+                                   (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
+                                   (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
+                                   (Hist *)nullptr);
+            DEM_HIP(hipGetLastError());
+            DEM_HIP(hipStreamSynchronize(c->stream));
+        }
+        DEM_HIP(hipMemcpy(out, c->out, sizeof(POut) * np, hipMemcpyDeviceToHost));
+    }
     return DEM_OK;
 }
 
@@ -1752,8 +1850,8 @@ int snap_save(dem_ctx *c, AdjSnap &S)
         DEM_HIP(hipMalloc(&S.jt, sizeof(unsigned) * S.pcap));
     }
     DEM_HIP(hipMemcpyAsync(S.rec, c->rec[c->rc], sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
-    DEM_HIP(hipMemcpyAsync(S.whist, c->whist[c->rc], sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
-    DEM_HIP(hipMemcpyAsync(S.inv, c->inv[c->rc], sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(S.whist, c->whist[c->pc], sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(S.inv, c->inv[c->pc], sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
     DEM_HIP(hipMemcpyAsync(S.ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
     if (c->have_list) {
         DEM_HIP(hipMemcpyAsync(S.off, c->off[c->lc], sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
@@ -1762,7 +1860,7 @@ int snap_save(dem_ctx *c, AdjSnap &S)
         DEM_HIP(hipMemcpyAsync(S.wlist, c->wlist, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
         if (np) {
             DEM_HIP(hipMemcpyAsync(S.ab, c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
-            DEM_HIP(hipMemcpyAsync(S.hist, c->hist[c->lc], sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(S.hist, c->hist[c->hc], sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
             DEM_HIP(hipMemcpyAsync(S.jt, c->jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
         }
     }
@@ -1784,11 +1882,14 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
     size_t n = c->n, np = S.npairs;
     int rc;
     DEM_HIP(hipStreamSynchronize(st));
-    if (np > c->cap[c->lc] && (rc = alloc_pairs(c, c->lc, (unsigned)(np * 3 / 2 + 1024)))) return rc;
+    unsigned grow = (unsigned)(np * 3 / 2 + 1024);
+    if (np > c->cap[c->lc] && (rc = alloc_list(c, c->lc, grow))) return rc;
+    for (int k = 0; k < 2; k++)
+        if (np > c->hcap[k] && (rc = alloc_hist(c, k, grow))) return rc;
     if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
     DEM_HIP(hipMemcpyAsync(c->rec[c->rc], S.rec, sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
-    DEM_HIP(hipMemcpyAsync(c->whist[c->rc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
-    DEM_HIP(hipMemcpyAsync(c->inv[c->rc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(c->whist[c->pc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
+    DEM_HIP(hipMemcpyAsync(c->inv[c->pc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
     DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
     if (S.have_list) {
         DEM_HIP(hipMemcpyAsync(c->off[c->lc], S.off, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
@@ -1797,7 +1898,7 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
         DEM_HIP(hipMemcpyAsync(c->wlist, S.wlist, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
         if (np) {
             DEM_HIP(hipMemcpyAsync(c->ab[c->lc], S.ab, sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
-            DEM_HIP(hipMemcpyAsync(c->hist[c->lc], S.hist, sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
+            DEM_HIP(hipMemcpyAsync(c->hist[c->hc], S.hist, sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
             DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
         }
     }
