@@ -7,7 +7,8 @@ dt = 1e-6 s, FP64, reference rebuild policy (displacement trigger).  Inputs are 
 before timing.  One step = [list rebuild when triggered] + pair forces + gather/walls/gravity/
 integration.
 
-The step runs the fused per-particle kernel (k_step) unless --split is given.
+The step path is chosen by the library (auto: one timed step of each path after the first and
+every 8th rebuild); --fused / --split force one.
 
 Prints ONE JSON line.  The CPU baseline is the oracle (the same PCCP algorithm in C with
 OpenMP) on a bounded sample of the same workload.
@@ -156,6 +157,38 @@ def bench_batched(dem, steps, interval, tune_steps=48):
             "autotune": tune}
 
 
+def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False):
+    """Dense window: the same 500K workload poured into a bed on the GPU (graph steps), then
+    `steps` plain steps from that bed with the displacement trigger."""
+    t0 = time.perf_counter()
+    ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0)
+    log(f"dense window: bed poured in {time.perf_counter() - t0:.1f} s "
+        f"(top {ws.meta['bed_top']:.1f})")
+    ctx = dem.Context(ws.instances)
+    if split is not None:
+        ctx.set_option(dem.OPT_FUSED_STEP, 0 if split else 1)
+    ctx.load(ws)
+    ctx.step(20)
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    kt = ctx.time_kernels(20)
+    P, N = s1.npairs, ws.n
+    out = {"value": N * steps / el, "steps": steps, "ms_per_step": el / steps * 1e3, "npairs": P,
+           "step_path": "fused" if ctx.stats().step_path else "split",
+           "pairs_per_particle": round(P / N, 3), "rebuilds": int(s1.rebuilds - s0.rebuilds),
+           "step_kernel_us": kt["particle_ms"] * 1e3, "walls_us": kt["pair_ms"] * 1e3,
+           "rebuild_ms": kt["rebuild_ms"], "settle_steps": settle_steps}
+    log(f"dense window: {out['value']:.4e} particle-steps/s, P/N {out['pairs_per_particle']}, "
+        f"{out['rebuilds']} rebuilds in {steps} steps, step kernel {out['step_kernel_us']:.1f} us")
+    ctx.close()
+    return out
+
+
 def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4):
     """Differentiable form on the 20K-sphere bed: checkpointed forward + adjoint backward
     against the plain forward (cost ratio), and the adjoint gradient of phi and CN in
@@ -288,12 +321,15 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--batched-interval", type=int, default=20)
+    ap.add_argument("--no-dense", action="store_true")
+    ap.add_argument("--dense-steps", type=int, default=1000)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--calib-instances", type=int, default=64)
     ap.add_argument("--calib-n", type=int, default=2000)
     ap.add_argument("--split", action="store_true",
-                    help="time the split path (pair kernel + gather) instead of the fused step")
+                    help="force the split path (pair kernel + gather); default: auto")
+    ap.add_argument("--fused", action="store_true", help="force the fused step; default: auto")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run 3 device-to-device copies of this many bytes (the known-size "
                          "kernel that scales the HBM traffic counters)")
@@ -319,7 +355,8 @@ def main():
     w = dem.config1(n=args.n)
     log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
     ctx = dem.Context(w.instances)
-    ctx.set_option(dem.OPT_FUSED_STEP, 0 if args.split else 1)
+    if args.split or args.fused:
+        ctx.set_option(dem.OPT_FUSED_STEP, 0 if args.split else 1)
     ctx.load(w)
     ctx.step(args.warmup)
     ctx.sync()
@@ -335,7 +372,9 @@ def main():
     kt = ctx.time_kernels(args.kernel_steps)
     P, N = s1.npairs, w.n
     kb = kernel_bytes(P, N)
-    if args.split:
+    fused = bool(ctx.stats().step_path)
+    log(f"step path: {'fused' if fused else 'split'}")
+    if not fused:
         dom = "pair" if kt["pair_ms"] >= kt["particle_ms"] else "particle"
         ms = kt[f"{dom}_ms"]
         kname = "k_pair_force" if dom == "pair" else "k_particle"
@@ -351,6 +390,10 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname, N),
             "kernel": kname, "bytes_per_launch": kb[dom], "avg_us": ms * 1e3}
     del ctx
+    dense = None
+    if not args.no_dense:
+        dense = bench_dense(dem, w, steps=args.dense_steps,
+                            split=True if args.split else (False if args.fused else None))
     batched = None
     if not args.no_batched:
         batched = bench_batched(dem, args.batched_steps, args.batched_interval)
@@ -383,12 +426,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": w.name, "n": N, "npairs": P, "pairs_per_particle": round(P / N, 3),
-                   "step_path": "split" if args.split else "fused",
+                   "step_path": "fused" if fused else "split",
                    "rebuild_policy": "displacement trigger |q| > 0.275",
                    "rebuilds_timed": int(s1.rebuilds - s0.rebuilds),
                    "kernel_ms": {k: round(v, 5) for k, v in kt.items() if k.endswith("_ms")}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "dense_window": dense,
         "batched": batched,
         "adjoint": adjoint,
     }

@@ -94,7 +94,7 @@ class Stats(ctypes.Structure):
         ("overflow", c_uint),
         ("maxhz", c_double),
         ("pcntot", c_uint),
-        ("pad", c_uint),
+        ("step_path", c_uint),
     ]
 
 
@@ -102,7 +102,7 @@ class TuneResult(ctypes.Structure):
     _fields_ = [
         ("skin", c_double), ("cell", c_double), ("interval", c_int),
         ("pair_block", c_uint), ("pair_grid_per_cu", c_uint), ("particle_block", c_uint),
-        ("ms_per_step", c_double), ("score_ms", c_double), ("trials", c_int), ("pad", c_int),
+        ("ms_per_step", c_double), ("score_ms", c_double), ("trials", c_int), ("step_path", c_int),
     ]
 
 
@@ -508,7 +508,13 @@ def settle_bed(w: Workload, steps: int = 100_000, interval: int = 50, tap: float
     ctx = Context(w.instances)
     ctx.set_state(w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
     ctx.set_option(OPT_REBUILD_INTERVAL, interval)
-    ctx.step_graph((steps // interval) * interval)
+    # graph replays in chunks: each call sizes the pair buffers for the list it starts from
+    chunk = max(interval, (10_000 // interval) * interval)
+    left = (steps // interval) * interval
+    while left > 0:
+        k = min(chunk, left)
+        ctx.step_graph(k)
+        left -= k
     st = ctx.state()
     ctx.close()
     dt = w.instances[0].dt
@@ -592,7 +598,7 @@ class Context:
         """Run the project autotuner (advances the simulation by its trial steps)."""
         r = TuneResult()
         _check(self._L.dem_autotune(self._h, int(steps), ctypes.byref(r)), "dem_autotune")
-        return {k: getattr(r, k) for k, _ in TuneResult._fields_ if k != "pad"}
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
 
     def set_active(self, n: int):
         _check(self._L.dem_set_active(self._h, int(n)), "dem_set_active")

@@ -179,6 +179,8 @@ struct dem_ctx {
     int pc = 0;   // index of the current wall histories and inverse slot map (flips at rebuilds)
     unsigned *contacti = nullptr;
     unsigned *wflag = nullptr, *wlist = nullptr;   // wall candidates of the current rebuild
+    double4 *cpos = nullptr;   // position + radius per slot, written by the reorder
+    unsigned *cid = nullptr;   // particle index per slot, written by the reorder
     double *wforce = nullptr;                      // k_walls output, 8 doubles per slot
     double *fdbg = nullptr;
     bool record_forces = false;
@@ -196,7 +198,14 @@ struct dem_ctx {
     Hist *hist[2] = {nullptr, nullptr};
     unsigned hcap[2] = {0, 0};
     int hc = 0;
-    bool fused = true;        // fused per-particle step (k_step) vs pair kernel + gather
+    bool fused = true;        // step path in use: fused per-particle (k_step) or split
+    int fused_mode = 2;       // DEM_OPT_FUSED_STEP: 0 split, 1 fused, 2 auto (timed probes)
+    int probe_left = 0;       // auto: steps left in the current probe (fused, then split)
+    unsigned reb_since_probe = 0, probe_every = 8;
+    bool probed = false;
+    float t_probe[2] = {0.f, 0.f};
+    hipEvent_t evp0 = nullptr, evp1 = nullptr;
+    int last_path = 0;        // path of the last step: 0 none, 1 split, 2 fused
     bool out_valid = false;   // fused: rec[1-rc] / hist[1-hc] hold the last step's inputs
     POut *out = nullptr;
     unsigned *jt = nullptr;
@@ -310,21 +319,23 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
                            c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
-                           c->wlist, c->ctl, c->tape_perm, active);
+                           c->wlist, c->ctl, c->tape_perm, c->cpos, c->cid, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
                            c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
-                           c->tape_perm, active);
+                           c->tape_perm, c->cpos, c->cid, active);
     c->rc = nr;
     c->pc = 1 - c->pc;
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const double4 *)c->cpos, (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->ng, c->nl, active);
     else
         hipLaunchKernelGGL(k_nbr_count<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
+                           (const double4 *)c->cpos, (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->ng, c->nl, active);
     int nl = 1 - c->lc;
@@ -352,13 +363,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     unsigned capo = std::min(std::min(c->cap[ol], c->out_cap), c->hcap[oh]);
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const double4 *)c->cpos, (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const double4 *)c->cpos, (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
@@ -467,6 +478,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
+        c->last_path = 2;
     } else if (c->batched) {
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
@@ -480,6 +492,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                            (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
                            c->contacti, fdbg, c->ctl, c->tape_rec);
     }
+    if (!c->fused) c->last_path = 1;
     if (end) (void)hipEventRecord(end, st);
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d);
 }
@@ -557,7 +570,7 @@ void free_ctx(dem_ctx *c)
     if (!c) return;
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
-                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos, c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->ctl};
     for (void *p : bufs)
@@ -571,6 +584,8 @@ void free_ctx(dem_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evs) (void)hipEventDestroy(c->evs);
+    if (c->evp0) (void)hipEventDestroy(c->evp0);
+    if (c->evp1) (void)hipEventDestroy(c->evp1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -735,6 +750,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipEventCreate(&c->ev0));
     DEM_HIP(hipEventCreate(&c->ev1));
     DEM_HIP(hipEventCreateWithFlags(&c->evs, hipEventDisableTiming));
+    DEM_HIP(hipEventCreate(&c->evp0));
+    DEM_HIP(hipEventCreate(&c->evp1));
     DEM_HIP(hipMalloc(&c->dinst, sizeof(InstDev) * ninst));
     DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice));
     for (int k = 0; k < 2; k++) {
@@ -749,6 +766,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->wflag, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->wlist, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->wforce, sizeof(double) * 8 * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cpos, sizeof(double4) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cid, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_b, sizeof(uint2) * (size_t)n));
@@ -803,11 +822,16 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     }
-    case DEM_OPT_FUSED_STEP:
-        c->fused = value != 0.0;
-        c->out_valid = false;
+    case DEM_OPT_FUSED_STEP: {
+        int m = (int)value;
+        if (m < 0 || m > 2 || (double)m != value) return DEM_ERR_INVALID;
+        c->fused_mode = m;
+        if (m < 2) c->fused = m == 1;
+        c->probed = false;
+        c->probe_left = 0;
         drop_graph(c);
         return DEM_OK;
+    }
     case DEM_OPT_ADJ_ROLL_REG:
         if (!(value >= 0.0)) return DEM_ERR_INVALID;
         c->adj_roll_k = value;
@@ -864,6 +888,7 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
     DEM_HIP(hipMemset(c->contacti, 0, sizeof(unsigned) * n));
     c->have_list = false;
     c->out_valid = false;
+    c->last_path = 0;
     c->need_rebuild = true;
     c->npairs = 0;
     DEM_HIP(hipMemset(c->ctl, 0, c->ctl_bytes));
@@ -960,9 +985,29 @@ int dem_step(dem_ctx *c, int nsteps)
         if (c->need_rebuild || (c->interval > 0 && c->since >= (unsigned)c->interval)) {
             int rc = rebuild_host(c);
             if (rc) return rc;
+            // auto path: after the first rebuild and every probe_every-th one, time one step
+            // of each path (same results bit for bit) and keep the faster
+            if (c->fused_mode == 2 && (!c->probed || ++c->reb_since_probe >= c->probe_every)) {
+                c->probe_left = 2;
+                c->reb_since_probe = 0;
+            }
+        }
+        bool probe = c->probe_left > 0;
+        if (probe) {
+            c->fused = c->probe_left == 2;
+            DEM_HIP(hipEventRecord(c->evp0, c->stream));
         }
         launch_force_step(c, pair_grid(c, c->npairs));
         DEM_HIP(hipGetLastError());
+        if (probe) {
+            DEM_HIP(hipEventRecord(c->evp1, c->stream));
+            DEM_HIP(hipEventSynchronize(c->evp1));
+            DEM_HIP(hipEventElapsedTime(&c->t_probe[c->fused ? 1 : 0], c->evp0, c->evp1));
+            if (--c->probe_left == 0) {
+                c->fused = c->t_probe[1] <= c->t_probe[0];
+                c->probed = true;
+            }
+        }
         c->steps++;
         c->since++;
         if (c->interval == 0) {
@@ -1226,28 +1271,52 @@ int dem_autotune(dem_ctx *c, int steps, dem_tune_result *out)
     }
     int rc = regrid(c, best_skin);
     if (rc) return rc;
-    // launch shapes at the chosen skin (force steps only)
+    // launch shapes at the chosen skin (force steps only): the split path's pair block, pair
+    // grid and particle block; the fused path's block; and, in auto mode, the faster path
     static const unsigned blocks[] = {64, 128, 256};
     static const unsigned grids[] = {4, 8, 16, 32};
-    double best_t = 1e300;
-    unsigned bp = c->pair_block, bg = c->pair_grid_per_cu, bq = c->particle_block;
+    bool try_split = c->fused_mode != 1, try_fused = c->fused_mode != 0;
+    double best_t = 1e300, best_fused_t = 1e300;
+    unsigned bp = c->pair_block, bg = c->pair_grid_per_cu, bq = c->particle_block, fq = c->particle_block;
     int sub = std::max(4, steps / 4);
-    for (unsigned pb : blocks)
-        for (unsigned pg : grids)
-            for (unsigned qb : blocks) {
-                if (qb != 256 && !(pb == 256 && pg == 16)) continue;   // particle block swept once
-                c->pair_block = pb;
-                c->pair_grid_per_cu = pg;
-                c->particle_block = qb;
-                double tf, tr;
-                int gap;
-                if ((rc = tune_trial(c, sub, &tf, &tr, &gap, e0, e1))) return rc;
-                trials++;
-                if (tf < best_t) { best_t = tf; bp = pb; bg = pg; bq = qb; }
-            }
+    if (try_split) {
+        c->fused = false;
+        for (unsigned pb : blocks)
+            for (unsigned pg : grids)
+                for (unsigned qb : blocks) {
+                    if (qb != 256 && !(pb == 256 && pg == 16)) continue;   // particle block swept once
+                    c->pair_block = pb;
+                    c->pair_grid_per_cu = pg;
+                    c->particle_block = qb;
+                    double tf, tr;
+                    int gap;
+                    if ((rc = tune_trial(c, sub, &tf, &tr, &gap, e0, e1))) return rc;
+                    trials++;
+                    if (tf < best_t) { best_t = tf; bp = pb; bg = pg; bq = qb; }
+                }
+    }
+    if (try_fused) {
+        c->fused = true;
+        for (unsigned qb : blocks) {
+            c->particle_block = qb;
+            double tf, tr;
+            int gap;
+            if ((rc = tune_trial(c, sub, &tf, &tr, &gap, e0, e1))) return rc;
+            trials++;
+            if (tf < best_fused_t) { best_fused_t = tf; fq = qb; }
+        }
+    }
+    bool use_fused = try_fused && (!try_split || best_fused_t < best_t);
+    c->fused = use_fused;
     c->pair_block = bp;
     c->pair_grid_per_cu = bg;
-    c->particle_block = bq;
+    c->particle_block = use_fused ? fq : bq;
+    if (use_fused) best_t = best_fused_t;
+    if (c->fused_mode == 2) {
+        c->probed = true;
+        c->reb_since_probe = 0;
+        c->probe_left = 0;
+    }
     c->interval = saved_interval;
     drop_graph(c);
     (void)hipEventDestroy(e0);
@@ -1257,7 +1326,8 @@ int dem_autotune(dem_ctx *c, int steps, dem_tune_result *out)
     out->interval = best_k;
     out->pair_block = bp;
     out->pair_grid_per_cu = bg;
-    out->particle_block = bq;
+    out->particle_block = c->particle_block;
+    out->step_path = use_fused ? 1 : 0;
     out->ms_per_step = best_t;
     out->score_ms = best_score;
     out->trials = trials;
@@ -1286,6 +1356,7 @@ int dem_get_stats(dem_ctx *c, dem_stats *o)
     std::memcpy(&z, &zb, 8);
     o->maxhz = z;
     o->pcntot = mp[0];
+    o->step_path = c->fused ? 1u : 0u;
     return DEM_OK;
 }
 
@@ -1358,7 +1429,7 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
     }
     if (hist) DEM_HIP(hipMemcpy(hist, c->hist[c->hc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
     if (out) {
-        if (c->fused) {
+        if (c->last_path == 2) {
             // the fused step keeps no per-pair output: evaluate the last step's pair laws
             // again from its inputs, which stay in the other buffers until the next step
             // (zeros when no step has run on the current list)

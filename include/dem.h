@@ -183,7 +183,7 @@ typedef struct dem_stats {
     uint overflow;
     double maxhz;                  /* instance 0: max(0, max particle z) after the last step */
     uint pcntot;                   /* instance 0: sum of per-particle contact counts */
-    uint pad;
+    uint step_path;                /* step path in use: 1 fused, 0 split */
 } dem_stats;
 
 /* Options for dem_set_option. */
@@ -197,9 +197,11 @@ enum {
     DEM_OPT_ADJ_ROLL_REG = 8,      /* adjoint: floor of |w_rel| in the rolling moment's
                                       linearisation, in units of the one-step chatter scale
                                       mu_r fcn dt^2 (1/I_a + 1/I_b); 0 = exact, default 1 */
-    DEM_OPT_FUSED_STEP = 9         /* 1 (default): fused per-particle step, each pair law
-                                      evaluated from both sides, no per-pair output buffer;
-                                      0: pair kernel + gather.  Same results bit for bit */
+    DEM_OPT_FUSED_STEP = 9         /* 1: fused per-particle step, each pair law evaluated
+                                      from both sides, no per-pair output buffer; 0: pair
+                                      kernel + gather; 2 (default): auto -- dem_step times one
+                                      step of each after the first and every 8th rebuild and
+                                      keeps the faster.  Same results bit for bit */
 };
 
 /* Result of dem_autotune. */
@@ -211,7 +213,7 @@ typedef struct dem_tune_result {
     double ms_per_step;            /* force-step time at the chosen shapes */
     double score_ms;               /* force step + rebuild / interval at the chosen skin */
     int trials;
-    int pad;
+    int step_path;                 /* 1 fused, 0 split */
 } dem_tune_result;
 
 /* Create a context for `ninstances` independent packings (1 = plain form).  pair_capacity

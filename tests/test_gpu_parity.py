@@ -232,9 +232,10 @@ def test_graph_equals_host_loop(dem, gpu_ok):
 
 @pytest.mark.parametrize("mode", ["trigger", "interval", "graph", "batched", "odd-graph"])
 def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
-    """The fused per-particle step (default) and the pair-kernel + gather path give the same
-    states, histories, contacts and pair outputs bit for bit, under both rebuild policies,
-    in graphs (even and odd group counts, even and odd K) and in the batched form."""
+    """The fused per-particle step, the pair-kernel + gather path and the auto mode (which
+    switches between them) give the same states, histories, contacts and pair outputs bit for
+    bit, under both rebuild policies, in graphs (even and odd group counts, even and odd K) and
+    in the batched form."""
     if mode == "batched":
         ws = [dense_workload(dem, n=n, seed=60 + i, box=(bx, bx, 60.0))
               for i, (n, bx) in enumerate([(1500, 12.0), (2600, 16.0)])]
@@ -247,7 +248,7 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
     if mode == "trigger":
         w.pdis = w.pdis * 100.0   # fast enough for displacement-triggered rebuilds
     res = []
-    for fused in (1, 0):
+    for fused in (1, 0, 2):   # fused, split, auto (timed probes switch paths mid-run)
         c = dem.Context(w.instances)
         c.set_option(dem.OPT_FUSED_STEP, fused)
         c.load(w)
@@ -273,12 +274,13 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
         ids, hist, out = c.pairs()
         res.append((st, dh, fh, f, t, cnt, ids, hist, out, c.stats().rebuilds))
         c.close()
-    a, b = res
-    for k in ("pos", "pdis", "angv", "qi"):
-        assert np.array_equal(a[0][k], b[0][k]), k
-    for i in range(1, 9):
-        assert np.array_equal(a[i], b[i]), i
-    assert a[9] == b[9] and a[9] >= 2
+    a = res[0]
+    for b in res[1:]:
+        for k in ("pos", "pdis", "angv", "qi"):
+            assert np.array_equal(a[0][k], b[0][k]), k
+        for i in range(1, 9):
+            assert np.array_equal(a[i], b[i]), i
+        assert a[9] == b[9] and a[9] >= 2
     assert int(a[5].sum()) > 0
 
 
