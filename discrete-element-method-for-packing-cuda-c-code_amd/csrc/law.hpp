@@ -11,7 +11,9 @@
 //
 // Every expression is written in the same association order as the CPU oracle and the
 // library is compiled with -ffp-contract=off, so the two agree to the last bit wherever
-// sqrt and division are correctly rounded on both sides.  The law is exactly antisymmetric:
+// sqrt and division are correctly rounded on both sides.  Divisions shared by the three
+// components of a vector are taken once (1/dist, cn/dt, ct/dt, fct/|ht|): FP64 division is
+// a multi-instruction sequence on the GPU.  The law is exactly antisymmetric:
 // swapping the two bodies negates n, vrel, wrel and the stored history bit-for-bit, so a
 // pair's result does not depend on which member is "first".
 #pragma once
@@ -58,8 +60,8 @@ DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
     double sdn = sqrt(dn);
     double fcn = P.estar * sqrt(rs) * dn * sdn;
     double cn = P.gn * sqrt(6.0 * ms * P.estar * sqrt(rs * dn));
-    d3 fn = d3{-fcn * n.x + cn * vn.x / dt, -fcn * n.y + cn * vn.y / dt,
-               -fcn * n.z + cn * vn.z / dt};
+    double cnd = cn / dt;   // one division for the three components
+    d3 fn = d3{-fcn * n.x + cnd * vn.x, -fcn * n.y + cnd * vn.y, -fcn * n.z + cnd * vn.z};
 
     // vprojec(hin, n, &ht, 1)
     d3 q = cross3(n, hin);
@@ -86,10 +88,9 @@ DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
         double om = 1.0 - xi;
         double fct = P.mu * fcn * (1.0 - om * sqrt(om));
         double ct = P.gt * sqrt(6.0 * P.mu * ms * fcn * sqrt(om) / dtmax);
-        double sx = htm > 0.0 ? fct * ht.x / htm : 0.0;
-        double sy = htm > 0.0 ? fct * ht.y / htm : 0.0;
-        double sz = htm > 0.0 ? fct * ht.z / htm : 0.0;
-        ft = d3{sx + ct * vt.x / dt, sy + ct * vt.y / dt, sz + ct * vt.z / dt};
+        double fh = htm > 0.0 ? fct / htm : 0.0;   // spring force per unit history
+        double ctd = ct / dt;
+        ft = d3{fh * ht.x + ctd * vt.x, fh * ht.y + ctd * vt.y, fh * ht.z + ctd * vt.z};
         double ftm = sqrt(dot3(ft, ft));
         double flim = P.mu * fcn;
         if (ftm > flim) {
@@ -128,7 +129,8 @@ DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
         fout = 0.0;
         return false;
     }
-    d3 n = d3{dx.x / dist, dx.y / dist, dx.z / dist};
+    double idist = 1.0 / dist;
+    d3 n = d3{dx.x * idist, dx.y * idist, dx.z * idist};
     double rs = ra * rb / (ra + rb);
     double ms = ma * mb / (ma + mb);
     d3 s = d3{ra * wa.x + rb * wb.x, ra * wa.y + rb * wb.y, ra * wa.z + rb * wb.z};

@@ -89,7 +89,8 @@ DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel
         sA = sqrt(A);
         ct = P.gt * sA;
         if (htm2 > 0.0) u = scale3(ht2, 1.0 / htm2);
-        ft0 = add3(scale3(u, fct), scale3(vt, ct / dt));
+        double fh = htm2 > 0.0 ? fct / htm2 : 0.0;   // the forward's expression
+        ft0 = add3(scale3(ht2, fh), scale3(vt, ct / dt));
         ftm = sqrt(dot3(ft0, ft0));
         double flim = P.mu * fcn;
         if (ftm > flim) {
@@ -218,7 +219,8 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
     double dist = sqrt(dot3(dx, dx));
     double dn = (ra + rb) - dist;
     if (!(dn > 0.0)) return;
-    d3 n = d3{dx.x / dist, dx.y / dist, dx.z / dist};
+    double idist = 1.0 / dist;
+    d3 n = d3{dx.x * idist, dx.y * idist, dx.z * idist};
     double rs = ra * rb / (ra + rb);
     double ms = ma * mb / (ma + mb);
     d3 s = d3{ra * wa.x + rb * wb.x, ra * wa.y + rb * wb.y, ra * wa.z + rb * wb.z};

@@ -345,8 +345,9 @@ static inline void orc_law_core(const double *n, double dn, double rs, double ms
     double sdn = sqrt(dn);
     double fcn = estar * sqrt(rs) * dn * sdn;                 /* Hertz |F_cn| */
     double cn = gn * sqrt(6.0 * ms * estar * sqrt(rs * dn)); /* normal dashpot */
+    double cnd = cn / dt;
     double fn[3];
-    for (int k = 0; k < 3; k++) fn[k] = -fcn * n[k] + cn * vn[k] / dt;
+    for (int k = 0; k < 3; k++) fn[k] = -fcn * n[k] + cnd * vn[k];
 
     double ht[3];
     orc_vprojec(hin, n, ht, 1);
@@ -359,10 +360,9 @@ static inline void orc_law_core(const double *n, double dn, double rs, double ms
         double om = 1.0 - xi;
         double fct = mu * fcn * (1.0 - om * sqrt(om));
         double ct = gt * sqrt(6.0 * mu * ms * fcn * sqrt(om) / dtmax);
-        for (int k = 0; k < 3; k++) {
-            double sp = htm > 0.0 ? fct * ht[k] / htm : 0.0;
-            ft[k] = sp + ct * vt[k] / dt;
-        }
+        double fh = htm > 0.0 ? fct / htm : 0.0;
+        double ctd = ct / dt;
+        for (int k = 0; k < 3; k++) ft[k] = fh * ht[k] + ctd * vt[k];
         double ftm = orc_len(ft);
         double flim = mu * fcn;
         if (ftm > flim) orc_vscale(ft, flim, ftm);
@@ -398,7 +398,8 @@ int orc_law_pair(const double *xi, const double *xj, const double *vi, const dou
         *fout = 0.0;
         return 0;
     }
-    double n[3] = {dx[0] / dist, dx[1] / dist, dx[2] / dist};
+    double idist = 1.0 / dist;
+    double n[3] = {dx[0] * idist, dx[1] * idist, dx[2] * idist};
     double rs = ri * rj / (ri + rj);
     double ms = mi * mj / (mi + mj);
     double s[3], c[3], vrel[3], wrel[3];
