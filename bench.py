@@ -157,13 +157,28 @@ def bench_batched(dem, steps, interval, tune_steps=48):
             "autotune": tune}
 
 
-def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False):
-    """Dense window: the same 500K workload poured into a bed on the GPU (graph steps), then
-    `steps` plain steps from that bed with the displacement trigger."""
+def dense_bed(dem, w, settle_steps, load=None, save=None):
+    """The 500K workload poured into a bed on the GPU (graph steps), or loaded from a file
+    a previous run saved (so a profiler can run the dense window alone)."""
+    if load:
+        d = np.load(load)
+        ws = dem.Workload(w.name + "-settled", w.instances, d["pos"], d["pdis"], d["angv"], w.rad,
+                          w.rmass, w.inert, meta=dict(w.meta, bed_top=float(d["pos"][:, 2].max()),
+                                                      settle_steps=int(d["settle_steps"])))
+        log(f"dense window: bed loaded from {load}")
+        return ws
     t0 = time.perf_counter()
     ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0)
-    log(f"dense window: bed poured in {time.perf_counter() - t0:.1f} s "
-        f"(top {ws.meta['bed_top']:.1f})")
+    log(f"dense window: bed poured in {time.perf_counter() - t0:.1f} s (top {ws.meta['bed_top']:.1f})")
+    if save:
+        np.savez(save, pos=ws.pos, pdis=ws.pdis, angv=ws.angv, settle_steps=settle_steps)
+        log(f"dense window: bed saved to {save}")
+    return ws
+
+
+def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False, load=None, save=None):
+    """Dense window: `steps` plain steps, displacement trigger, from the poured bed."""
+    ws = dense_bed(dem, w, settle_steps, load, save)
     ctx = dem.Context(ws.instances)
     if split is not None:
         ctx.set_option(dem.OPT_FUSED_STEP, 0 if split else 1)
@@ -182,7 +197,7 @@ def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False):
            "step_path": "fused" if ctx.stats().step_path else "split",
            "pairs_per_particle": round(P / N, 3), "rebuilds": int(s1.rebuilds - s0.rebuilds),
            "step_kernel_us": kt["particle_ms"] * 1e3, "walls_us": kt["pair_ms"] * 1e3,
-           "rebuild_ms": kt["rebuild_ms"], "settle_steps": settle_steps}
+           "rebuild_ms": kt["rebuild_ms"], "settle_steps": ws.meta["settle_steps"]}
     log(f"dense window: {out['value']:.4e} particle-steps/s, P/N {out['pairs_per_particle']}, "
         f"{out['rebuilds']} rebuilds in {steps} steps, step kernel {out['step_kernel_us']:.1f} us")
     ctx.close()
@@ -323,6 +338,11 @@ def main():
     ap.add_argument("--batched-interval", type=int, default=20)
     ap.add_argument("--no-dense", action="store_true")
     ap.add_argument("--dense-steps", type=int, default=1000)
+    ap.add_argument("--dense-settle", type=int, default=200_000)
+    ap.add_argument("--dense-save", default=None, help="save the poured bed (.npz)")
+    ap.add_argument("--dense-load", default=None, help="load the poured bed (.npz)")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="run only the dense window (no plain leg, no JSON line)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--calib-instances", type=int, default=64)
@@ -354,6 +374,12 @@ def main():
     t0 = time.perf_counter()
     w = dem.config1(n=args.n)
     log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
+    split_opt = True if args.split else (False if args.fused else None)
+    if args.dense_only:
+        d = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
+                        split=split_opt, load=args.dense_load, save=args.dense_save)
+        log(json.dumps(d))
+        return
     ctx = dem.Context(w.instances)
     if args.split or args.fused:
         ctx.set_option(dem.OPT_FUSED_STEP, 0 if args.split else 1)
@@ -392,8 +418,8 @@ def main():
     del ctx
     dense = None
     if not args.no_dense:
-        dense = bench_dense(dem, w, steps=args.dense_steps,
-                            split=True if args.split else (False if args.fused else None))
+        dense = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
+                            split=split_opt, load=args.dense_load, save=args.dense_save)
     batched = None
     if not args.no_batched:
         batched = bench_batched(dem, args.batched_steps, args.batched_interval)
