@@ -46,6 +46,8 @@ OPT_PAIR_GRID = 6
 OPT_PARTICLE_BLOCK = 7
 OPT_ADJ_ROLL_REG = 8
 OPT_FUSED_STEP = 9
+OPT_TIME_STEPS = 10
+OPT_PIPELINE = 11
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0
@@ -162,7 +164,7 @@ EXPORTS = [
     "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_law_pair_vjp",
     "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
     "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
-    "dem_adj_release",
+    "dem_adj_release", "dem_get_step_timing",
 ]
 
 
@@ -196,6 +198,7 @@ def lib() -> ctypes.CDLL:
         "dem_sync": ([P], c_int),
         "dem_last_ms": ([P], c_double),
         "dem_time_kernels": ([P, c_int, P, P], c_int),
+        "dem_get_step_timing": ([P, P], c_int),
         "dem_get_hash": ([P, P], c_int),
         "dem_get_sorted": ([P, P], c_int),
         "dem_get_cell_start": ([P, P, c_uint], c_int),
@@ -628,6 +631,15 @@ class Context:
                "dem_time_kernels")
         return {"pair_ms": out[0], "particle_ms": out[1], "rebuild_ms": out[2],
                 "total_ms": out[3], "rebuilds": nr.value}
+
+    def step_timing(self) -> dict:
+        """Kernel times dem_step accumulated under OPT_TIME_STEPS (ms), then reset."""
+        o = np.zeros(10)
+        _check(self._L.dem_get_step_timing(self._h, _ptr(o)), "dem_get_step_timing")
+        return {"split": {"pair_walls_ms": o[0], "particle_ms": o[1], "steps": int(o[2]),
+                          "pairs_sum": o[3]},
+                "fused": {"walls_ms": o[4], "step_ms": o[5], "steps": int(o[6]), "pairs_sum": o[7]},
+                "rebuild_ms": o[8], "rebuilds": int(o[9])}
 
     def sync(self):
         _check(self._L.dem_sync(self._h), "dem_sync")

@@ -64,7 +64,8 @@ struct DevCtl {
     unsigned npairs;         // written by the pair scan
     unsigned overflow;       // pair capacity exceeded at the last rebuild
     unsigned nwall;          // wall-candidate count of the last rebuild
-    unsigned pad[3];
+    unsigned skip;           // copy of need_rebuild made by k_publish: the gate of queued steps
+    unsigned pad[2];
 };
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
@@ -74,6 +75,10 @@ __device__ __forceinline__ void st2(double *p, double a, double b)
 }
 
 __device__ __forceinline__ bool is_active(const unsigned *active) { return !active || *active != 0u; }
+
+// Gate of a queued step: a step launched before the host saw the previous step's rebuild
+// trigger does nothing (every kernel of it returns at once) when that trigger was raised.
+__device__ __forceinline__ bool gated_off(const unsigned *gate) { return gate && *gate != 0u; }
 
 // XCD-aware block order.  Workgroups are dispatched to the 8 XCDs round-robin (block b to
 // XCD b mod 8), each XCD with its own L2.  Renumbering so that XCD x runs the contiguous
@@ -104,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
     if (s == 0 && ctl) {
         ctl->need_rebuild = 0u;
+        ctl->skip = 0u;
         ctl->nwall = 0u;
     }
     if (s >= n) return;
@@ -403,8 +409,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                                                        const PRec *rec, const Hist *hist,
                                                        Hist *hist_out, POut *out,
                                                        const InstDev *inst, LawParams pp,
-                                                       double dt, Hist *tape_hist)
+                                                       double dt, Hist *tape_hist,
+                                                       const unsigned *gate)
 {
+    if (gated_off(gate)) return;
     const unsigned np = *npairs;
     // each block takes a contiguous range of pairs, XCD-aware (pairs are ordered by their
     // first member's slot, so a range touches nearby records)
@@ -461,8 +469,10 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
                                                   const unsigned *wlist, const DevCtl *ctl,
                                                   const InstDev *inst, InstDev one,
-                                                  double *wforce, Hist *tape_whist)
+                                                  double *wforce, Hist *tape_whist,
+                                                  const unsigned *gate)
 {
+    if (gated_off(gate)) return;
     const unsigned nw = ctl->nwall;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
         unsigned a = wlist[i];
@@ -578,8 +588,10 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
     const unsigned *toff, const unsigned *jt, const POut *out, unsigned n, const InstDev *inst,
-    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec)
+    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec,
+    const unsigned *gate)
 {
+    if (gated_off(gate)) return;
     const unsigned np = ctl->npairs;
     unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool live = a < n;
@@ -654,8 +666,9 @@ __global__ __launch_bounds__(kBlock) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const Hist *hin, Hist *hout, const unsigned *wflag, const double *wforce,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, Hist *tape_hist)
+    PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
 {
+    if (gated_off(gate)) return;
     const unsigned np = ctl->npairs;
     unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool live = a < n;
@@ -712,11 +725,17 @@ __global__ __launch_bounds__(kBlock) void k_step(
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 
-// Publish the control block to the host-mapped mirror (one lane; after k_particle).
-__global__ void k_publish(DevCtl *ctl, unsigned char *mirror)
+// Publish the control block to the host-mapped mirror (one lane; after the step kernels).
+// With a queue slot, also reports the step's outcome there (0 ran, 1 ran and raised the
+// trigger); a gated-off step leaves the slot as the host set it.
+__global__ void k_publish(DevCtl *ctl, unsigned char *mirror, const unsigned *gate,
+                          unsigned *slot)
 {
+    if (gated_off(gate)) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->steps = ctl->steps + 1u;
+        ctl->skip = ctl->need_rebuild;
+        if (slot) *reinterpret_cast<volatile unsigned *>(slot) = ctl->need_rebuild;
         volatile unsigned *src = reinterpret_cast<volatile unsigned *>(ctl);
         unsigned *dst = reinterpret_cast<unsigned *>(mirror);
         for (unsigned k = 0; k < sizeof(DevCtl) / 4; k++) dst[k] = src[k];

@@ -205,6 +205,25 @@ struct dem_ctx {
     bool probed = false;
     float t_probe[2] = {0.f, 0.f};
     hipEvent_t evp0 = nullptr, evp1 = nullptr;
+    // kernel timing of dem_step (DEM_OPT_TIME_STEPS): a ring of event sets, read back lazily
+    struct TimedStep {
+        hipEvent_t e[3] = {nullptr, nullptr, nullptr};   // before, mid, after the step kernels
+        hipEvent_t r[2] = {nullptr, nullptr};            // around a rebuild
+        bool used = false, reb = false, fused = false;
+        unsigned np = 0;
+    };
+    static constexpr int kRing = 64;
+    TimedStep ring[kRing];
+    int ring_pos = 0;
+    bool timing = false;
+    // queued steps under the displacement trigger: up to `pipeline` steps in flight, each
+    // reporting its outcome in a host-mapped slot (2 = not run)
+    static constexpr int kQueue = 8;
+    int pipeline = 3;
+    hipEvent_t evq[kQueue] = {};
+    unsigned *qflag_h = nullptr, *qflag_d = nullptr;
+    double t_acc[2][2] = {{0, 0}, {0, 0}};   // [path][first kernel group, per-particle kernel]
+    double np_acc[2] = {0, 0}, n_acc[2] = {0, 0}, t_reb = 0, n_reb = 0;
     int last_path = 0;        // path of the last step: 0 none, 1 split, 2 fused
     bool out_valid = false;   // fused: rec[1-rc] / hist[1-hc] hold the last step's inputs
     POut *out = nullptr;
@@ -427,7 +446,8 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
 // from the current buffers into the other ones; the buffer indices flip).  `mid` is
 // recorded before the per-particle kernel, `end` after it.
 void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
-                       hipEvent_t end = nullptr)
+                       hipEvent_t end = nullptr, const unsigned *gate = nullptr,
+                       unsigned *slot = nullptr)
 {
     hipStream_t st = c->stream;
     unsigned pb = c->particle_block;
@@ -442,20 +462,20 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         if (c->batched)
             hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist);
+                               dinst, one.pp, one.dt, c->tape_hist, gate);
         else
             hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist);
+                               dinst, one.pp, one.dt, c->tape_hist, gate);
     }
     if (c->batched)
         hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
                            c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
-                           dinst, one, c->wforce, c->tape_whist);
+                           dinst, one, c->wforce, c->tape_whist, gate);
     else
         hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
                            c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
-                           dinst, one, c->wforce, c->tape_whist);
+                           dinst, one, c->wforce, c->tape_whist, gate);
     if (mid) (void)hipEventRecord(mid, st);
     if (c->fused) {
         int nr = 1 - c->rc, nh = 1 - c->hc;
@@ -467,14 +487,14 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const unsigned *)c->toff, (const unsigned *)c->jt,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
-                               c->ctl, c->tape_rec, c->tape_hist);
+                               c->ctl, c->tape_rec, c->tape_hist, gate);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
-                               c->ctl, c->tape_rec, c->tape_hist);
+                               c->ctl, c->tape_rec, c->tape_hist, gate);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -484,17 +504,17 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
-                           c->contacti, fdbg, c->ctl, c->tape_rec);
+                           c->contacti, fdbg, c->ctl, c->tape_rec, gate);
     } else {
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
-                           c->contacti, fdbg, c->ctl, c->tape_rec);
+                           c->contacti, fdbg, c->ctl, c->tape_rec, gate);
     }
     if (!c->fused) c->last_path = 1;
     if (end) (void)hipEventRecord(end, st);
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d, gate, slot);
 }
 
 InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
@@ -584,6 +604,12 @@ void free_ctx(dem_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evs) (void)hipEventDestroy(c->evs);
+    for (auto &T : c->ring) {
+        for (auto &e : T.e) if (e) (void)hipEventDestroy(e);
+        for (auto &e : T.r) if (e) (void)hipEventDestroy(e);
+    }
+    for (auto &e : c->evq) if (e) (void)hipEventDestroy(e);
+    if (c->qflag_h) (void)hipHostFree(c->qflag_h);
     if (c->evp0) (void)hipEventDestroy(c->evp0);
     if (c->evp1) (void)hipEventDestroy(c->evp1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -596,6 +622,38 @@ int sync_mirror(dem_ctx *c)
     DEM_HIP(hipEventSynchronize(c->evs));
     const volatile DevCtl *m = reinterpret_cast<const volatile DevCtl *>(c->mirror_h);
     c->last_flag = m->need_rebuild;
+    return DEM_OK;
+}
+
+// Read back one timed step of the ring (waits for its last event).
+int ring_flush_slot(dem_ctx *c, dem_ctx::TimedStep &T)
+{
+    if (!T.used) return DEM_OK;
+    DEM_HIP(hipEventSynchronize(T.e[2]));
+    float a = 0.f, b = 0.f;
+    DEM_HIP(hipEventElapsedTime(&a, T.e[0], T.e[1]));
+    DEM_HIP(hipEventElapsedTime(&b, T.e[1], T.e[2]));
+    int p = T.fused ? 1 : 0;
+    c->t_acc[p][0] += a;
+    c->t_acc[p][1] += b;
+    c->np_acc[p] += T.np;
+    c->n_acc[p] += 1;
+    if (T.reb) {
+        float r = 0.f;
+        DEM_HIP(hipEventElapsedTime(&r, T.r[0], T.r[1]));
+        c->t_reb += r;
+        c->n_reb += 1;
+    }
+    T.used = false;
+    return DEM_OK;
+}
+
+int ring_flush_all(dem_ctx *c)
+{
+    for (int k = 0; k < dem_ctx::kRing; k++) {
+        int rc = ring_flush_slot(c, c->ring[(c->ring_pos + k) % dem_ctx::kRing]);
+        if (rc) return rc;
+    }
     return DEM_OK;
 }
 
@@ -752,6 +810,10 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipEventCreateWithFlags(&c->evs, hipEventDisableTiming));
     DEM_HIP(hipEventCreate(&c->evp0));
     DEM_HIP(hipEventCreate(&c->evp1));
+    for (auto &e : c->evq) DEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    DEM_HIP(hipHostMalloc((void **)&c->qflag_h, sizeof(unsigned) * dem_ctx::kQueue,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    DEM_HIP(hipHostGetDevicePointer((void **)&c->qflag_d, c->qflag_h, 0));
     DEM_HIP(hipMalloc(&c->dinst, sizeof(InstDev) * ninst));
     DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * ninst, hipMemcpyHostToDevice));
     for (int k = 0; k < 2; k++) {
@@ -832,6 +894,22 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     }
+    case DEM_OPT_PIPELINE: {
+        int d = (int)value;
+        if (d < 1 || d > dem_ctx::kQueue || (double)d != value) return DEM_ERR_INVALID;
+        c->pipeline = d;
+        return DEM_OK;
+    }
+    case DEM_OPT_TIME_STEPS:
+        if (value != 0.0 && !c->ring[0].e[0]) {
+            for (auto &T : c->ring) {
+                for (auto &e : T.e) DEM_HIP(hipEventCreate(&e));
+                for (auto &e : T.r) DEM_HIP(hipEventCreate(&e));
+            }
+        }
+        if (c->timing && value == 0.0 && ring_flush_all(c)) return DEM_ERR_HIP;
+        c->timing = value != 0.0;
+        return DEM_OK;
     case DEM_OPT_ADJ_ROLL_REG:
         if (!(value >= 0.0)) return DEM_ERR_INVALID;
         c->adj_roll_k = value;
@@ -977,14 +1055,124 @@ int dem_rebuild(dem_ctx *c)
     return DEM_OK;
 }
 
+namespace {
+
+// dem_step under the displacement trigger with steps queued ahead of the host.  A queued
+// step that starts after its predecessor raised the trigger does nothing (its kernels read
+// the gate word k_publish sets); when the host sees the trigger in the predecessor's slot it
+// drains the queue, undoes the host-side buffer flips of the steps that did not run, rebuilds
+// and issues them again.  The steps that run, and their results, are the synchronous loop's.
+int step_queued(dem_ctx *c, int nsteps)
+{
+    struct Issued {
+        int rc, hc, last_path;
+        bool out_valid;
+    };
+    Issued q[dem_ctx::kQueue];
+    const int depth = c->pipeline;
+    const unsigned *gate = &c->ctl->skip;
+    volatile unsigned *flag = c->qflag_h;
+    int done = 0, head = 0, inflight = 0;
+    while (done < nsteps) {
+        if (inflight == 0) {
+            if (c->need_rebuild) {
+                int rc = rebuild_host(c);
+                if (rc) return rc;
+                if (c->fused_mode == 2 && (!c->probed || ++c->reb_since_probe >= c->probe_every)) {
+                    c->probe_left = 2;
+                    c->reb_since_probe = 0;
+                }
+            }
+            if (c->probe_left > 0) {   // a timed probe step runs on its own
+                c->fused = c->probe_left == 2;
+                DEM_HIP(hipEventRecord(c->evp0, c->stream));
+                launch_force_step(c, pair_grid(c, c->npairs));
+                DEM_HIP(hipEventRecord(c->evp1, c->stream));
+                DEM_HIP(hipGetLastError());
+                DEM_HIP(hipEventSynchronize(c->evp1));
+                DEM_HIP(hipEventElapsedTime(&c->t_probe[c->fused ? 1 : 0], c->evp0, c->evp1));
+                if (--c->probe_left == 0) {
+                    c->fused = c->t_probe[1] <= c->t_probe[0];
+                    c->probed = true;
+                }
+                c->steps++;
+                c->since++;
+                int rc = sync_mirror(c);
+                if (rc) return rc;
+                c->need_rebuild = c->last_flag != 0u;
+                done++;
+                continue;
+            }
+        }
+        while (inflight < depth && done + inflight < nsteps) {
+            int k = (head + inflight) % dem_ctx::kQueue;
+            q[k] = Issued{c->rc, c->hc, c->last_path, c->out_valid};
+            flag[k] = 2u;
+            launch_force_step(c, pair_grid(c, c->npairs), nullptr, nullptr, gate, c->qflag_d + k);
+            DEM_HIP(hipEventRecord(c->evq[k], c->stream));
+            inflight++;
+        }
+        DEM_HIP(hipGetLastError());
+        DEM_HIP(hipEventSynchronize(c->evq[head]));
+        unsigned f = flag[head];
+        if (f == 2u) return DEM_ERR_HIP;   // the oldest queued step always runs
+        head = (head + 1) % dem_ctx::kQueue;
+        inflight--;
+        done++;
+        c->steps++;
+        c->since++;
+        if (f == 1u) {
+            // the trigger: the queued steps after this one did nothing -- drain them, undo
+            // their host-side buffer flips, and rebuild before issuing them again
+            if (inflight > 0) {
+                int last = (head + inflight - 1) % dem_ctx::kQueue;
+                DEM_HIP(hipEventSynchronize(c->evq[last]));
+                const Issued &u = q[head];
+                c->rc = u.rc;
+                c->hc = u.hc;
+                c->last_path = u.last_path;
+                c->out_valid = u.out_valid;
+                inflight = 0;
+            }
+            c->need_rebuild = true;
+        }
+    }
+    c->last_flag = c->need_rebuild ? 1u : 0u;
+    return DEM_OK;
+}
+
+}  // namespace
+
 int dem_step(dem_ctx *c, int nsteps)
 {
     if (!c || nsteps < 0) return DEM_ERR_INVALID;
+    if (c->interval == 0 && !c->timing && c->pipeline > 1) {
+        DEM_HIP(hipEventRecord(c->ev0, c->stream));
+        int rc = step_queued(c, nsteps);
+        if (rc) return rc;
+        DEM_HIP(hipEventRecord(c->ev1, c->stream));
+        DEM_HIP(hipEventSynchronize(c->ev1));
+        DEM_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+        return DEM_OK;
+    }
     DEM_HIP(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < nsteps; s++) {
+        dem_ctx::TimedStep *T = nullptr;
+        if (c->timing) {
+            T = &c->ring[c->ring_pos];
+            c->ring_pos = (c->ring_pos + 1) % dem_ctx::kRing;
+            int rc = ring_flush_slot(c, *T);   // the slot's previous step, long finished
+            if (rc) return rc;
+            T->reb = false;
+        }
         if (c->need_rebuild || (c->interval > 0 && c->since >= (unsigned)c->interval)) {
+            if (T) {
+                T->reb = true;
+                DEM_HIP(hipEventRecord(T->r[0], c->stream));
+            }
             int rc = rebuild_host(c);
             if (rc) return rc;
+            if (T) DEM_HIP(hipEventRecord(T->r[1], c->stream));
             // auto path: after the first rebuild and every probe_every-th one, time one step
             // of each path (same results bit for bit) and keep the faster
             if (c->fused_mode == 2 && (!c->probed || ++c->reb_since_probe >= c->probe_every)) {
@@ -997,7 +1185,15 @@ int dem_step(dem_ctx *c, int nsteps)
             c->fused = c->probe_left == 2;
             DEM_HIP(hipEventRecord(c->evp0, c->stream));
         }
-        launch_force_step(c, pair_grid(c, c->npairs));
+        if (T) {
+            T->fused = c->fused;
+            T->np = c->npairs;
+            T->used = true;
+            DEM_HIP(hipEventRecord(T->e[0], c->stream));
+            launch_force_step(c, pair_grid(c, c->npairs), T->e[1], T->e[2]);
+        } else {
+            launch_force_step(c, pair_grid(c, c->npairs));
+        }
         DEM_HIP(hipGetLastError());
         if (probe) {
             DEM_HIP(hipEventRecord(c->evp1, c->stream));
@@ -1176,6 +1372,28 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
     out_ms[2] = nreb ? t_reb / nreb : 0.0;
     out_ms[3] = t_all + t_reb;
     if (nrebuild) *nrebuild = nreb;
+    return DEM_OK;
+}
+
+// Accumulated kernel times of dem_step under DEM_OPT_TIME_STEPS (ms), then reset:
+// out[0..3] = split path: first group (pair kernel + walls) total, per-particle kernel total,
+// steps, sum of pair counts; out[4..7] = fused path: walls total, step kernel total, steps,
+// sum of pair counts; out[8] = rebuild total, out[9] = rebuilds.
+int dem_get_step_timing(dem_ctx *c, double *out)
+{
+    if (!c || !out) return DEM_ERR_INVALID;
+    int rc = ring_flush_all(c);
+    if (rc) return rc;
+    for (int p = 0; p < 2; p++) {
+        out[4 * p] = c->t_acc[p][0];
+        out[4 * p + 1] = c->t_acc[p][1];
+        out[4 * p + 2] = c->n_acc[p];
+        out[4 * p + 3] = c->np_acc[p];
+        c->t_acc[p][0] = c->t_acc[p][1] = c->n_acc[p] = c->np_acc[p] = 0.0;
+    }
+    out[8] = c->t_reb;
+    out[9] = c->n_reb;
+    c->t_reb = c->n_reb = 0.0;
     return DEM_OK;
 }
 
@@ -1444,13 +1662,13 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr);
+                                   (Hist *)nullptr, (const unsigned *)nullptr);
             else
                 hipLaunchKernelGGL(k_pair_force<false>, dim3(g), dim3(c->pair_block), 0, c->stream,
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr);
+                                   (Hist *)nullptr, (const unsigned *)nullptr);
             DEM_HIP(hipGetLastError());
             DEM_HIP(hipStreamSynchronize(c->stream));
         }

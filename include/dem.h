@@ -197,11 +197,15 @@ enum {
     DEM_OPT_ADJ_ROLL_REG = 8,      /* adjoint: floor of |w_rel| in the rolling moment's
                                       linearisation, in units of the one-step chatter scale
                                       mu_r fcn dt^2 (1/I_a + 1/I_b); 0 = exact, default 1 */
-    DEM_OPT_FUSED_STEP = 9         /* 1: fused per-particle step, each pair law evaluated
+    DEM_OPT_FUSED_STEP = 9,        /* 1: fused per-particle step, each pair law evaluated
                                       from both sides, no per-pair output buffer; 0: pair
                                       kernel + gather; 2 (default): auto -- dem_step times one
                                       step of each after the first and every 8th rebuild and
                                       keeps the faster.  Same results bit for bit */
+    DEM_OPT_TIME_STEPS = 10,       /* 1: dem_step records HIP events around its kernels
+                                      (read with dem_get_step_timing) */
+    DEM_OPT_PIPELINE = 11          /* steps dem_step keeps in flight under the displacement
+                                      trigger (1..8, default 3; 1 = wait for each step) */
 };
 
 /* Result of dem_autotune. */
@@ -266,6 +270,11 @@ double dem_last_ms(dem_ctx *ctx);
  * pair-force kernel time, out_ms[1] mean per-particle kernel time, out_ms[2] mean rebuild
  * time, out_ms[3] total; *nrebuild = rebuilds run. */
 int dem_time_kernels(dem_ctx *ctx, int nsteps, double *out_ms, int *nrebuild);
+/* With DEM_OPT_TIME_STEPS: kernel times accumulated by dem_step since the last call (ms),
+ * then reset.  out[10] = split path: pair kernel + walls total, per-particle kernel total,
+ * steps, sum of pair counts; fused path: walls total, step kernel total, steps, sum of pair
+ * counts; rebuild total, rebuilds. */
+int dem_get_step_timing(dem_ctx *ctx, double *out);
 
 /* Introspection (host buffers), for parity checks against the reference layout. */
 /* calcHash output of the last rebuild: 2n uint (cell, index), index order. */

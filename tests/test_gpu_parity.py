@@ -284,6 +284,32 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
     assert int(a[5].sum()) > 0
 
 
+def test_queued_steps_equal_synchronous(dem, gpu_ok):
+    """Under the displacement trigger dem_step queues steps ahead of the host (a queued step
+    after a trigger does nothing and is issued again after the rebuild); the steps run, the
+    rebuilds and the results equal the synchronous loop's bit for bit, whatever the depth."""
+    w = dense_workload(dem, n=4000, seed=52)
+    w.pdis = w.pdis * 100.0   # frequent triggers
+    res = []
+    for depth in (1, 2, 3, 8):
+        c = dem.Context(w.instances)
+        c.set_option(dem.OPT_PIPELINE, depth)
+        c.load(w)
+        for k in (1, 7, 50, 142):   # calls of several lengths
+            c.step(k)
+        st = c.state()
+        s = c.stats()
+        res.append((st, s.steps, s.rebuilds, c.contacts(), c.wall_history()[0]))
+        c.close()
+    a = res[0]
+    assert a[2] >= 4, a[2]
+    for b in res[1:]:
+        for k in ("pos", "pdis", "angv", "qi"):
+            assert np.array_equal(a[0][k], b[0][k]), k
+        assert a[1] == b[1] == 200 and a[2] == b[2]
+        assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+
+
 def test_batched_equals_plain(dem, gpu_ok):
     ws = [dense_workload(dem, n=n, seed=40 + i, mu=mu, e=e, box=(bx, bx, 60.0))
           for i, (n, mu, e, bx) in enumerate([(1500, 0.2, 0.4, 12.0), (3100, 0.5, 0.8, 16.0),
