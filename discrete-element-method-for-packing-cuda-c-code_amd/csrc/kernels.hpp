@@ -359,10 +359,11 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
 }
 
 // Replace each transposed entry (partner slot b, smaller particle index) by the index of
-// pair (b, a) in b's segment.
+// pair (b, a) in b's segment, and keep b in jf[k] (the fused step reads the first member's
+// slot from there instead of from the pair list at a scattered index).
 __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const unsigned *off,
                                                       const unsigned *toff, unsigned n,
-                                                      unsigned cap, unsigned *jt,
+                                                      unsigned cap, unsigned *jt, unsigned *jf,
                                                       const unsigned *active)
 {
     if (!is_active(active)) return;
@@ -375,7 +376,18 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const uns
         unsigned lo = off[b], hi = off[b + 1];
         if (hi > cap) hi = cap;
         jt[k] = lo < hi ? find_second(ab, lo, hi, a) : kEmpty;
+        jf[k] = b;
     }
+}
+
+// jf from a restored list: jf[k] = first member of pair jt[k].
+__global__ __launch_bounds__(kBlock) void k_owner(const uint2 *ab, const unsigned *jt,
+                                                  unsigned np, unsigned *jf)
+{
+    unsigned k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= np) return;
+    unsigned q = jt[k];
+    jf[k] = q < np ? ab[q].x : kEmpty;
 }
 
 // Pair-capacity check after the scan: npairs = min(total, cap); overflow when total > cap.
@@ -661,12 +673,15 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // finish_particle.  No per-pair output buffer: each law is evaluated twice instead of its
 // 80-byte result being written once and gathered twice.  Records and histories are read
 // from rin / hin and written to rout / hout, so no lane sees a neighbour's updated state.
+// A candidate is first tested for contact from the partner's position and radius alone
+// (law_touch, the law's own test); the rest of the partner's record and the pair's history
+// are read only for pairs in contact (a pair out of contact has an all-zero law result).
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
-    const unsigned *jt, const Hist *hin, Hist *hout, const unsigned *wflag, const double *wforce,
-    unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
+    const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
+    const double *wforce, unsigned n, const InstDev *inst, InstDev one, unsigned *contacti,
+    double *fdbg, DevCtl *ctl, PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
 {
     if (gated_off(gate)) return;
     const unsigned np = ctl->npairs;
@@ -675,6 +690,9 @@ __global__ __launch_bounds__(kBlock) void k_step(
     unsigned flag = 0;
     if (live) {
         const double *RA = reinterpret_cast<const double *>(rin + a);
+        double2 a0 = ld2(RA), a1 = ld2(RA + 2);
+        const d3 xa = d3{a0.x, a0.y, a1.x};
+        const double ra = a1.y;
         unsigned inst_a = BATCHED ? rin[a].inst : 0u;
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
@@ -684,18 +702,23 @@ __global__ __launch_bounds__(kBlock) void k_step(
         if (q1 > np) q1 = np;
         for (unsigned p = p0; p < p1; p++) {
             unsigned b = ab[p].y;
-            const double *H = reinterpret_cast<const double *>(hin + p);
-            double2 h0 = ld2(H), h1 = ld2(H + 2);
-            if (tape_hist) {
-                double *T = reinterpret_cast<double *>(tape_hist + p);
-                st2(T, h0.x, h0.y);
-                st2(T + 2, h1.x, h1.y);
+            const double *B = reinterpret_cast<const double *>(rin + b);
+            double2 b0 = ld2(B), b1 = ld2(B + 2);
+            bool con = law_touch(xa, d3{b0.x, b0.y, b1.x}, ra, b1.y);
+            d3 fa = d3{0.0, 0.0, 0.0}, ta = fa, tb = fa, ho = fa;
+            double fo = 0.0;
+            if (con || tape_hist) {
+                const double *H = reinterpret_cast<const double *>(hin + p);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                if (tape_hist) {
+                    double *T = reinterpret_cast<double *>(tape_hist + p);
+                    st2(T, h0.x, h0.y);
+                    st2(T + 2, h1.x, h1.y);
+                }
+                if (con)
+                    pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
+                                               inst_a, fa, ta, tb, ho, fo);
             }
-            d3 fa, ta, tb, ho;
-            double fo;
-            bool con = pair_from_records<BATCHED>(RA, reinterpret_cast<const double *>(rin + b),
-                                                  d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
-                                                  inst_a, fa, ta, tb, ho, fo);
             double *HW = reinterpret_cast<double *>(hout + p);
             st2(HW, ho.x, ho.y);
             st2(HW + 2, ho.z, fo);
@@ -706,14 +729,18 @@ __global__ __launch_bounds__(kBlock) void k_step(
         for (unsigned k = q0; k < q1; k++) {
             unsigned q = jt[k];
             if (q >= np) continue;
-            unsigned f = ab[q].x;
-            const double *H = reinterpret_cast<const double *>(hin + q);
-            double2 h0 = ld2(H), h1 = ld2(H + 2);
-            d3 fa, ta, tb, ho;
-            double fo;
-            bool con = pair_from_records<BATCHED>(reinterpret_cast<const double *>(rin + f), RA,
-                                                  d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
-                                                  inst_a, fa, ta, tb, ho, fo);
+            unsigned f = jf[k];
+            const double *F = reinterpret_cast<const double *>(rin + f);
+            double2 f0 = ld2(F), f1 = ld2(F + 2);
+            bool con = law_touch(d3{f0.x, f0.y, f1.x}, xa, f1.y, ra);
+            d3 fa = d3{0.0, 0.0, 0.0}, ta = fa, tb = fa, ho = fa;
+            double fo = 0.0;
+            if (con) {
+                const double *H = reinterpret_cast<const double *>(hin + q);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
+                                           inst_a, fa, ta, tb, ho, fo);
+            }
             fx -= fa.x; fy -= fa.y; fz -= fa.z;
             tx += tb.x; ty += tb.y; tz += tb.z;
             c += con ? 1u : 0u;

@@ -114,6 +114,16 @@ DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
     }
 }
 
+// Contact test of law_pair (the same expressions, so the same decision bit for bit): true
+// when spheres a and b overlap.  Needs only the positions and radii.
+DEM_HD bool law_touch(d3 xa, d3 xb, double ra, double rb)
+{
+    d3 dx = d3{xb.x - xa.x, xb.y - xa.y, xb.z - xa.z};
+    double dist = sqrt(dot3(dx, dx));
+    double dn = (ra + rb) - dist;
+    return dn > 0.0;
+}
+
 // Pair law between bodies a and b.  Returns true on contact; on no contact every output
 // is zero (history reset), matching the reference's no-contact branch (neiglist.cu:220-227).
 DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,

@@ -200,7 +200,8 @@ struct dem_ctx {
     int hc = 0;
     bool fused = true;        // step path in use: fused per-particle (k_step) or split
     int fused_mode = 2;       // DEM_OPT_FUSED_STEP: 0 split, 1 fused, 2 auto (timed probes)
-    int probe_left = 0;       // auto: steps left in the current probe (fused, then split)
+    int probe_left = 0;       // auto: steps left in the current probe (fused, then split;
+                              // the first probe runs the pair twice and keeps the second)
     unsigned reb_since_probe = 0, probe_every = 8;
     bool probed = false;
     float t_probe[2] = {0.f, 0.f};
@@ -228,8 +229,10 @@ struct dem_ctx {
     bool out_valid = false;   // fused: rec[1-rc] / hist[1-hc] hold the last step's inputs
     POut *out = nullptr;
     unsigned *jt = nullptr;
+    unsigned *jf = nullptr;   // jt's companion: the slot of the pair's first member
     unsigned out_cap = 0;
     unsigned npairs = 0;
+    unsigned np_peak = 0;     // largest list built by rebuild_host since the last reset
     bool have_list = false;
     unsigned n_list = 0;
 
@@ -290,10 +293,13 @@ int alloc_out(dem_ctx *c, unsigned cap)
 {
     if (c->out) DEM_HIP(hipFree(c->out));
     if (c->jt) DEM_HIP(hipFree(c->jt));
+    if (c->jf) DEM_HIP(hipFree(c->jf));
     c->out = nullptr;
     c->jt = nullptr;
+    c->jf = nullptr;
     DEM_HIP(hipMalloc(&c->out, (size_t)std::max(cap, 1u) * sizeof(POut)));
     DEM_HIP(hipMalloc(&c->jt, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
+    DEM_HIP(hipMalloc(&c->jf, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
     c->out_cap = cap;
     c->buf_gen++;
     return DEM_OK;
@@ -394,7 +400,7 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
                            c->tape_src, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
-                       active);
+                       c->jf, active);
     DEM_HIP(hipGetLastError());
     c->lc = nl;
     c->hc = nh;
@@ -434,6 +440,7 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
         if ((rc = alloc_hist(c, 1 - c->hc, grow))) return rc;
     }
     c->npairs = np;
+    c->np_peak = std::max(c->np_peak, (unsigned)np);
     c->rebuilds++;
     c->since = 0;
     c->need_rebuild = false;
@@ -484,14 +491,14 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         if (c->batched)
             hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
-                               (const unsigned *)c->toff, (const unsigned *)c->jt,
+                               (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
                                c->ctl, c->tape_rec, c->tape_hist, gate);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
-                               (const unsigned *)c->toff, (const unsigned *)c->jt,
+                               (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
                                c->ctl, c->tape_rec, c->tape_hist, gate);
@@ -592,7 +599,7 @@ void free_ctx(dem_ctx *c)
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos, c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
-                    c->hist[1], c->out, c->jt, c->ctl};
+                    c->hist[1], c->out, c->jt, c->jf, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->S.release();
@@ -1079,12 +1086,12 @@ int step_queued(dem_ctx *c, int nsteps)
                 int rc = rebuild_host(c);
                 if (rc) return rc;
                 if (c->fused_mode == 2 && (!c->probed || ++c->reb_since_probe >= c->probe_every)) {
-                    c->probe_left = 2;
+                    c->probe_left = c->probed ? 2 : 4;
                     c->reb_since_probe = 0;
                 }
             }
             if (c->probe_left > 0) {   // a timed probe step runs on its own
-                c->fused = c->probe_left == 2;
+                c->fused = c->probe_left % 2 == 0;
                 DEM_HIP(hipEventRecord(c->evp0, c->stream));
                 launch_force_step(c, pair_grid(c, c->npairs));
                 DEM_HIP(hipEventRecord(c->evp1, c->stream));
@@ -1176,13 +1183,13 @@ int dem_step(dem_ctx *c, int nsteps)
             // auto path: after the first rebuild and every probe_every-th one, time one step
             // of each path (same results bit for bit) and keep the faster
             if (c->fused_mode == 2 && (!c->probed || ++c->reb_since_probe >= c->probe_every)) {
-                c->probe_left = 2;
+                c->probe_left = c->probed ? 2 : 4;
                 c->reb_since_probe = 0;
             }
         }
         bool probe = c->probe_left > 0;
         if (probe) {
-            c->fused = c->probe_left == 2;
+            c->fused = c->probe_left % 2 == 0;
             DEM_HIP(hipEventRecord(c->evp0, c->stream));
         }
         if (T) {
@@ -1247,17 +1254,21 @@ int dem_step_graph(dem_ctx *c, int nsteps)
     if (c->hcap[1 - c->hc] < want && (rc = alloc_hist(c, 1 - c->hc, want))) return rc;
     if ((rc = grow_hist_keep(c, c->hc, want))) return rc;
     if (c->out_cap < want) {
-        // out / jt hold the live transposed index: keep jt's contents
+        // out / jt / jf hold the live transposed index: keep jt's and jf's contents
         POut *no = nullptr;
-        unsigned *nj = nullptr;
+        unsigned *nj = nullptr, *nf = nullptr;
         DEM_HIP(hipMalloc(&no, (size_t)want * sizeof(POut)));
         DEM_HIP(hipMalloc(&nj, (size_t)want * sizeof(unsigned)));
+        DEM_HIP(hipMalloc(&nf, (size_t)want * sizeof(unsigned)));
         DEM_HIP(hipMemcpyAsync(nj, c->jt, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
+        DEM_HIP(hipMemcpyAsync(nf, c->jf, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
         DEM_HIP(hipStreamSynchronize(c->stream));
         DEM_HIP(hipFree(c->out));
         DEM_HIP(hipFree(c->jt));
+        DEM_HIP(hipFree(c->jf));
         c->out = no;
         c->jt = nj;
+        c->jf = nf;
         c->out_cap = want;
         c->buf_gen++;
     }
@@ -2189,6 +2200,10 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
             DEM_HIP(hipMemcpyAsync(c->ab[c->lc], S.ab, sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
             DEM_HIP(hipMemcpyAsync(c->hist[c->hc], S.hist, sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
             DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_owner, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, st,
+                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->jt, (unsigned)np,
+                               c->jf);
+            DEM_HIP(hipGetLastError());
         }
     }
     c->npairs = S.npairs;
@@ -2469,18 +2484,23 @@ int dem_adj_forward(dem_ctx *c, int nsteps, int every, const dem_obs *obs, doubl
     A->nseg = (nsteps + every - 1) / every;
     A->n = c->n;
     A->ck.resize(A->nseg + 1);
-    unsigned max_np = c->have_list ? c->npairs : 0;
+    c->np_peak = c->have_list ? c->npairs : 0;
+    // under the displacement trigger the segments run as queued steps (dem_step's schedule
+    // and results, without a host wait per step)
+    bool queued = c->interval == 0 && !c->timing && c->pipeline > 1;
     DEM_HIP(hipEventRecord(c->ev0, c->stream));
     for (int s = 0; s < A->nseg; s++) {
         if ((rc = snap_save(c, A->ck[s]))) return rc;
         int ks = std::min(every, nsteps - s * every);
-        for (int j = 0; j < ks; j++) {
-            if ((rc = adj_step(c, nullptr, -1, nullptr, nullptr))) return rc;
-            max_np = std::max(max_np, c->npairs);
+        if (queued) {
+            if ((rc = step_queued(c, ks))) return rc;
+        } else {
+            for (int j = 0; j < ks; j++)
+                if ((rc = adj_step(c, nullptr, -1, nullptr, nullptr))) return rc;
         }
     }
     if ((rc = snap_save(c, A->ck[A->nseg]))) return rc;
-    A->max_np = max_np;
+    A->max_np = c->np_peak;
     DEM_HIP(hipEventRecord(c->ev1, c->stream));
     DEM_HIP(hipEventSynchronize(c->ev1));
     DEM_HIP(hipEventElapsedTime(&A->fwd_ms, c->ev0, c->ev1));
