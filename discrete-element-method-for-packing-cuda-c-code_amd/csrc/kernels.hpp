@@ -752,6 +752,15 @@ __global__ __launch_bounds__(kBlock) void k_step(
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 
+// A bounded busy period of `ticks` wall-clock ticks (one lane; the clock runs at a fixed
+// rate, so the loop always ends).
+__global__ void k_delay(unsigned long long ticks)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long t0 = wall_clock64();
+    while (wall_clock64() - t0 < ticks) __builtin_amdgcn_s_sleep(2);
+}
+
 // Publish the control block to the host-mapped mirror (one lane; after the step kernels).
 // With a queue slot, also reports the step's outcome there (0 ran, 1 ran and raised the
 // trigger); a gated-off step leaves the slot as the host set it.

@@ -205,6 +205,7 @@ struct dem_ctx {
     unsigned reb_since_probe = 0, probe_every = 8;
     bool probed = false;
     float t_probe[2] = {0.f, 0.f};
+    unsigned long long probe_delay = 0;   // wall-clock ticks of the busy period before a probe
     hipEvent_t evp0 = nullptr, evp1 = nullptr;
     // kernel timing of dem_step (DEM_OPT_TIME_STEPS): a ring of event sets, read back lazily
     struct TimedStep {
@@ -623,6 +624,15 @@ void free_ctx(dem_ctx *c)
     delete c;
 }
 
+// A path probe times one step between two events.  Launched first, a short busy period
+// keeps the GPU occupied while the host queues the step's kernels, so the probe measures the
+// kernels back to back (as queued steps run them), not the host's launch gaps.
+void launch_probe_delay(dem_ctx *c)
+{
+    if (c->probe_delay)
+        hipLaunchKernelGGL(k_delay, dim3(1), dim3(64), 0, c->stream, c->probe_delay);
+}
+
 int sync_mirror(dem_ctx *c)
 {
     DEM_HIP(hipEventRecord(c->evs, c->stream));
@@ -785,6 +795,12 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     dem_ctx *c = new dem_ctx();
     DEM_HIP(hipGetDevice(&c->device));
     c->cus = current_device_cus();
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) == hipSuccess &&
+            khz > 0 && khz <= 1000000)
+            c->probe_delay = (unsigned long long)khz * 60ull / 1000ull;   // 60 us
+    }
     c->ninst = ninst;
     c->batched = ninst > 1;
     unsigned cell_off = 0, p_off = 0;
@@ -1092,6 +1108,7 @@ int step_queued(dem_ctx *c, int nsteps)
             }
             if (c->probe_left > 0) {   // a timed probe step runs on its own
                 c->fused = c->probe_left % 2 == 0;
+                launch_probe_delay(c);
                 DEM_HIP(hipEventRecord(c->evp0, c->stream));
                 launch_force_step(c, pair_grid(c, c->npairs));
                 DEM_HIP(hipEventRecord(c->evp1, c->stream));
@@ -1190,6 +1207,7 @@ int dem_step(dem_ctx *c, int nsteps)
         bool probe = c->probe_left > 0;
         if (probe) {
             c->fused = c->probe_left % 2 == 0;
+            launch_probe_delay(c);
             DEM_HIP(hipEventRecord(c->evp0, c->stream));
         }
         if (T) {
@@ -2070,10 +2088,16 @@ struct AdjData {
     int nsteps = 0, K = 0, nseg = 0;
     unsigned n = 0, max_np = 0;
     std::vector<AdjSnap> ck;          // nseg + 1 boundaries
-    PRec *t_rec = nullptr;            // K records per slot
-    Hist *t_hist = nullptr;           // K x max_np incoming pair histories
-    Hist *t_whist = nullptr;          // K incoming wall histories per slot
-    std::vector<AdjList> pool;        // list versions of the segment being reversed
+    // Tapes and list versions of a segment, two sets: the recompute of segment s writes set
+    // s mod 2 while the reverse of segment s + 1 reads the other set on the reverse stream.
+    PRec *t_rec[2] = {nullptr, nullptr};     // K records per slot
+    Hist *t_hist[2] = {nullptr, nullptr};    // K x max_np incoming pair histories
+    Hist *t_whist[2] = {nullptr, nullptr};   // K incoming wall histories per slot
+    std::vector<AdjList> pool[2];            // list versions built during the segment
+    int tape_sets = 0;
+    hipStream_t rs = nullptr;                // reverse stream (default priority)
+    hipEvent_t rec_done[2] = {nullptr, nullptr}, rev_done[2] = {nullptr, nullptr};
+    bool rev_pending[2] = {false, false};
     SAdj *lam[2] = {nullptr, nullptr};
     double *hb[2] = {nullptr, nullptr};   // 4 doubles per pair
     double *ftb = nullptr, *thacc = nullptr;
@@ -2110,8 +2134,14 @@ void adj_release(AdjData *A)
 {
     if (!A) return;
     for (AdjSnap &S : A->ck) snap_free(S);
-    for (AdjList &L : A->pool) list_free(L);
-    ADJ_FREE(A->t_rec); ADJ_FREE(A->t_hist); ADJ_FREE(A->t_whist);
+    if (A->rs) (void)hipStreamSynchronize(A->rs);
+    for (int k = 0; k < 2; k++) {
+        for (AdjList &L : A->pool[k]) list_free(L);
+        ADJ_FREE(A->t_rec[k]); ADJ_FREE(A->t_hist[k]); ADJ_FREE(A->t_whist[k]);
+        if (A->rec_done[k]) (void)hipEventDestroy(A->rec_done[k]);
+        if (A->rev_done[k]) (void)hipEventDestroy(A->rev_done[k]);
+    }
+    if (A->rs) (void)hipStreamDestroy(A->rs);
     ADJ_FREE(A->lam[0]); ADJ_FREE(A->lam[1]); ADJ_FREE(A->hb[0]); ADJ_FREE(A->hb[1]);
     ADJ_FREE(A->ftb); ADJ_FREE(A->thacc); ADJ_FREE(A->pa);
     ADJ_FREE(A->pobs); ADJ_FREE(A->sobs); ADJ_FREE(A->part); ADJ_FREE(A->sums); ADJ_FREE(A->coef);
@@ -2253,7 +2283,7 @@ int record_hook(dem_ctx *c, void *arg, unsigned np)
 // One step of dem_step's schedule.  With `L` non-null a rebuild records its permutation,
 // history sources and resulting list into L (*rebuilt set); with `tape_step` >= 0 the force
 // step writes the tapes of that step.
-int adj_step(dem_ctx *c, AdjData *A, int tape_step, AdjList *L, bool *rebuilt)
+int adj_step(dem_ctx *c, AdjData *A, int set, int tape_step, AdjList *L, bool *rebuilt)
 {
     int rc;
     if (rebuilt) *rebuilt = false;
@@ -2283,9 +2313,9 @@ int adj_step(dem_ctx *c, AdjData *A, int tape_step, AdjList *L, bool *rebuilt)
     }
     if (A && c->npairs > A->max_np && tape_step >= 0) return DEM_ERR_CAPACITY;
     if (tape_step >= 0) {
-        c->tape_rec = A->t_rec + (size_t)tape_step * c->n;
-        c->tape_hist = A->t_hist + (size_t)tape_step * A->max_np;
-        c->tape_whist = A->t_whist + (size_t)tape_step * c->n;
+        c->tape_rec = A->t_rec[set] + (size_t)tape_step * c->n;
+        c->tape_hist = A->t_hist[set] + (size_t)tape_step * A->max_np;
+        c->tape_whist = A->t_whist[set] + (size_t)tape_step * c->n;
     }
     launch_force_step(c, pair_grid(c, c->npairs));
     c->tape_rec = nullptr;
@@ -2399,18 +2429,18 @@ AdjData *adj_get(dem_ctx *c)
 }
 
 // Reverse sweep over the recomputed segment [ck[seg], ck[seg+1]): `ks` steps.
-int reverse_segment(dem_ctx *c, AdjData *A, int ks, const std::vector<ListView> &views,
+int reverse_segment(dem_ctx *c, AdjData *A, int set, int ks, const std::vector<ListView> &views,
                     const std::vector<int> &step_ver, const std::vector<char> &step_reb, int &hc)
 {
-    hipStream_t st = c->stream;
+    hipStream_t st = A->rs;
     unsigned n = c->n;
     InstDev one = c->hinst[0];
     unsigned gn = cdiv(n, kBlock);
     for (int j = ks - 1; j >= 0; j--) {
         const ListView &V = views[step_ver[j]];
-        const PRec *tr = A->t_rec + (size_t)j * n;
-        const Hist *th = A->t_hist + (size_t)j * A->max_np;
-        const Hist *twh = A->t_whist + (size_t)j * n;
+        const PRec *tr = A->t_rec[set] + (size_t)j * n;
+        const Hist *th = A->t_hist[set] + (size_t)j * A->max_np;
+        const Hist *twh = A->t_whist[set] + (size_t)j * n;
         if (c->batched)
             hipLaunchKernelGGL(k_adj_particle<true>, dim3(gn), dim3(kBlock), 0, st, tr, twh, V.wflag,
                                A->lam[A->cur], A->ftb, A->thacc, n, (const InstDev *)c->dinst, one,
@@ -2496,7 +2526,7 @@ int dem_adj_forward(dem_ctx *c, int nsteps, int every, const dem_obs *obs, doubl
             if ((rc = step_queued(c, ks))) return rc;
         } else {
             for (int j = 0; j < ks; j++)
-                if ((rc = adj_step(c, nullptr, -1, nullptr, nullptr))) return rc;
+                if ((rc = adj_step(c, nullptr, 0, -1, nullptr, nullptr))) return rc;
         }
     }
     if ((rc = snap_save(c, A->ck[A->nseg]))) return rc;
@@ -2516,11 +2546,24 @@ int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *
     int rc;
     hipStream_t st = c->stream;
     size_t n = c->n, K = (size_t)A->K, mnp = std::max<size_t>(A->max_np, 1);
-    if (A->tape_K != A->K) {
-        ADJ_FREE(A->t_rec); ADJ_FREE(A->t_whist);
-        DEM_HIP(hipMalloc(&A->t_rec, sizeof(PRec) * K * n));
-        DEM_HIP(hipMalloc(&A->t_whist, sizeof(Hist) * K * n));
+    int sets = A->nseg > 1 ? 2 : 1;
+    if (A->tape_K != A->K || A->tape_sets < sets) {
+        for (int k = 0; k < 2; k++) {
+            ADJ_FREE(A->t_rec[k]); ADJ_FREE(A->t_whist[k]);
+        }
+        for (int k = 0; k < sets; k++) {
+            DEM_HIP(hipMalloc(&A->t_rec[k], sizeof(PRec) * K * n));
+            DEM_HIP(hipMalloc(&A->t_whist[k], sizeof(Hist) * K * n));
+        }
         A->tape_K = A->K;
+        A->tape_sets = sets;
+    }
+    if (!A->rs) {
+        DEM_HIP(hipStreamCreateWithFlags(&A->rs, hipStreamNonBlocking));
+        for (int k = 0; k < 2; k++) {
+            DEM_HIP(hipEventCreateWithFlags(&A->rec_done[k], hipEventDisableTiming));
+            DEM_HIP(hipEventCreateWithFlags(&A->rev_done[k], hipEventDisableTiming));
+        }
     }
     if (!A->lam[0]) {
         DEM_HIP(hipMalloc(&A->lam[0], sizeof(SAdj) * n));
@@ -2529,8 +2572,9 @@ int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *
         DEM_HIP(hipMalloc(&A->thacc, sizeof(double) * kThetaPer * n));
     }
     // pair-sized buffers follow the largest list of the forward
-    ADJ_FREE(A->t_hist); ADJ_FREE(A->hb[0]); ADJ_FREE(A->hb[1]); ADJ_FREE(A->pa);
-    DEM_HIP(hipMalloc(&A->t_hist, sizeof(Hist) * K * mnp));
+    ADJ_FREE(A->t_hist[0]); ADJ_FREE(A->t_hist[1]);
+    ADJ_FREE(A->hb[0]); ADJ_FREE(A->hb[1]); ADJ_FREE(A->pa);
+    for (int k = 0; k < sets; k++) DEM_HIP(hipMalloc(&A->t_hist[k], sizeof(Hist) * K * mnp));
     DEM_HIP(hipMalloc(&A->hb[0], sizeof(double) * 4 * mnp));
     DEM_HIP(hipMalloc(&A->hb[1], sizeof(double) * 4 * mnp));
     DEM_HIP(hipMalloc(&A->pa, sizeof(PAdj) * mnp));
@@ -2560,7 +2604,18 @@ int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *
                        A->thacc, z);
     DEM_HIP(hipGetLastError());
 
+    // Segments from the last: restore the checkpoint and recompute the segment with tapes on
+    // the context stream (the host follows the rebuild trigger step by step), then queue its
+    // reverse sweep on the reverse stream.  The reverse of segment s overlaps the recompute
+    // of segment s - 1, which writes the other tape set; before a set is rewritten the host
+    // waits for the reverse that last read it (segment s + 2).
+    A->rev_pending[0] = A->rev_pending[1] = false;
     for (int s = A->nseg - 1; s >= 0; s--) {
+        const int set = s % sets;
+        if (A->rev_pending[set]) {
+            DEM_HIP(hipEventSynchronize(A->rev_done[set]));
+            A->rev_pending[set] = false;
+        }
         const AdjSnap &S = A->ck[s];
         if ((rc = snap_restore(c, S))) return rc;
         int ks = std::min(A->K, A->nsteps - s * A->K);
@@ -2569,11 +2624,12 @@ int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *
         std::vector<int> step_ver(ks);
         std::vector<char> step_reb(ks, 0);
         int used = 0;
+        std::vector<AdjList> &pool = A->pool[set];
         for (int j = 0; j < ks; j++) {
-            if ((size_t)used >= A->pool.size()) A->pool.emplace_back();
+            if ((size_t)used >= pool.size()) pool.emplace_back();
             bool reb = false;
-            AdjList &L = A->pool[used];
-            if ((rc = adj_step(c, A, j, &L, &reb))) return rc;
+            AdjList &L = pool[used];
+            if ((rc = adj_step(c, A, set, j, &L, &reb))) return rc;
             if (reb) {
                 views.push_back(ListView{L.ab, L.off, L.toff, L.jt, L.wflag, L.perm, L.src, L.np});
                 used++;
@@ -2581,8 +2637,15 @@ int dem_adj_backward(dem_ctx *c, const double *gphi, const double *gcn, double *
             }
             step_ver[j] = (int)views.size() - 1;
         }
-        if ((rc = reverse_segment(c, A, ks, views, step_ver, step_reb, hc))) return rc;
+        DEM_HIP(hipEventRecord(A->rec_done[set], st));
+        DEM_HIP(hipStreamWaitEvent(A->rs, A->rec_done[set], 0));
+        if ((rc = reverse_segment(c, A, set, ks, views, step_ver, step_reb, hc))) return rc;
+        DEM_HIP(hipEventRecord(A->rev_done[set], A->rs));
+        A->rev_pending[set] = true;
     }
+    // the last reverse (segment 0) is the latest work on the reverse stream
+    DEM_HIP(hipStreamWaitEvent(st, A->rev_done[0], 0));
+    A->rev_pending[0] = A->rev_pending[1] = false;
     DEM_HIP(hipEventRecord(c->ev1, st));
     DEM_HIP(hipEventSynchronize(c->ev1));
     DEM_HIP(hipEventElapsedTime(&A->bwd_ms, c->ev0, c->ev1));
@@ -2638,7 +2701,8 @@ int dem_adj_info(dem_ctx *c, double *out)
         if (S.valid)
             snaps += (sizeof(PRec) + sizeof(Hist) + 6 * sizeof(unsigned)) * (size_t)A->n +
                      S.pcap * (sizeof(uint2) + sizeof(Hist) + sizeof(unsigned));
-    size_t tapes = (size_t)A->tape_K * (A->n * (sizeof(PRec) + sizeof(Hist)) + (size_t)A->max_np * sizeof(Hist));
+    size_t tapes = (size_t)A->tape_sets * A->tape_K *
+                   (A->n * (sizeof(PRec) + sizeof(Hist)) + (size_t)A->max_np * sizeof(Hist));
     out[4] = (double)snaps;
     out[5] = (double)tapes;
     return DEM_OK;
