@@ -134,33 +134,61 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(const uint2 *ab, unsigned n
 
 // Per slot: add the pair adjoints of the slot's own pairs (first member) and of the pairs
 // where it is second, in the forward gather's order; pair-law material adjoints of the own
-// pairs go to the particle index.
+// pairs go to the particle index.  Records are read two at a time (both loads in flight
+// before either is added; the additions keep the order).
+__device__ __forceinline__ void adj_add_own(double *s, double *t, const double2 *x)
+{
+    s[0] += x[0].x; s[1] += x[0].y; s[2] += x[1].x; s[3] += x[1].y; s[4] += x[2].x;
+    s[5] += x[2].y; s[6] += x[3].x; s[7] += x[3].y; s[8] += x[4].x;
+    t[0] += x[4].y; t[1] += x[5].x; t[2] += x[5].y; t[3] += x[6].x; t[4] += x[6].y;
+}
+
+__device__ __forceinline__ void adj_add_second(double *s, const double2 *x)
+{
+    s[0] += x[0].x; s[1] += x[0].y; s[2] += x[1].x; s[3] += x[1].y; s[4] += x[2].x;
+    s[5] += x[2].y; s[6] += x[3].x; s[7] += x[3].y; s[8] += x[4].x;
+}
+
 __global__ __launch_bounds__(kBlock) void k_adj_gather(SAdj *lam, const PAdj *pa, const unsigned *off,
                                                        const unsigned *toff, const unsigned *jt,
                                                        unsigned np, const PRec *tr,
                                                        double *thacc, unsigned n)
 {
+    static_assert(offsetof(PAdj, th) == 9 * sizeof(double) && offsetof(PAdj, b) == 14 * sizeof(double),
+                  "PAdj layout: a[9], th[5], b[9]");
     unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (a >= n) return;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double t[5] = {0, 0, 0, 0, 0};
     unsigned p0 = off[a], p1 = off[a + 1];
     if (p1 > np) p1 = np;
-    for (unsigned p = p0; p < p1; p++) {
-        const PAdj &P = pa[p];
+    for (unsigned p = p0; p < p1; p += 2) {
+        const bool two = p + 1 < p1;
+        const double *A0 = reinterpret_cast<const double *>(pa + p);
+        const double *A1 = reinterpret_cast<const double *>(pa + (two ? p + 1 : p));
+        double2 x0[7], x1[7];
 #pragma unroll
-        for (int k = 0; k < 9; k++) s[k] += P.a[k];
+        for (int j = 0; j < 7; j++) x0[j] = ld2(A0 + 2 * j);
 #pragma unroll
-        for (int k = 0; k < 5; k++) t[k] += P.th[k];
+        for (int j = 0; j < 7; j++) x1[j] = ld2(A1 + 2 * j);
+        adj_add_own(s, t, x0);
+        if (two) adj_add_own(s, t, x1);
     }
     unsigned q0 = toff[a], q1 = toff[a + 1];
     if (q1 > np) q1 = np;
-    for (unsigned k = q0; k < q1; k++) {
-        unsigned q = jt[k];
-        if (q >= np) continue;
-        const PAdj &P = pa[q];
+    for (unsigned k = q0; k < q1; k += 2) {
+        const bool two = k + 1 < q1;
+        unsigned qa = jt[k], qb = two ? jt[k + 1] : kEmpty;
+        const bool ua = qa < np, ub = qb < np;
+        const double *B0 = reinterpret_cast<const double *>(pa + (ua ? qa : 0u)) + 14;
+        const double *B1 = reinterpret_cast<const double *>(pa + (ub ? qb : 0u)) + 14;
+        double2 x0[5], x1[5];
 #pragma unroll
-        for (int j = 0; j < 9; j++) s[j] += P.b[j];
+        for (int j = 0; j < 5; j++) x0[j] = ld2(B0 + 2 * j);
+#pragma unroll
+        for (int j = 0; j < 5; j++) x1[j] = ld2(B1 + 2 * j);
+        if (ua) adj_add_second(s, x0);
+        if (ub) adj_add_second(s, x1);
     }
     SAdj &L = lam[a];
 #pragma unroll

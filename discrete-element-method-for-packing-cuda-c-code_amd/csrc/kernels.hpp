@@ -673,9 +673,18 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // finish_particle.  No per-pair output buffer: each law is evaluated twice instead of its
 // 80-byte result being written once and gathered twice.  Records and histories are read
 // from rin / hin and written to rout / hout, so no lane sees a neighbour's updated state.
-// A candidate is first tested for contact from the partner's position and radius alone
-// (law_touch, the law's own test); the rest of the partner's record and the pair's history
-// are read only for pairs in contact (a pair out of contact has an all-zero law result).
+//
+// Candidates go in chunks of up to 32: a first pass reads the partners' positions and radii
+// four candidates at a time (independent loads in flight together) and applies the law's
+// own contact test (law_touch); pairs out of contact get the all-zero law result (history
+// reset) at once, and a second pass evaluates the law of the pairs in contact, in order,
+// reading the rest of the partner's record and the history only for them.  A pair out of
+// contact contributes +0.0 to every sum; the running sums start at +0.0 and so are never
+// -0.0, for which x + 0.0 == x and x - 0.0 == x bit for bit: leaving those terms out keeps
+// the reference order's result.
+constexpr unsigned kChunk = 32;
+constexpr unsigned kLook = 4;
+
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
@@ -700,50 +709,96 @@ __global__ __launch_bounds__(kBlock) void k_step(
         if (p1 > np) p1 = np;
         unsigned q0 = toff[a], q1 = toff[a + 1];
         if (q1 > np) q1 = np;
-        for (unsigned p = p0; p < p1; p++) {
-            unsigned b = ab[p].y;
-            const double *B = reinterpret_cast<const double *>(rin + b);
-            double2 b0 = ld2(B), b1 = ld2(B + 2);
-            bool con = law_touch(xa, d3{b0.x, b0.y, b1.x}, ra, b1.y);
-            d3 fa = d3{0.0, 0.0, 0.0}, ta = fa, tb = fa, ho = fa;
-            double fo = 0.0;
-            if (con || tape_hist) {
+        if (tape_hist)   // the incoming histories of the own pairs, for the adjoint
+            for (unsigned p = p0; p < p1; p++) {
+                const double *H = reinterpret_cast<const double *>(hin + p);
+                double *T = reinterpret_cast<double *>(tape_hist + p);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                st2(T, h0.x, h0.y);
+                st2(T + 2, h1.x, h1.y);
+            }
+        for (unsigned base = p0; base < p1; base += kChunk) {
+            const unsigned end = min(base + kChunk, p1);
+            unsigned mask = 0;
+            for (unsigned p = base; p < end; p += kLook) {
+                unsigned b[kLook];
+                double2 c0[kLook], c1[kLook];
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    const double *B = reinterpret_cast<const double *>(rin + b[i]);
+                    c0[i] = ld2(B);
+                    c1[i] = ld2(B + 2);
+                }
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    if (p + i >= end) break;
+                    if (law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y)) {
+                        mask |= 1u << (p + i - base);
+                    } else {
+                        double *HW = reinterpret_cast<double *>(hout + p + i);
+                        st2(HW, 0.0, 0.0);
+                        st2(HW + 2, 0.0, 0.0);
+                    }
+                }
+            }
+            while (mask) {
+                const unsigned p = base + (unsigned)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
                 const double *H = reinterpret_cast<const double *>(hin + p);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
-                if (tape_hist) {
-                    double *T = reinterpret_cast<double *>(tape_hist + p);
-                    st2(T, h0.x, h0.y);
-                    st2(T + 2, h1.x, h1.y);
-                }
-                if (con)
-                    pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
-                                               inst_a, fa, ta, tb, ho, fo);
+                d3 fa, ta, tb, ho;
+                double fo;
+                bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst, one.pp,
+                                                      one.dt, inst_a, fa, ta, tb, ho, fo);
+                double *HW = reinterpret_cast<double *>(hout + p);
+                st2(HW, ho.x, ho.y);
+                st2(HW + 2, ho.z, fo);
+                fx += fa.x; fy += fa.y; fz += fa.z;
+                tx += ta.x; ty += ta.y; tz += ta.z;
+                c += con ? 1u : 0u;
             }
-            double *HW = reinterpret_cast<double *>(hout + p);
-            st2(HW, ho.x, ho.y);
-            st2(HW + 2, ho.z, fo);
-            fx += fa.x; fy += fa.y; fz += fa.z;
-            tx += ta.x; ty += ta.y; tz += ta.z;
-            c += con ? 1u : 0u;
         }
-        for (unsigned k = q0; k < q1; k++) {
-            unsigned q = jt[k];
-            if (q >= np) continue;
-            unsigned f = jf[k];
-            const double *F = reinterpret_cast<const double *>(rin + f);
-            double2 f0 = ld2(F), f1 = ld2(F + 2);
-            bool con = law_touch(d3{f0.x, f0.y, f1.x}, xa, f1.y, ra);
-            d3 fa = d3{0.0, 0.0, 0.0}, ta = fa, tb = fa, ho = fa;
-            double fo = 0.0;
-            if (con) {
+        for (unsigned base = q0; base < q1; base += kChunk) {
+            const unsigned end = min(base + kChunk, q1);
+            unsigned mask = 0;
+            for (unsigned k = base; k < end; k += kLook) {
+                unsigned q[kLook], f[kLook];
+                double2 c0[kLook], c1[kLook];
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    bool in = k + i < end;
+                    q[i] = in ? jt[k + i] : kEmpty;
+                    f[i] = in ? jf[k + i] : a;
+                }
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    const double *F = reinterpret_cast<const double *>(rin + (q[i] < np ? f[i] : a));
+                    c0[i] = ld2(F);
+                    c1[i] = ld2(F + 2);
+                }
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++)
+                    if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
+                        mask |= 1u << (k + i - base);
+            }
+            while (mask) {
+                const unsigned k = base + (unsigned)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const unsigned q = jt[k];
+                const double *F = reinterpret_cast<const double *>(rin + jf[k]);
                 const double *H = reinterpret_cast<const double *>(hin + q);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
-                pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst, one.pp, one.dt,
-                                           inst_a, fa, ta, tb, ho, fo);
+                d3 fa, ta, tb, ho;
+                double fo;
+                bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst, one.pp,
+                                                      one.dt, inst_a, fa, ta, tb, ho, fo);
+                fx -= fa.x; fy -= fa.y; fz -= fa.z;
+                tx += tb.x; ty += tb.y; tz += tb.z;
+                c += con ? 1u : 0u;
             }
-            fx -= fa.x; fy -= fa.y; fz -= fa.z;
-            tx += tb.x; ty += tb.y; tz += tb.z;
-            c += con ? 1u : 0u;
         }
         flag = finish_particle<BATCHED>(RA, reinterpret_cast<double *>(rout + a), a, fx, fy, fz,
                                         tx, ty, tz, c, wflag, wforce, inst, one, contacti, fdbg,
