@@ -380,6 +380,19 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const uns
     }
 }
 
+// Compact positions of restored records: cpos[s] = (px, py, pz, rad) of rec[s].
+__global__ __launch_bounds__(kBlock) void k_cpos_from_rec(const PRec *rec, unsigned n,
+                                                          double4 *cpos)
+{
+    unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const double *R = reinterpret_cast<const double *>(rec + s);
+    double2 p01 = ld2(R), p23 = ld2(R + 2);
+    double *C = reinterpret_cast<double *>(cpos + s);
+    st2(C, p01.x, p01.y);
+    st2(C + 2, p23.x, p23.y);
+}
+
 // jf from a restored list: jf[k] = first member of pair jt[k].
 __global__ __launch_bounds__(kBlock) void k_owner(const uint2 *ab, const unsigned *jt,
                                                   unsigned np, unsigned *jf)
@@ -531,7 +544,7 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
                                                     const unsigned *wflag, const double *wforce,
                                                     const InstDev *inst, const InstDev &one,
                                                     unsigned *contacti, double *fdbg,
-                                                    PRec *tape_rec)
+                                                    PRec *tape_rec, double4 *cpos_out)
 {
     double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
             r5 = ld2(R + 10), r6 = ld2(R + 12);
@@ -587,6 +600,9 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
     st2(W + 12, qx, qy);
     W[14] = qzz;
     *reinterpret_cast<uint2 *>(W + 15) = tail;
+    double *CP = reinterpret_cast<double *>(cpos_out + a);
+    st2(CP, x.x, x.y);
+    st2(CP + 2, x.z, r1.y);
     contacti[a] = c;
     double q2 = qx * qx + qy * qy + qzz * qzz;
     return q2 > I.ann2 ? 1u : 0u;
@@ -599,9 +615,9 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
-    const unsigned *toff, const unsigned *jt, const POut *out, unsigned n, const InstDev *inst,
-    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec,
-    const unsigned *gate)
+    const unsigned *toff, const unsigned *jt, const POut *out, double4 *cpos, unsigned n,
+    const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
+    PRec *tape_rec, const unsigned *gate)
 {
     if (gated_off(gate)) return;
     const unsigned np = ctl->npairs;
@@ -637,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_particle(
         }
         double *R = reinterpret_cast<double *>(rec + a);
         flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wflag, wforce, inst,
-                                        one, contacti, fdbg, tape_rec);
+                                        one, contacti, fdbg, tape_rec, cpos);
     }
     // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
@@ -675,8 +691,9 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // from rin / hin and written to rout / hout, so no lane sees a neighbour's updated state.
 //
 // Candidates go in chunks of up to 32: a first pass reads the partners' positions and radii
-// four candidates at a time (independent loads in flight together) and applies the law's
-// own contact test (law_touch); pairs out of contact get the all-zero law result (history
+// from the compact copy cin (four slots per cache line), four candidates at a time
+// (independent loads in flight together), and applies the law's own contact test
+// (law_touch); pairs out of contact get the all-zero law result (history
 // reset) at once, and a second pass evaluates the law of the pairs in contact, in order,
 // reading the rest of the partner's record and the history only for them.  A pair out of
 // contact contributes +0.0 to every sum; the running sums start at +0.0 and so are never
@@ -684,25 +701,44 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // the reference order's result.
 constexpr unsigned kChunk = 32;
 constexpr unsigned kLook = 4;
+// The block's own records are staged through LDS: read from rin and written to rout in
+// 16-byte chunks by consecutive lanes (1 KB of contiguous memory per wave instruction instead
+// of 64 scattered lines), each lane working on its row in between.  A row is the 128-byte
+// record plus 16 bytes of padding, which spreads the lanes' 16-byte row accesses over all
+// LDS banks.  Dynamic LDS: blockDim.x * kRow doubles.
+constexpr unsigned kRow = 18;
 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
-    const double *wforce, unsigned n, const InstDev *inst, InstDev one, unsigned *contacti,
-    double *fdbg, DevCtl *ctl, PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
+    const double *wforce, const double4 *cin, double4 *cout, unsigned n, const InstDev *inst,
+    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec,
+    Hist *tape_hist, const unsigned *gate)
 {
     if (gated_off(gate)) return;
+    extern __shared__ double lrec[];
     const unsigned np = ctl->npairs;
-    unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    bool live = a < n;
+    const unsigned bd = blockDim.x, t = threadIdx.x;
+    const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * bd;
+    const unsigned nb = s0 < n ? min(bd, n - s0) : 0u;
+    {
+        const double *G = reinterpret_cast<const double *>(rin + s0);
+        for (unsigned k = t; k < 8u * nb; k += bd) {
+            double2 v = ld2(G + 2 * k);
+            st2(lrec + (k >> 3) * kRow + 2 * (k & 7u), v.x, v.y);
+        }
+    }
+    __syncthreads();
+    const unsigned a = s0 + t;
+    double *RA = lrec + t * kRow;
+    bool live = t < nb;
     unsigned flag = 0;
     if (live) {
-        const double *RA = reinterpret_cast<const double *>(rin + a);
         double2 a0 = ld2(RA), a1 = ld2(RA + 2);
         const d3 xa = d3{a0.x, a0.y, a1.x};
         const double ra = a1.y;
-        unsigned inst_a = BATCHED ? rin[a].inst : 0u;
+        unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
         unsigned p0 = off[a], p1 = off[a + 1];
@@ -727,7 +763,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
                 for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) {
-                    const double *B = reinterpret_cast<const double *>(rin + b[i]);
+                    const double *B = reinterpret_cast<const double *>(cin + b[i]);
                     c0[i] = ld2(B);
                     c1[i] = ld2(B + 2);
                 }
@@ -775,7 +811,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
                 }
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) {
-                    const double *F = reinterpret_cast<const double *>(rin + (q[i] < np ? f[i] : a));
+                    const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
                     c0[i] = ld2(F);
                     c1[i] = ld2(F + 2);
                 }
@@ -800,9 +836,16 @@ __global__ __launch_bounds__(kBlock) void k_step(
                 c += con ? 1u : 0u;
             }
         }
-        flag = finish_particle<BATCHED>(RA, reinterpret_cast<double *>(rout + a), a, fx, fy, fz,
-                                        tx, ty, tz, c, wflag, wforce, inst, one, contacti, fdbg,
-                                        tape_rec);
+        flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, wflag, wforce,
+                                        inst, one, contacti, fdbg, tape_rec, cout);
+    }
+    __syncthreads();
+    {
+        double *G = reinterpret_cast<double *>(rout + s0);
+        for (unsigned k = t; k < 8u * nb; k += bd) {
+            double2 v = ld2(lrec + (k >> 3) * kRow + 2 * (k & 7u));
+            st2(G + 2 * k, v.x, v.y);
+        }
     }
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }

@@ -179,7 +179,10 @@ struct dem_ctx {
     int pc = 0;   // index of the current wall histories and inverse slot map (flips at rebuilds)
     unsigned *contacti = nullptr;
     unsigned *wflag = nullptr, *wlist = nullptr;   // wall candidates of the current rebuild
-    double4 *cpos = nullptr;   // position + radius per slot, written by the reorder
+    // Position + radius per slot (32 bytes: four slots per cache line), cpos[rc] mirroring
+    // rec[rc]: written by the reorder and by both step paths, read by the candidate searches
+    // and the fused step's contact tests.
+    double4 *cpos[2] = {nullptr, nullptr};
     unsigned *cid = nullptr;   // particle index per slot, written by the reorder
     double *wforce = nullptr;                      // k_walls output, 8 doubles per slot
     double *fdbg = nullptr;
@@ -345,23 +348,23 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
                            c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
-                           c->wlist, c->ctl, c->tape_perm, c->cpos, c->cid, active);
+                           c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cid, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
                            c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
-                           c->tape_perm, c->cpos, c->cid, active);
+                           c->tape_perm, c->cpos[nr], c->cid, active);
     c->rc = nr;
     c->pc = 1 - c->pc;
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos, (const unsigned *)c->cid,
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->ng, c->nl, active);
     else
         hipLaunchKernelGGL(k_nbr_count<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos, (const unsigned *)c->cid,
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->ng, c->nl, active);
     int nl = 1 - c->lc;
@@ -389,13 +392,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     unsigned capo = std::min(std::min(c->cap[ol], c->out_cap), c->hcap[oh]);
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos, (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos, (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
@@ -490,19 +493,21 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by k_walls, so they move with the records
         if (c->batched)
-            hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
+            hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
-                               c->ctl, c->tape_rec, c->tape_hist, gate);
+                               (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->n, dinst, one, c->contacti, fdbg, c->ctl, c->tape_rec,
+                               c->tape_hist, gate);
         else
-            hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), 0, st, rin, c->rec[nr],
+            hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               (const double *)c->wforce, c->n, dinst, one, c->contacti, fdbg,
-                               c->ctl, c->tape_rec, c->tape_hist, gate);
+                               (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->n, dinst, one, c->contacti, fdbg, c->ctl, c->tape_rec,
+                               c->tape_hist, gate);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -511,14 +516,14 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
-                           (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
-                           c->contacti, fdbg, c->ctl, c->tape_rec, gate);
+                           (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
+                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate);
     } else {
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
-                           (const unsigned *)c->jt, (const POut *)c->out, c->n, dinst, one,
-                           c->contacti, fdbg, c->ctl, c->tape_rec, gate);
+                           (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
+                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate);
     }
     if (!c->fused) c->last_path = 1;
     if (end) (void)hipEventRecord(end, st);
@@ -598,7 +603,7 @@ void free_ctx(dem_ctx *c)
     if (!c) return;
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
-                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos, c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->jf, c->ctl};
     for (void *p : bufs)
@@ -851,7 +856,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->wflag, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->wlist, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->wforce, sizeof(double) * 8 * (size_t)n));
-    DEM_HIP(hipMalloc(&c->cpos, sizeof(double4) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cpos[0], sizeof(double4) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cpos[1], sizeof(double4) * (size_t)n));
     DEM_HIP(hipMalloc(&c->cid, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
@@ -2219,6 +2225,8 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
     if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
     DEM_HIP(hipMemcpyAsync(c->rec[c->rc], S.rec, sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
     DEM_HIP(hipMemcpyAsync(c->whist[c->pc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_cpos_from_rec, dim3(cdiv((unsigned)n, kBlock)), dim3(kBlock), 0, st,
+                       (const PRec *)c->rec[c->rc], (unsigned)n, c->cpos[c->rc]);
     DEM_HIP(hipMemcpyAsync(c->inv[c->pc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
     DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
     if (S.have_list) {
