@@ -707,6 +707,10 @@ constexpr unsigned kLook = 4;
 // record plus 16 bytes of padding, which spreads the lanes' 16-byte row accesses over all
 // LDS banks.  Dynamic LDS: blockDim.x * kRow doubles.
 constexpr unsigned kRow = 18;
+// Own pairs of a block out of contact get their zero history in one cooperative pass over the
+// block's contiguous own-pair range (consecutive lanes on consecutive pairs), steered by a
+// contact bitmap in LDS; a block with more than kBits own pairs zeroes them lane by lane.
+constexpr unsigned kBits = 8192;
 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_step(
@@ -718,10 +722,16 @@ __global__ __launch_bounds__(kBlock) void k_step(
 {
     if (gated_off(gate)) return;
     extern __shared__ double lrec[];
+    __shared__ unsigned cbits[kBits / 32];
     const unsigned np = ctl->npairs;
     const unsigned bd = blockDim.x, t = threadIdx.x;
     const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * bd;
     const unsigned nb = s0 < n ? min(bd, n - s0) : 0u;
+    // the block's own pairs [pb0, pb1)
+    const unsigned pb1 = nb ? min(off[s0 + nb], np) : 0u;
+    const unsigned pb0 = nb ? min(off[s0], pb1) : 0u;
+    const bool coop = pb1 - pb0 <= kBits;
+    for (unsigned k = t; k < kBits / 32; k += bd) cbits[k] = 0u;
     {
         const double *G = reinterpret_cast<const double *>(rin + s0);
         for (unsigned k = t; k < 8u * nb; k += bd) {
@@ -772,7 +782,8 @@ __global__ __launch_bounds__(kBlock) void k_step(
                     if (p + i >= end) break;
                     if (law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y)) {
                         mask |= 1u << (p + i - base);
-                    } else {
+                        if (coop) atomicOr(&cbits[(p + i - pb0) >> 5], 1u << ((p + i - pb0) & 31u));
+                    } else if (!coop) {
                         double *HW = reinterpret_cast<double *>(hout + p + i);
                         st2(HW, 0.0, 0.0);
                         st2(HW + 2, 0.0, 0.0);
@@ -847,6 +858,13 @@ __global__ __launch_bounds__(kBlock) void k_step(
             st2(G + 2 * k, v.x, v.y);
         }
     }
+    if (coop)
+        for (unsigned k = t; k < pb1 - pb0; k += bd)
+            if (!((cbits[k >> 5] >> (k & 31u)) & 1u)) {
+                double *HW = reinterpret_cast<double *>(hout + pb0 + k);
+                st2(HW, 0.0, 0.0);
+                st2(HW + 2, 0.0, 0.0);
+            }
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 
