@@ -710,15 +710,44 @@ constexpr unsigned kRow = 18;
 // Own pairs of a block out of contact get their zero history in one cooperative pass over the
 // block's contiguous own-pair range (consecutive lanes on consecutive pairs), steered by a
 // contact bitmap in LDS; a block with more than kBits own pairs zeroes them lane by lane.
+// zs[p] bit k records that history buffer k holds an exact zero for pair p.  zmode: bits 0-1
+// the bit of the buffer written now (hout), bit 2 set when zs is valid -- then a pair out of
+// contact whose bit is already set is not rewritten (most pairs of a loose bed stay out of
+// contact for many steps); otherwise every pair out of contact is written and zs rebuilt.
+// Both history buffers therefore always hold the exact history of every pair.
 constexpr unsigned kBits = 8192;
+
+__device__ __forceinline__ void zero_hist(Hist *h, unsigned p)
+{
+    double *HW = reinterpret_cast<double *>(h + p);
+    st2(HW, 0.0, 0.0);
+    st2(HW + 2, 0.0, 0.0);
+}
+
+// zs update of pair p for this step (con: in contact); writes the zero history if needed.
+__device__ __forceinline__ void zs_pair(unsigned char *zs, Hist *hout, unsigned p, bool con,
+                                        unsigned zmode)
+{
+    const unsigned wbit = zmode & 3u;
+    const bool lazy = (zmode & 4u) != 0u;
+    const unsigned z = lazy ? zs[p] : 0u;
+    unsigned nz;
+    if (con) {
+        nz = z & ~wbit;
+    } else {
+        if (!(z & wbit)) zero_hist(hout, p);
+        nz = z | wbit;
+    }
+    if (!lazy || nz != z) zs[p] = (unsigned char)nz;
+}
 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
-    const double *wforce, const double4 *cin, double4 *cout, unsigned n, const InstDev *inst,
-    InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl, PRec *tape_rec,
-    Hist *tape_hist, const unsigned *gate)
+    const double *wforce, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
+    unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
+    PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
 {
     if (gated_off(gate)) return;
     extern __shared__ double lrec[];
@@ -780,13 +809,12 @@ __global__ __launch_bounds__(kBlock) void k_step(
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) {
                     if (p + i >= end) break;
-                    if (law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y)) {
-                        mask |= 1u << (p + i - base);
-                        if (coop) atomicOr(&cbits[(p + i - pb0) >> 5], 1u << ((p + i - pb0) & 31u));
-                    } else if (!coop) {
-                        double *HW = reinterpret_cast<double *>(hout + p + i);
-                        st2(HW, 0.0, 0.0);
-                        st2(HW + 2, 0.0, 0.0);
+                    const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
+                    if (con) mask |= 1u << (p + i - base);
+                    if (coop) {
+                        if (con) atomicOr(&cbits[(p + i - pb0) >> 5], 1u << ((p + i - pb0) & 31u));
+                    } else {
+                        zs_pair(zs, hout, p + i, con, zmode);
                     }
                 }
             }
@@ -860,11 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_step(
     }
     if (coop)
         for (unsigned k = t; k < pb1 - pb0; k += bd)
-            if (!((cbits[k >> 5] >> (k & 31u)) & 1u)) {
-                double *HW = reinterpret_cast<double *>(hout + pb0 + k);
-                st2(HW, 0.0, 0.0);
-                st2(HW + 2, 0.0, 0.0);
-            }
+            zs_pair(zs, hout, pb0 + k, (cbits[k >> 5] >> (k & 31u)) & 1u, zmode);
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 

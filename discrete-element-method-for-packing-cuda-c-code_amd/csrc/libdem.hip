@@ -234,6 +234,11 @@ struct dem_ctx {
     POut *out = nullptr;
     unsigned *jt = nullptr;
     unsigned *jf = nullptr;   // jt's companion: the slot of the pair's first member
+    // Per pair of the current list, bit k: hist[k] holds an exact zero for the pair.  Valid
+    // (zs_ok) only across consecutive fused steps: a rebuild, a split step or a restore of the
+    // histories leaves it stale, and the next fused step then rewrites it from scratch.
+    unsigned char *zs = nullptr;
+    bool zs_ok = false;
     unsigned out_cap = 0;
     unsigned npairs = 0;
     unsigned np_peak = 0;     // largest list built by rebuild_host since the last reset
@@ -298,12 +303,16 @@ int alloc_out(dem_ctx *c, unsigned cap)
     if (c->out) DEM_HIP(hipFree(c->out));
     if (c->jt) DEM_HIP(hipFree(c->jt));
     if (c->jf) DEM_HIP(hipFree(c->jf));
+    if (c->zs) DEM_HIP(hipFree(c->zs));
     c->out = nullptr;
     c->jt = nullptr;
     c->jf = nullptr;
+    c->zs = nullptr;
+    c->zs_ok = false;
     DEM_HIP(hipMalloc(&c->out, (size_t)std::max(cap, 1u) * sizeof(POut)));
     DEM_HIP(hipMalloc(&c->jt, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
     DEM_HIP(hipMalloc(&c->jf, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
+    DEM_HIP(hipMalloc(&c->zs, (size_t)std::max(cap, 1u)));
     c->out_cap = cap;
     c->buf_gen++;
     return DEM_OK;
@@ -411,6 +420,7 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     c->have_list = true;
     c->n_list = n;
     c->out_valid = false;
+    c->zs_ok = false;
     return DEM_OK;
 }
 
@@ -490,6 +500,10 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
     if (mid) (void)hipEventRecord(mid, st);
     if (c->fused) {
         int nr = 1 - c->rc, nh = 1 - c->hc;
+        // zero-history bookkeeping: bit nh of zs describes the buffer written now; with
+        // valid bookkeeping (lazy) pairs already known to be zero there are not rewritten
+        const unsigned zmode = (1u << nh) | (c->zs_ok ? 4u : 0u);
+        c->zs_ok = true;
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by k_walls, so they move with the records
         if (c->batched)
@@ -498,16 +512,16 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
-                               c->n, dinst, one, c->contacti, fdbg, c->ctl, c->tape_rec,
-                               c->tape_hist, gate);
+                               c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
+                               c->tape_rec, c->tape_hist, gate);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
-                               c->n, dinst, one, c->contacti, fdbg, c->ctl, c->tape_rec,
-                               c->tape_hist, gate);
+                               c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
+                               c->tape_rec, c->tape_hist, gate);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -525,7 +539,10 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                            (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
                            one, c->contacti, fdbg, c->ctl, c->tape_rec, gate);
     }
-    if (!c->fused) c->last_path = 1;
+    if (!c->fused) {
+        c->last_path = 1;
+        c->zs_ok = false;   // the split step rewrites hist[hc] in place
+    }
     if (end) (void)hipEventRecord(end, st);
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d, gate, slot);
 }
@@ -605,7 +622,7 @@ void free_ctx(dem_ctx *c)
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
-                    c->hist[1], c->out, c->jt, c->jf, c->ctl};
+                    c->hist[1], c->out, c->jt, c->jf, c->zs, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->S.release();
@@ -995,6 +1012,7 @@ int dem_set_state(dem_ctx *c, const double *pos, const double *pdis, const doubl
     DEM_HIP(hipMemset(c->contacti, 0, sizeof(unsigned) * n));
     c->have_list = false;
     c->out_valid = false;
+    c->zs_ok = false;
     c->last_path = 0;
     c->need_rebuild = true;
     c->npairs = 0;
@@ -1290,9 +1308,13 @@ int dem_step_graph(dem_ctx *c, int nsteps)
         DEM_HIP(hipFree(c->out));
         DEM_HIP(hipFree(c->jt));
         DEM_HIP(hipFree(c->jf));
+        DEM_HIP(hipFree(c->zs));
         c->out = no;
         c->jt = nj;
         c->jf = nf;
+        c->zs = nullptr;
+        c->zs_ok = false;
+        DEM_HIP(hipMalloc(&c->zs, (size_t)want));
         c->out_cap = want;
         c->buf_gen++;
     }
@@ -2224,6 +2246,7 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
         if (np > c->hcap[k] && (rc = alloc_hist(c, k, grow))) return rc;
     if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
     DEM_HIP(hipMemcpyAsync(c->rec[c->rc], S.rec, sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
+    c->zs_ok = false;
     DEM_HIP(hipMemcpyAsync(c->whist[c->pc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_cpos_from_rec, dim3(cdiv((unsigned)n, kBlock)), dim3(kBlock), 0, st,
                        (const PRec *)c->rec[c->rc], (unsigned)n, c->cpos[c->rc]);
