@@ -487,9 +487,40 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
     }
 }
 
-// Wall contact and history for the rebuild's wall candidates (calculateforceiwD's wall part,
-// neiglist.cu:286-301).  wforce[slot] = (F xyz, hits, T xyz, 0); history zeroed when no wall
-// is touched.  Grid-stride over the device-resident candidate count.
+// Wall contact of slot a (calculateforceiwD's wall part, neiglist.cu:286-301) from its record
+// R (pos/rad, pdis/mass, angv/inert): reads the slot's wall history, writes it back (zeroed
+// when no wall is touched) and returns the number of walls hit with the wall force and
+// torque.  tape_whist receives the incoming history, for the adjoint.
+template <bool BATCHED>
+__device__ __forceinline__ int wall_slot(const double *R, const InstDev &I, Hist *whist,
+                                         unsigned a, Hist *tape_whist, d3 &fw, d3 &tw)
+{
+    double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
+            r5 = ld2(R + 10);
+    double *WH = reinterpret_cast<double *>(whist + a);
+    double2 w0 = ld2(WH), w1 = ld2(WH + 2);
+    if (tape_whist) {
+        double *T = reinterpret_cast<double *>(tape_whist + a);
+        st2(T, w0.x, w0.y);
+        st2(T + 2, w1.x, w1.y);
+    }
+    d3 hw;
+    double fow;
+    int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
+                         d3{r4.x, r4.y, r5.x}, r1.y, r3.y, d3{w0.x, w0.y, w1.x}, I.dt, I.pw,
+                         fw, tw, hw, fow);
+    if (hits) {
+        st2(WH, hw.x, hw.y);
+        st2(WH + 2, hw.z, fow);
+    } else {
+        st2(WH, 0.0, 0.0);
+        st2(WH + 2, 0.0, 0.0);
+    }
+    return hits;
+}
+
+// Wall contacts of the rebuild's wall candidates for the split path: wforce[slot] =
+// (F xyz, hits, T xyz, 0).  Grid-stride over the device-resident candidate count.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
                                                   const unsigned *wlist, const DevCtl *ctl,
@@ -501,29 +532,10 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
     const unsigned nw = ctl->nwall;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
         unsigned a = wlist[i];
-        const double *R = reinterpret_cast<const double *>(rec + a);
-        double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
-                r5 = ld2(R + 10);
         const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
-        double *WH = reinterpret_cast<double *>(whist + a);
-        double2 w0 = ld2(WH), w1 = ld2(WH + 2);
-        if (tape_whist) {
-            double *T = reinterpret_cast<double *>(tape_whist + a);
-            st2(T, w0.x, w0.y);
-            st2(T + 2, w1.x, w1.y);
-        }
-        d3 fw, tw, hw;
-        double fow;
-        int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
-                             d3{r4.x, r4.y, r5.x}, r1.y, r3.y, d3{w0.x, w0.y, w1.x}, I.dt, I.pw,
-                             fw, tw, hw, fow);
-        if (hits) {
-            st2(WH, hw.x, hw.y);
-            st2(WH + 2, hw.z, fow);
-        } else {
-            st2(WH, 0.0, 0.0);
-            st2(WH + 2, 0.0, 0.0);
-        }
+        d3 fw, tw;
+        int hits = wall_slot<BATCHED>(reinterpret_cast<const double *>(rec + a), I, whist, a,
+                                      tape_whist, fw, tw);
         double *W = wforce + 8 * (size_t)a;
         st2(W, fw.x, fw.y);
         st2(W + 2, fw.z, (double)hits);
@@ -540,9 +552,9 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
 template <bool BATCHED>
 __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, unsigned a,
                                                     double fx, double fy, double fz, double tx,
-                                                    double ty, double tz, unsigned c,
-                                                    const unsigned *wflag, const double *wforce,
-                                                    const InstDev *inst, const InstDev &one,
+                                                    double ty, double tz, unsigned c, bool wall,
+                                                    d3 fw, d3 tw, const InstDev *inst,
+                                                    const InstDev &one,
                                                     unsigned *contacti, double *fdbg,
                                                     PRec *tape_rec, double4 *cpos_out)
 {
@@ -568,14 +580,10 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
     fx = fx + 0.0;
     fy = fy + 0.0;
     fz = fz + (-m);
-    // walls (computed by k_walls for this rebuild's wall candidates)
-    if (wflag[a]) {
-        const double *WF = wforce + 8 * (size_t)a;
-        double2 w0 = ld2(WF), w1 = ld2(WF + 2), w2 = ld2(WF + 4), w3 = ld2(WF + 6);
-        if (w1.y != 0.0) {
-            fx += w0.x; fy += w0.y; fz += w1.x;
-            tx += w2.x; ty += w2.y; tz += w3.x;
-        }
+    // walls (wall candidates of this rebuild that touch a wall)
+    if (wall) {
+        fx += fw.x; fy += fw.y; fz += fw.z;
+        tx += tw.x; ty += tw.y; tz += tw.z;
     }
     if (fdbg) {
         double *D = fdbg + 6 * (size_t)a;
@@ -651,8 +659,19 @@ __global__ __launch_bounds__(kBlock) void k_particle(
             tx += o3.y; ty += o4.x; tz += o4.y;
             c += o1.y != 0.0 ? 1u : 0u;
         }
+        bool wall = false;
+        d3 fw = d3{0.0, 0.0, 0.0}, tw = d3{0.0, 0.0, 0.0};
+        if (wflag[a]) {   // k_walls' result for this rebuild's wall candidates
+            const double *WF = wforce + 8 * (size_t)a;
+            double2 w0 = ld2(WF), w1 = ld2(WF + 2), w2 = ld2(WF + 4), w3 = ld2(WF + 6);
+            if (w1.y != 0.0) {
+                wall = true;
+                fw = d3{w0.x, w0.y, w1.x};
+                tw = d3{w2.x, w2.y, w3.x};
+            }
+        }
         double *R = reinterpret_cast<double *>(rec + a);
-        flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wflag, wforce, inst,
+        flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wall, fw, tw, inst,
                                         one, contacti, fdbg, tape_rec, cpos);
     }
     // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
@@ -689,17 +708,25 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // finish_particle.  No per-pair output buffer: each law is evaluated twice instead of its
 // 80-byte result being written once and gathered twice.  Records and histories are read
 // from rin / hin and written to rout / hout, so no lane sees a neighbour's updated state.
+// Wall candidates of the last rebuild evaluate their wall contact here too (wall history
+// per slot, in place), before the pair work, while few registers are live.
 //
-// Candidates go in chunks of up to 32: a first pass reads the partners' positions and radii
-// from the compact copy cin (four slots per cache line), four candidates at a time
-// (independent loads in flight together), and applies the law's own contact test
-// (law_touch); pairs out of contact get the all-zero law result (history
-// reset) at once, and a second pass evaluates the law of the pairs in contact, in order,
-// reading the rest of the partner's record and the history only for them.  A pair out of
-// contact contributes +0.0 to every sum; the running sums start at +0.0 and so are never
-// -0.0, for which x + 0.0 == x and x - 0.0 == x bit for bit: leaving those terms out keeps
-// the reference order's result.
+// Contact tests first: the partners' positions and radii come from the compact copy cin
+// (four slots per cache line), several candidates at a time (independent loads in flight
+// together), through the law's own contact test (law_touch); pairs out of contact get the
+// all-zero law result (history reset) at once.  The laws of the pairs in contact then run
+// in order, reading the rest of the partner's record and the history only for them.  A pair
+// out of contact contributes +0.0 to every sum; the running sums start at +0.0 and so are
+// never -0.0, for which x + 0.0 == x and x - 0.0 == x bit for bit: leaving those terms out
+// keeps the reference order's result.
+//
+// A lane with at most kFast candidates on each side tests both sides in one pass (own and
+// transposed loads in flight together) and runs one law loop over its contacts, own pairs
+// first: a wave then waits for its longest lane's contact count, not for the longest own
+// count plus the longest transposed count.  Lanes with more candidates go through chunks of
+// kChunk per side, own side first.
 constexpr unsigned kChunk = 32;
+constexpr unsigned kFast = 64;
 constexpr unsigned kLook = 4;
 // The block's own records are staged through LDS: read from rin and written to rout in
 // 16-byte chunks by consecutive lanes (1 KB of contiguous memory per wave instruction instead
@@ -741,13 +768,31 @@ __device__ __forceinline__ void zs_pair(unsigned char *zs, Hist *hout, unsigned 
     if (!lazy || nz != z) zs[p] = (unsigned char)nz;
 }
 
+// Contact-test outcome of own pair p (index k in the block's own-pair range): recorded in the
+// block's bitmap (cooperative zeroing later) or applied to zs / the history at once.
+__device__ __forceinline__ void own_outcome(bool coop, unsigned *cbits, unsigned k,
+                                            unsigned char *zs, Hist *hout, unsigned p, bool con,
+                                            unsigned zmode)
+{
+    if (coop) {
+        if (con) atomicOr(&cbits[k >> 5], 1u << (k & 31u));
+    } else {
+        zs_pair(zs, hout, p, con, zmode);
+    }
+}
+
+__device__ __forceinline__ d3 sel3(bool c, d3 u, d3 v)
+{
+    return d3{c ? u.x : v.x, c ? u.y : v.y, c ? u.z : v.z};
+}
+
 template <bool BATCHED>
-__global__ __launch_bounds__(kBlock) void k_step(
+__global__ __launch_bounds__(kBlock, 4) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
-    const double *wforce, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
+    Hist *whist, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, Hist *tape_hist, const unsigned *gate)
+    PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate)
 {
     if (gated_off(gate)) return;
     extern __shared__ double lrec[];
@@ -756,6 +801,16 @@ __global__ __launch_bounds__(kBlock) void k_step(
     const unsigned bd = blockDim.x, t = threadIdx.x;
     const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * bd;
     const unsigned nb = s0 < n ? min(bd, n - s0) : 0u;
+    const unsigned a = s0 + t;
+    const bool live = t < nb;
+    // the lane's list offsets and wall flag: independent of the staged records, so their
+    // loads overlap the staging
+    unsigned p0 = 0, p1 = 0, q0 = 0, q1 = 0, wf = 0;
+    if (live) {
+        p0 = off[a]; p1 = off[a + 1];
+        q0 = toff[a]; q1 = toff[a + 1];
+        wf = wflag[a];
+    }
     // the block's own pairs [pb0, pb1)
     const unsigned pb1 = nb ? min(off[s0 + nb], np) : 0u;
     const unsigned pb0 = nb ? min(off[s0], pb1) : 0u;
@@ -769,21 +824,23 @@ __global__ __launch_bounds__(kBlock) void k_step(
         }
     }
     __syncthreads();
-    const unsigned a = s0 + t;
     double *RA = lrec + t * kRow;
-    bool live = t < nb;
     unsigned flag = 0;
     if (live) {
+        unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
+        const InstDev &IA = BATCHED ? inst[inst_a] : one;
+        // wall contact first (its law needs many registers; few are live here)
+        int hits = 0;
+        d3 fw = d3{0.0, 0.0, 0.0}, tw = d3{0.0, 0.0, 0.0};
+        if (wf) hits = wall_slot<BATCHED>(RA, IA, whist, a, tape_whist, fw, tw);
         double2 a0 = ld2(RA), a1 = ld2(RA + 2);
         const d3 xa = d3{a0.x, a0.y, a1.x};
         const double ra = a1.y;
-        unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
         double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
         unsigned c = 0;
-        unsigned p0 = off[a], p1 = off[a + 1];
         if (p1 > np) p1 = np;
-        unsigned q0 = toff[a], q1 = toff[a + 1];
         if (q1 > np) q1 = np;
+        const unsigned no = p1 > p0 ? p1 - p0 : 0u, nt = q1 > q0 ? q1 - q0 : 0u;
         if (tape_hist)   // the incoming histories of the own pairs, for the adjoint
             for (unsigned p = p0; p < p1; p++) {
                 const double *H = reinterpret_cast<const double *>(hin + p);
@@ -792,90 +849,168 @@ __global__ __launch_bounds__(kBlock) void k_step(
                 st2(T, h0.x, h0.y);
                 st2(T + 2, h1.x, h1.y);
             }
-        for (unsigned base = p0; base < p1; base += kChunk) {
-            const unsigned end = min(base + kChunk, p1);
-            unsigned mask = 0;
-            for (unsigned p = base; p < end; p += kLook) {
-                unsigned b[kLook];
-                double2 c0[kLook], c1[kLook];
+        if (no <= kFast && nt <= kFast) {
+            // both sides' contact tests in one pass
+            unsigned long long mo = 0, mt = 0;
+            const unsigned m = no > nt ? no : nt;
+            for (unsigned i = 0; i < m; i += kLook) {
+                unsigned b[kLook], f[kLook];
+                bool vt[kLook];
+                double2 c0[kLook], c1[kLook], d0[kLook], d1[kLook];
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
-#pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    const double *B = reinterpret_cast<const double *>(cin + b[i]);
-                    c0[i] = ld2(B);
-                    c1[i] = ld2(B + 2);
+                for (unsigned k = 0; k < kLook; k++) {
+                    b[k] = i + k < no ? ab[p0 + i + k].y : a;
+                    unsigned q = i + k < nt ? jt[q0 + i + k] : kEmpty;
+                    vt[k] = q < np;
+                    f[k] = i + k < nt ? jf[q0 + i + k] : a;
                 }
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    if (p + i >= end) break;
-                    const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
-                    if (con) mask |= 1u << (p + i - base);
-                    if (coop) {
-                        if (con) atomicOr(&cbits[(p + i - pb0) >> 5], 1u << ((p + i - pb0) & 31u));
-                    } else {
-                        zs_pair(zs, hout, p + i, con, zmode);
+                for (unsigned k = 0; k < kLook; k++) {
+                    const double *B = reinterpret_cast<const double *>(cin + b[k]);
+                    c0[k] = ld2(B);
+                    c1[k] = ld2(B + 2);
+                    const double *F = reinterpret_cast<const double *>(cin + (vt[k] ? f[k] : a));
+                    d0[k] = ld2(F);
+                    d1[k] = ld2(F + 2);
+                }
+#pragma unroll
+                for (unsigned k = 0; k < kLook; k++) {
+                    if (i + k < no) {
+                        const bool con = law_touch(xa, d3{c0[k].x, c0[k].y, c1[k].x}, ra, c1[k].y);
+                        if (con) mo |= 1ull << (i + k);
+                        own_outcome(coop, cbits, p0 + i + k - pb0, zs, hout, p0 + i + k, con, zmode);
+                    }
+                    if (vt[k] && law_touch(d3{d0[k].x, d0[k].y, d1[k].x}, xa, d1[k].y, ra))
+                        mt |= 1ull << (i + k);
+                }
+            }
+            // the laws of the pairs in contact: own pairs in order, then transposed ones
+            while (mo | mt) {
+                const bool own = mo != 0ull;
+                unsigned idx;
+                if (own) {
+                    idx = (unsigned)__builtin_ctzll(mo);
+                    mo &= mo - 1ull;
+                } else {
+                    idx = (unsigned)__builtin_ctzll(mt);
+                    mt &= mt - 1ull;
+                }
+                const unsigned pp = own ? p0 + idx : jt[q0 + idx];
+                const unsigned pb = own ? ab[pp].y : jf[q0 + idx];
+                const double *B = reinterpret_cast<const double *>(rin + pb);
+                const double *H = reinterpret_cast<const double *>(hin + pp);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6),
+                        b4 = ld2(B + 8), b5 = ld2(B + 10);
+                double2 s1 = ld2(RA + 2), s2 = ld2(RA + 4), s3 = ld2(RA + 6), s4 = ld2(RA + 8),
+                        s5 = ld2(RA + 10);
+                const d3 xs = xa, vs = d3{s2.x, s2.y, s3.x}, ws = d3{s4.x, s4.y, s5.x};
+                const d3 xp = d3{b0.x, b0.y, b1.x}, vp = d3{b2.x, b2.y, b3.x},
+                         wp = d3{b4.x, b4.y, b5.x};
+                const double rs = s1.y, ms = s3.y, rp = b1.y, mp = b3.y;
+                LawParams P = BATCHED ? IA.pp : one.pp;
+                const double tdt = BATCHED ? IA.dt : one.dt;
+                d3 fa, ta, tb, ho;
+                double fo;
+                // first member = lower particle index: this lane on its own pairs
+                bool con = law_pair(sel3(own, xs, xp), sel3(own, xp, xs), sel3(own, vs, vp),
+                                    sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
+                                    own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
+                                    d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo);
+                if (own) {
+                    double *HW = reinterpret_cast<double *>(hout + pp);
+                    st2(HW, ho.x, ho.y);
+                    st2(HW + 2, ho.z, fo);
+                    fx += fa.x; fy += fa.y; fz += fa.z;
+                    tx += ta.x; ty += ta.y; tz += ta.z;
+                } else {
+                    fx -= fa.x; fy -= fa.y; fz -= fa.z;
+                    tx += tb.x; ty += tb.y; tz += tb.z;
+                }
+                c += con ? 1u : 0u;
+            }
+        } else {
+            for (unsigned base = p0; base < p1; base += kChunk) {
+                const unsigned end = min(base + kChunk, p1);
+                unsigned mask = 0;
+                for (unsigned p = base; p < end; p += kLook) {
+                    unsigned b[kLook];
+                    double2 c0[kLook], c1[kLook];
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++) {
+                        const double *B = reinterpret_cast<const double *>(cin + b[i]);
+                        c0[i] = ld2(B);
+                        c1[i] = ld2(B + 2);
+                    }
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++) {
+                        if (p + i >= end) break;
+                        const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
+                        if (con) mask |= 1u << (p + i - base);
+                        own_outcome(coop, cbits, p + i - pb0, zs, hout, p + i, con, zmode);
                     }
                 }
+                while (mask) {
+                    const unsigned p = base + (unsigned)__builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
+                    const double *H = reinterpret_cast<const double *>(hin + p);
+                    double2 h0 = ld2(H), h1 = ld2(H + 2);
+                    d3 fa, ta, tb, ho;
+                    double fo;
+                    bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
+                                                          one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                    double *HW = reinterpret_cast<double *>(hout + p);
+                    st2(HW, ho.x, ho.y);
+                    st2(HW + 2, ho.z, fo);
+                    fx += fa.x; fy += fa.y; fz += fa.z;
+                    tx += ta.x; ty += ta.y; tz += ta.z;
+                    c += con ? 1u : 0u;
+                }
             }
-            while (mask) {
-                const unsigned p = base + (unsigned)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
-                const double *H = reinterpret_cast<const double *>(hin + p);
-                double2 h0 = ld2(H), h1 = ld2(H + 2);
-                d3 fa, ta, tb, ho;
-                double fo;
-                bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst, one.pp,
-                                                      one.dt, inst_a, fa, ta, tb, ho, fo);
-                double *HW = reinterpret_cast<double *>(hout + p);
-                st2(HW, ho.x, ho.y);
-                st2(HW + 2, ho.z, fo);
-                fx += fa.x; fy += fa.y; fz += fa.z;
-                tx += ta.x; ty += ta.y; tz += ta.z;
-                c += con ? 1u : 0u;
+            for (unsigned base = q0; base < q1; base += kChunk) {
+                const unsigned end = min(base + kChunk, q1);
+                unsigned mask = 0;
+                for (unsigned k = base; k < end; k += kLook) {
+                    unsigned q[kLook], f[kLook];
+                    double2 c0[kLook], c1[kLook];
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++) {
+                        bool in = k + i < end;
+                        q[i] = in ? jt[k + i] : kEmpty;
+                        f[i] = in ? jf[k + i] : a;
+                    }
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++) {
+                        const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
+                        c0[i] = ld2(F);
+                        c1[i] = ld2(F + 2);
+                    }
+#pragma unroll
+                    for (unsigned i = 0; i < kLook; i++)
+                        if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
+                            mask |= 1u << (k + i - base);
+                }
+                while (mask) {
+                    const unsigned k = base + (unsigned)__builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    const unsigned q = jt[k];
+                    const double *F = reinterpret_cast<const double *>(rin + jf[k]);
+                    const double *H = reinterpret_cast<const double *>(hin + q);
+                    double2 h0 = ld2(H), h1 = ld2(H + 2);
+                    d3 fa, ta, tb, ho;
+                    double fo;
+                    bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
+                                                          one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                    fx -= fa.x; fy -= fa.y; fz -= fa.z;
+                    tx += tb.x; ty += tb.y; tz += tb.z;
+                    c += con ? 1u : 0u;
+                }
             }
         }
-        for (unsigned base = q0; base < q1; base += kChunk) {
-            const unsigned end = min(base + kChunk, q1);
-            unsigned mask = 0;
-            for (unsigned k = base; k < end; k += kLook) {
-                unsigned q[kLook], f[kLook];
-                double2 c0[kLook], c1[kLook];
-#pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    bool in = k + i < end;
-                    q[i] = in ? jt[k + i] : kEmpty;
-                    f[i] = in ? jf[k + i] : a;
-                }
-#pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
-                    c0[i] = ld2(F);
-                    c1[i] = ld2(F + 2);
-                }
-#pragma unroll
-                for (unsigned i = 0; i < kLook; i++)
-                    if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
-                        mask |= 1u << (k + i - base);
-            }
-            while (mask) {
-                const unsigned k = base + (unsigned)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const unsigned q = jt[k];
-                const double *F = reinterpret_cast<const double *>(rin + jf[k]);
-                const double *H = reinterpret_cast<const double *>(hin + q);
-                double2 h0 = ld2(H), h1 = ld2(H + 2);
-                d3 fa, ta, tb, ho;
-                double fo;
-                bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst, one.pp,
-                                                      one.dt, inst_a, fa, ta, tb, ho, fo);
-                fx -= fa.x; fy -= fa.y; fz -= fa.z;
-                tx += tb.x; ty += tb.y; tz += tb.z;
-                c += con ? 1u : 0u;
-            }
-        }
-        flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, wflag, wforce,
+        flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, hits != 0, fw, tw,
                                         inst, one, contacti, fdbg, tape_rec, cout);
     }
     __syncthreads();

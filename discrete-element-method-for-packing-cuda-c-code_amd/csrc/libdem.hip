@@ -489,14 +489,16 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
                                dinst, one.pp, one.dt, c->tape_hist, gate);
     }
-    if (c->batched)
-        hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
-                           c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
-                           dinst, one, c->wforce, c->tape_whist, gate);
-    else
-        hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
-                           c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
-                           dinst, one, c->wforce, c->tape_whist, gate);
+    if (!c->fused) {   // the fused step evaluates the wall contacts itself
+        if (c->batched)
+            hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+                               c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
+                               dinst, one, c->wforce, c->tape_whist, gate);
+        else
+            hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+                               c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
+                               dinst, one, c->wforce, c->tape_whist, gate);
+    }
     if (mid) (void)hipEventRecord(mid, st);
     if (c->fused) {
         int nr = 1 - c->rc, nh = 1 - c->hc;
@@ -511,17 +513,17 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, gate);
+                               c->tape_rec, c->tape_hist, c->tape_whist, gate);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               (const double *)c->wforce, (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, gate);
+                               c->tape_rec, c->tape_hist, c->tape_whist, gate);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
