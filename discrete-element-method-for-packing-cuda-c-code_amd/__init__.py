@@ -48,6 +48,7 @@ OPT_ADJ_ROLL_REG = 8
 OPT_FUSED_STEP = 9
 OPT_TIME_STEPS = 10
 OPT_PIPELINE = 11
+OPT_FUSED_FAST = 12
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0

@@ -65,7 +65,8 @@ struct DevCtl {
     unsigned overflow;       // pair capacity exceeded at the last rebuild
     unsigned nwall;          // wall-candidate count of the last rebuild
     unsigned skip;           // copy of need_rebuild made by k_publish: the gate of queued steps
-    unsigned pad[2];
+    unsigned zero;           // always 0: k_step's gate when the step is not queued
+    unsigned pad;
 };
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
@@ -751,13 +752,14 @@ __device__ __forceinline__ void zero_hist(Hist *h, unsigned p)
     st2(HW + 2, 0.0, 0.0);
 }
 
-// zs update of pair p for this step (con: in contact); writes the zero history if needed.
-__device__ __forceinline__ void zs_pair(unsigned char *zs, Hist *hout, unsigned p, bool con,
-                                        unsigned zmode)
+// zs update of pair p for this step (con: in contact), given its current zs byte zc; writes
+// the zero history if needed.
+__device__ __forceinline__ void zs_pair_z(unsigned char *zs, Hist *hout, unsigned p, bool con,
+                                          unsigned zmode, unsigned zc)
 {
     const unsigned wbit = zmode & 3u;
     const bool lazy = (zmode & 4u) != 0u;
-    const unsigned z = lazy ? zs[p] : 0u;
+    const unsigned z = lazy ? zc : 0u;
     unsigned nz;
     if (con) {
         nz = z & ~wbit;
@@ -766,6 +768,12 @@ __device__ __forceinline__ void zs_pair(unsigned char *zs, Hist *hout, unsigned 
         nz = z | wbit;
     }
     if (!lazy || nz != z) zs[p] = (unsigned char)nz;
+}
+
+__device__ __forceinline__ void zs_pair(unsigned char *zs, Hist *hout, unsigned p, bool con,
+                                        unsigned zmode)
+{
+    zs_pair_z(zs, hout, p, con, zmode, (zmode & 4u) ? zs[p] : 0u);
 }
 
 // Contact-test outcome of own pair p (index k in the block's own-pair range): recorded in the
@@ -781,6 +789,54 @@ __device__ __forceinline__ void own_outcome(bool coop, unsigned *cbits, unsigned
     }
 }
 
+// Pops the next contact of a lane (own pairs first, in order, then transposed ones): its
+// side, pair index and partner slot.
+__device__ __forceinline__ void next_contact(unsigned long long &mo, unsigned long long &mt,
+                                             unsigned p0, unsigned q0, const uint2 *ab,
+                                             const unsigned *jt, const unsigned *jf, bool &own,
+                                             unsigned &pp, unsigned &pb)
+{
+    own = mo != 0ull;
+    if (own) {
+        pp = p0 + (unsigned)__builtin_ctzll(mo);
+        mo &= mo - 1ull;
+        pb = ab[pp].y;
+    } else {
+        const unsigned k = q0 + (unsigned)__builtin_ctzll(mt);
+        mt &= mt - 1ull;
+        pp = jt[k];
+        pb = jf[k];
+    }
+}
+
+// Position of the k-th set bit (k counted from 0) of m, which has more than k set bits.
+__device__ __forceinline__ unsigned nth_bit(unsigned long long m, unsigned k)
+{
+    unsigned pos = 0;
+#pragma unroll
+    for (unsigned w = 32; w; w >>= 1) {
+        const unsigned cnt = (unsigned)__popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (k >= cnt) {
+            k -= cnt;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
+// Cross-lane reads of 64-bit values (two 32-bit lane permutes).
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long v, unsigned src)
+{
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)v, (int)src, 64);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)(v >> 32), (int)src, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ double shfl_d(double v, int src)
+{
+    return __longlong_as_double((long long)shfl64((unsigned long long)__double_as_longlong(v),
+                                                  (unsigned)src));
+}
+
 __device__ __forceinline__ d3 sel3(bool c, d3 u, d3 v)
 {
     return d3{c ? u.x : v.x, c ? u.y : v.y, c ? u.z : v.z};
@@ -792,52 +848,70 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
     Hist *whist, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate)
+    PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate, unsigned fast_max)
 {
-    if (gated_off(gate)) return;
     extern __shared__ double lrec[];
     __shared__ unsigned cbits[kBits / 32];
+    __shared__ unsigned zsl[kBlock];
+    // Every load of the prologue is in bounds whether or not the step is gated off and
+    // whatever the lane, so all of them are issued before any of their values is used: the
+    // gate, the list offsets, the zs words and the block's records are in flight together.
+    const unsigned gv = *gate;   // never null (a zero word for steps that are not queued)
     const unsigned np = ctl->npairs;
     const unsigned bd = blockDim.x, t = threadIdx.x;
     const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * bd;
     const unsigned nb = s0 < n ? min(bd, n - s0) : 0u;
     const unsigned a = s0 + t;
     const bool live = t < nb;
-    // the lane's list offsets and wall flag: independent of the staged records, so their
-    // loads overlap the staging
-    unsigned p0 = 0, p1 = 0, q0 = 0, q1 = 0, wf = 0;
-    if (live) {
-        p0 = off[a]; p1 = off[a + 1];
-        q0 = toff[a]; q1 = toff[a + 1];
-        wf = wflag[a];
-    }
+    // the lane's list offsets and wall flag (off and toff hold n + 1 entries)
+    const unsigned ai = live ? a : n;
+    unsigned p0 = off[ai], p1 = off[ai + (live ? 1u : 0u)];
+    unsigned q0 = toff[ai], q1 = toff[ai + (live ? 1u : 0u)];
+    const unsigned wfa = wflag[live ? a : 0u];   // n >= 1 when the kernel runs
+    const unsigned wf = live ? wfa : 0u;
     // the block's own pairs [pb0, pb1)
-    const unsigned pb1 = nb ? min(off[s0 + nb], np) : 0u;
-    const unsigned pb0 = nb ? min(off[s0], pb1) : 0u;
-    const bool coop = pb1 - pb0 <= kBits;
-    for (unsigned k = t; k < kBits / 32; k += bd) cbits[k] = 0u;
-    {
-        const double *G = reinterpret_cast<const double *>(rin + s0);
-        for (unsigned k = t; k < 8u * nb; k += bd) {
-            double2 v = ld2(G + 2 * k);
-            st2(lrec + (k >> 3) * kRow + 2 * (k & 7u), v.x, v.y);
-        }
+    const unsigned ob0 = off[min(s0, n)], ob1 = off[min(s0 + nb, n)];
+    // the block's records: up to 8 16-byte chunks per lane, all loads in flight at once
+    const double *G = reinterpret_cast<const double *>(rin + s0);
+    double2 v[8];
+#pragma unroll
+    for (unsigned i = 0; i < 8; i++) {
+        const unsigned k = t + i * bd;
+        if (k < 8u * nb) v[i] = ld2(G + 2 * k);
     }
+    const unsigned pb1 = nb ? min(ob1, np) : 0u;
+    const unsigned pb0 = nb ? min(ob0, pb1) : 0u;
+    const bool coop = pb1 - pb0 <= kBits;
+    // zs bytes of the block's own pairs, one 4-byte word per lane from word pb0 / 4 (the
+    // allocation is padded to whole words): the cooperative pass at the end reads them from
+    // LDS; pairs past the words covered read zs directly
+    const unsigned zw0 = pb0 >> 2;
+    const bool zin = coop && zw0 + t < ((pb1 + 3u) >> 2);
+    unsigned zword = 0;
+    if (zin) zword = reinterpret_cast<const unsigned *>(zs)[zw0 + t];
+    for (unsigned k = t; k < kBits / 32; k += bd) cbits[k] = 0u;
+#pragma unroll
+    for (unsigned i = 0; i < 8; i++) {
+        const unsigned k = t + i * bd;
+        if (k < 8u * nb) st2(lrec + (k >> 3) * kRow + 2 * (k & 7u), v[i].x, v[i].y);
+    }
+    if (zin) zsl[t] = zword;
+    if (gv != 0u) return;   // a gated-off step: uniform over the grid, no lane reaches the barrier
     __syncthreads();
     double *RA = lrec + t * kRow;
     unsigned flag = 0;
+    // per-lane state of the pair work (zero on lanes that are not live)
+    int hits = 0;
+    d3 fw = d3{0.0, 0.0, 0.0}, tw = d3{0.0, 0.0, 0.0};
+    double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+    unsigned c = 0;
+    unsigned long long mo = 0ull, mt = 0ull;   // contacts found by the one-pass tests
+    bool fast = false;
     if (live) {
-        unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
+        const unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
         const InstDev &IA = BATCHED ? inst[inst_a] : one;
         // wall contact first (its law needs many registers; few are live here)
-        int hits = 0;
-        d3 fw = d3{0.0, 0.0, 0.0}, tw = d3{0.0, 0.0, 0.0};
         if (wf) hits = wall_slot<BATCHED>(RA, IA, whist, a, tape_whist, fw, tw);
-        double2 a0 = ld2(RA), a1 = ld2(RA + 2);
-        const d3 xa = d3{a0.x, a0.y, a1.x};
-        const double ra = a1.y;
-        double fx = 0.0, fy = 0.0, fz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
-        unsigned c = 0;
         if (p1 > np) p1 = np;
         if (q1 > np) q1 = np;
         const unsigned no = p1 > p0 ? p1 - p0 : 0u, nt = q1 > q0 ? q1 - q0 : 0u;
@@ -849,9 +923,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 st2(T, h0.x, h0.y);
                 st2(T + 2, h1.x, h1.y);
             }
-        if (no <= kFast && nt <= kFast) {
+        fast = no <= fast_max && nt <= fast_max;
+        if (fast) {
             // both sides' contact tests in one pass
-            unsigned long long mo = 0, mt = 0;
+            double2 a0 = ld2(RA), a1 = ld2(RA + 2);
+            const d3 xa = d3{a0.x, a0.y, a1.x};
+            const double ra = a1.y;
             const unsigned m = no > nt ? no : nt;
             for (unsigned i = 0; i < m; i += kLook) {
                 unsigned b[kLook], f[kLook];
@@ -884,146 +961,224 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                         mt |= 1ull << (i + k);
                 }
             }
-            // the laws of the pairs in contact: own pairs in order, then transposed ones
-            while (mo | mt) {
-                const bool own = mo != 0ull;
-                unsigned idx;
+        }
+    }
+    // The laws of the one-pass lanes' contacts, balanced over the wave: contact g of the
+    // wave (lanes in order, each lane's own pairs in order, then its transposed ones) is
+    // evaluated in round g / 64 by lane g % 64, whatever lane owns it; each owner then adds
+    // its results in contact order, so its sums are the reference order's bit for bit.  A
+    // wave thus runs ceil(total / 64) laws instead of as many as its busiest lane has
+    // contacts.  Every lane takes part (uniform control flow for the cross-lane reads).
+    {
+        const unsigned lane = __lane_id();
+        const unsigned z = (unsigned)(__popcll(mo) + __popcll(mt));
+        unsigned ex = z;   // inclusive scan over the wave, then exclusive
+#pragma unroll
+        for (unsigned d = 1; d < 64; d <<= 1) {
+            const unsigned y = (unsigned)__shfl_up((int)ex, d, 64);
+            if (lane >= d) ex += y;
+        }
+        const unsigned total = (unsigned)__builtin_amdgcn_readlane((int)ex, 63);
+        ex -= z;
+        const unsigned wbase = t - lane;   // the wave's first lane in the block
+        for (unsigned r0 = 0; r0 < total; r0 += 64u) {
+            const unsigned g = r0 + lane;
+            // owner: the last lane whose first contact index is <= g
+            unsigned L = 0;
+#pragma unroll
+            for (unsigned st = 32; st; st >>= 1) {
+                const unsigned e = (unsigned)__shfl((int)ex, (int)(L + st), 64);
+                if (e <= g) L += st;
+            }
+            const unsigned exL = (unsigned)__shfl((int)ex, (int)L, 64);
+            const unsigned long long moL = shfl64(mo, L), mtL = shfl64(mt, L);
+            const unsigned p0L = (unsigned)__shfl((int)p0, (int)L, 64);
+            const unsigned q0L = (unsigned)__shfl((int)q0, (int)L, 64);
+            const bool work = g < total;
+            double rfx = 0.0, rfy = 0.0, rfz = 0.0, rtx = 0.0, rty = 0.0, rtz = 0.0;
+            bool rc = false;
+            if (work) {
+                const unsigned j = g - exL;
+                const unsigned co = (unsigned)__popcll(moL);
+                const bool own = j < co;
+                const unsigned idx = nth_bit(own ? moL : mtL, own ? j : j - co);
+                unsigned pp, pb;
                 if (own) {
-                    idx = (unsigned)__builtin_ctzll(mo);
-                    mo &= mo - 1ull;
+                    pp = p0L + idx;
+                    pb = ab[pp].y;
                 } else {
-                    idx = (unsigned)__builtin_ctzll(mt);
-                    mt &= mt - 1ull;
+                    pp = jt[q0L + idx];
+                    pb = jf[q0L + idx];
                 }
-                const unsigned pp = own ? p0 + idx : jt[q0 + idx];
-                const unsigned pb = own ? ab[pp].y : jf[q0 + idx];
+                const double *S = lrec + (wbase + L) * kRow;   // the owner's record
                 const double *B = reinterpret_cast<const double *>(rin + pb);
                 const double *H = reinterpret_cast<const double *>(hin + pp);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
                 double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6),
                         b4 = ld2(B + 8), b5 = ld2(B + 10);
-                double2 s1 = ld2(RA + 2), s2 = ld2(RA + 4), s3 = ld2(RA + 6), s4 = ld2(RA + 8),
-                        s5 = ld2(RA + 10);
-                const d3 xs = xa, vs = d3{s2.x, s2.y, s3.x}, ws = d3{s4.x, s4.y, s5.x};
+                double2 s0v = ld2(S), s1 = ld2(S + 2), s2 = ld2(S + 4), s3 = ld2(S + 6),
+                        s4 = ld2(S + 8), s5 = ld2(S + 10);
+                const d3 xs = d3{s0v.x, s0v.y, s1.x}, vs = d3{s2.x, s2.y, s3.x},
+                         ws = d3{s4.x, s4.y, s5.x};
                 const d3 xp = d3{b0.x, b0.y, b1.x}, vp = d3{b2.x, b2.y, b3.x},
                          wp = d3{b4.x, b4.y, b5.x};
                 const double rs = s1.y, ms = s3.y, rp = b1.y, mp = b3.y;
-                LawParams P = BATCHED ? IA.pp : one.pp;
-                const double tdt = BATCHED ? IA.dt : one.dt;
+                LawParams P = one.pp;
+                double tdt = one.dt;
+                if (BATCHED) {
+                    const InstDev &IO = inst[reinterpret_cast<const uint2 *>(S + 15)->x];
+                    P = IO.pp;
+                    tdt = IO.dt;
+                }
                 d3 fa, ta, tb, ho;
                 double fo;
-                // first member = lower particle index: this lane on its own pairs
-                bool con = law_pair(sel3(own, xs, xp), sel3(own, xp, xs), sel3(own, vs, vp),
-                                    sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
-                                    own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
-                                    d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo);
+                // first member = lower particle index: the owner on its own pairs
+                rc = law_pair(sel3(own, xs, xp), sel3(own, xp, xs), sel3(own, vs, vp),
+                              sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
+                              own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
+                              d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo);
                 if (own) {
                     double *HW = reinterpret_cast<double *>(hout + pp);
                     st2(HW, ho.x, ho.y);
                     st2(HW + 2, ho.z, fo);
-                    fx += fa.x; fy += fa.y; fz += fa.z;
-                    tx += ta.x; ty += ta.y; tz += ta.z;
-                } else {
-                    fx -= fa.x; fy -= fa.y; fz -= fa.z;
-                    tx += tb.x; ty += tb.y; tz += tb.z;
                 }
+                // the owner's terms: +F and the first member's torque on its own pairs,
+                // -F (x - y == x + (-y) bit for bit) and the second member's torque otherwise
+                rfx = own ? fa.x : -fa.x;
+                rfy = own ? fa.y : -fa.y;
+                rfz = own ? fa.z : -fa.z;
+                rtx = own ? ta.x : tb.x;
+                rty = own ? ta.y : tb.y;
+                rtz = own ? ta.z : tb.z;
+            }
+            // each lane collects its contacts of this round, in order
+            const unsigned lo = ex > r0 ? ex : r0;
+            const unsigned hi = min(ex + z, r0 + 64u);
+            for (unsigned k = lo; __any(k < hi); k++) {
+                const bool mine = k < hi;
+                const int src = mine ? (int)(k - r0) : (int)lane;
+                const double vfx = shfl_d(rfx, src), vfy = shfl_d(rfy, src), vfz = shfl_d(rfz, src);
+                const double vtx = shfl_d(rtx, src), vty = shfl_d(rty, src), vtz = shfl_d(rtz, src);
+                const int vc = __shfl((int)rc, src, 64);
+                if (mine) {
+                    fx += vfx; fy += vfy; fz += vfz;
+                    tx += vtx; ty += vty; tz += vtz;
+                    c += vc ? 1u : 0u;
+                }
+            }
+        }
+    }
+    if (live && !fast) {
+        const unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
+        double2 a0 = ld2(RA), a1 = ld2(RA + 2);
+        const d3 xa = d3{a0.x, a0.y, a1.x};
+        const double ra = a1.y;
+        for (unsigned base = p0; base < p1; base += kChunk) {
+            const unsigned end = min(base + kChunk, p1);
+            unsigned mask = 0;
+            for (unsigned p = base; p < end; p += kLook) {
+                unsigned b[kLook];
+                double2 c0[kLook], c1[kLook];
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    const double *B = reinterpret_cast<const double *>(cin + b[i]);
+                    c0[i] = ld2(B);
+                    c1[i] = ld2(B + 2);
+                }
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    if (p + i >= end) break;
+                    const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
+                    if (con) mask |= 1u << (p + i - base);
+                    own_outcome(coop, cbits, p + i - pb0, zs, hout, p + i, con, zmode);
+                }
+            }
+            while (mask) {
+                const unsigned p = base + (unsigned)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
+                const double *H = reinterpret_cast<const double *>(hin + p);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                d3 fa, ta, tb, ho;
+                double fo;
+                bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
+                                                      one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                double *HW = reinterpret_cast<double *>(hout + p);
+                st2(HW, ho.x, ho.y);
+                st2(HW + 2, ho.z, fo);
+                fx += fa.x; fy += fa.y; fz += fa.z;
+                tx += ta.x; ty += ta.y; tz += ta.z;
                 c += con ? 1u : 0u;
             }
-        } else {
-            for (unsigned base = p0; base < p1; base += kChunk) {
-                const unsigned end = min(base + kChunk, p1);
-                unsigned mask = 0;
-                for (unsigned p = base; p < end; p += kLook) {
-                    unsigned b[kLook];
-                    double2 c0[kLook], c1[kLook];
+        }
+        for (unsigned base = q0; base < q1; base += kChunk) {
+            const unsigned end = min(base + kChunk, q1);
+            unsigned mask = 0;
+            for (unsigned k = base; k < end; k += kLook) {
+                unsigned q[kLook], f[kLook];
+                double2 c0[kLook], c1[kLook];
 #pragma unroll
-                    for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
-#pragma unroll
-                    for (unsigned i = 0; i < kLook; i++) {
-                        const double *B = reinterpret_cast<const double *>(cin + b[i]);
-                        c0[i] = ld2(B);
-                        c1[i] = ld2(B + 2);
-                    }
-#pragma unroll
-                    for (unsigned i = 0; i < kLook; i++) {
-                        if (p + i >= end) break;
-                        const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
-                        if (con) mask |= 1u << (p + i - base);
-                        own_outcome(coop, cbits, p + i - pb0, zs, hout, p + i, con, zmode);
-                    }
+                for (unsigned i = 0; i < kLook; i++) {
+                    bool in = k + i < end;
+                    q[i] = in ? jt[k + i] : kEmpty;
+                    f[i] = in ? jf[k + i] : a;
                 }
-                while (mask) {
-                    const unsigned p = base + (unsigned)__builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
-                    const double *H = reinterpret_cast<const double *>(hin + p);
-                    double2 h0 = ld2(H), h1 = ld2(H + 2);
-                    d3 fa, ta, tb, ho;
-                    double fo;
-                    bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
-                                                          one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
-                    double *HW = reinterpret_cast<double *>(hout + p);
-                    st2(HW, ho.x, ho.y);
-                    st2(HW + 2, ho.z, fo);
-                    fx += fa.x; fy += fa.y; fz += fa.z;
-                    tx += ta.x; ty += ta.y; tz += ta.z;
-                    c += con ? 1u : 0u;
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++) {
+                    const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
+                    c0[i] = ld2(F);
+                    c1[i] = ld2(F + 2);
                 }
+#pragma unroll
+                for (unsigned i = 0; i < kLook; i++)
+                    if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
+                        mask |= 1u << (k + i - base);
             }
-            for (unsigned base = q0; base < q1; base += kChunk) {
-                const unsigned end = min(base + kChunk, q1);
-                unsigned mask = 0;
-                for (unsigned k = base; k < end; k += kLook) {
-                    unsigned q[kLook], f[kLook];
-                    double2 c0[kLook], c1[kLook];
-#pragma unroll
-                    for (unsigned i = 0; i < kLook; i++) {
-                        bool in = k + i < end;
-                        q[i] = in ? jt[k + i] : kEmpty;
-                        f[i] = in ? jf[k + i] : a;
-                    }
-#pragma unroll
-                    for (unsigned i = 0; i < kLook; i++) {
-                        const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
-                        c0[i] = ld2(F);
-                        c1[i] = ld2(F + 2);
-                    }
-#pragma unroll
-                    for (unsigned i = 0; i < kLook; i++)
-                        if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
-                            mask |= 1u << (k + i - base);
-                }
-                while (mask) {
-                    const unsigned k = base + (unsigned)__builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    const unsigned q = jt[k];
-                    const double *F = reinterpret_cast<const double *>(rin + jf[k]);
-                    const double *H = reinterpret_cast<const double *>(hin + q);
-                    double2 h0 = ld2(H), h1 = ld2(H + 2);
-                    d3 fa, ta, tb, ho;
-                    double fo;
-                    bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
-                                                          one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
-                    fx -= fa.x; fy -= fa.y; fz -= fa.z;
-                    tx += tb.x; ty += tb.y; tz += tb.z;
-                    c += con ? 1u : 0u;
-                }
+            while (mask) {
+                const unsigned k = base + (unsigned)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const unsigned q = jt[k];
+                const double *F = reinterpret_cast<const double *>(rin + jf[k]);
+                const double *H = reinterpret_cast<const double *>(hin + q);
+                double2 h0 = ld2(H), h1 = ld2(H + 2);
+                d3 fa, ta, tb, ho;
+                double fo;
+                bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
+                                                      one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                fx -= fa.x; fy -= fa.y; fz -= fa.z;
+                tx += tb.x; ty += tb.y; tz += tb.z;
+                c += con ? 1u : 0u;
             }
         }
+    }
+    if (live)
         flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, hits != 0, fw, tw,
                                         inst, one, contacti, fdbg, tape_rec, cout);
-    }
     __syncthreads();
     {
-        double *G = reinterpret_cast<double *>(rout + s0);
-        for (unsigned k = t; k < 8u * nb; k += bd) {
-            double2 v = ld2(lrec + (k >> 3) * kRow + 2 * (k & 7u));
-            st2(G + 2 * k, v.x, v.y);
+        double *GW = reinterpret_cast<double *>(rout + s0);
+        double2 u[8];
+#pragma unroll
+        for (unsigned i = 0; i < 8; i++) {
+            const unsigned k = t + i * bd;
+            u[i] = k < 8u * nb ? ld2(lrec + (k >> 3) * kRow + 2 * (k & 7u)) : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (unsigned i = 0; i < 8; i++) {
+            const unsigned k = t + i * bd;
+            if (k < 8u * nb) st2(GW + 2 * k, u[i].x, u[i].y);
         }
     }
-    if (coop)
-        for (unsigned k = t; k < pb1 - pb0; k += bd)
-            zs_pair(zs, hout, pb0 + k, (cbits[k >> 5] >> (k & 31u)) & 1u, zmode);
+    if (coop) {
+        const unsigned zend = (zw0 + bd) << 2;   // first pair past the words in zsl
+        for (unsigned k = t; k < pb1 - pb0; k += bd) {
+            const unsigned p = pb0 + k;
+            const unsigned z = p < zend ? (zsl[(p >> 2) - zw0] >> (8u * (p & 3u))) & 0xffu : zs[p];
+            zs_pair_z(zs, hout, p, (cbits[k >> 5] >> (k & 31u)) & 1u, zmode, z);
+        }
+    }
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
 }
 

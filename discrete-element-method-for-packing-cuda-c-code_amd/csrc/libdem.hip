@@ -203,6 +203,7 @@ struct dem_ctx {
     int hc = 0;
     bool fused = true;        // step path in use: fused per-particle (k_step) or split
     int fused_mode = 2;       // DEM_OPT_FUSED_STEP: 0 split, 1 fused, 2 auto (timed probes)
+    unsigned fused_fast = kFast;   // DEM_OPT_FUSED_FAST: one-pass limit of the fused step
     int probe_left = 0;       // auto: steps left in the current probe (fused, then split;
                               // the first probe runs the pair twice and keeps the second)
     unsigned reb_since_probe = 0, probe_every = 8;
@@ -298,6 +299,9 @@ int alloc_hist(dem_ctx *c, int side, unsigned cap)
     return DEM_OK;
 }
 
+// zs is read by k_step in 4-byte words: whole words cover every pair.
+size_t zs_bytes(unsigned cap) { return ((size_t)cap + 3) & ~(size_t)3; }
+
 int alloc_out(dem_ctx *c, unsigned cap)
 {
     if (c->out) DEM_HIP(hipFree(c->out));
@@ -312,7 +316,7 @@ int alloc_out(dem_ctx *c, unsigned cap)
     DEM_HIP(hipMalloc(&c->out, (size_t)std::max(cap, 1u) * sizeof(POut)));
     DEM_HIP(hipMalloc(&c->jt, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
     DEM_HIP(hipMalloc(&c->jf, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
-    DEM_HIP(hipMalloc(&c->zs, (size_t)std::max(cap, 1u)));
+    DEM_HIP(hipMalloc(&c->zs, zs_bytes(std::max(cap, 1u))));
     c->out_cap = cap;
     c->buf_gen++;
     return DEM_OK;
@@ -505,6 +509,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         // zero-history bookkeeping: bit nh of zs describes the buffer written now; with
         // valid bookkeeping (lazy) pairs already known to be zero there are not rewritten
         const unsigned zmode = (1u << nh) | (c->zs_ok ? 4u : 0u);
+        const unsigned *kgate = gate ? gate : &c->ctl->zero;
         c->zs_ok = true;
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by k_walls, so they move with the records
@@ -515,7 +520,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, c->tape_whist, gate);
+                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
@@ -523,7 +528,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, c->tape_whist, gate);
+                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -942,6 +947,13 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     }
+    case DEM_OPT_FUSED_FAST: {
+        if (!(value >= 0.0 && value <= (double)kFast) || (double)(unsigned)value != value)
+            return DEM_ERR_INVALID;
+        c->fused_fast = (unsigned)value;
+        drop_graph(c);
+        return DEM_OK;
+    }
     case DEM_OPT_PIPELINE: {
         int d = (int)value;
         if (d < 1 || d > dem_ctx::kQueue || (double)d != value) return DEM_ERR_INVALID;
@@ -1316,7 +1328,7 @@ int dem_step_graph(dem_ctx *c, int nsteps)
         c->jf = nf;
         c->zs = nullptr;
         c->zs_ok = false;
-        DEM_HIP(hipMalloc(&c->zs, (size_t)want));
+        DEM_HIP(hipMalloc(&c->zs, zs_bytes(want)));
         c->out_cap = want;
         c->buf_gen++;
     }

@@ -232,10 +232,10 @@ def test_graph_equals_host_loop(dem, gpu_ok):
 
 @pytest.mark.parametrize("mode", ["trigger", "interval", "graph", "batched", "odd-graph"])
 def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
-    """The fused per-particle step, the pair-kernel + gather path and the auto mode (which
-    switches between them) give the same states, histories, contacts and pair outputs bit for
-    bit, under both rebuild policies, in graphs (even and odd group counts, even and odd K) and
-    in the batched form."""
+    """The fused per-particle step (one-pass and chunked contact tests), the pair-kernel +
+    gather path and the auto mode (which switches between them) give the same states,
+    histories, contacts and pair outputs bit for bit, under both rebuild policies, in graphs
+    (even and odd group counts, even and odd K) and in the batched form."""
     if mode == "batched":
         ws = [dense_workload(dem, n=n, seed=60 + i, box=(bx, bx, 60.0))
               for i, (n, bx) in enumerate([(1500, 12.0), (2600, 16.0)])]
@@ -248,9 +248,12 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
     if mode == "trigger":
         w.pdis = w.pdis * 100.0   # fast enough for displacement-triggered rebuilds
     res = []
-    for fused in (1, 0, 2):   # fused, split, auto (timed probes switch paths mid-run)
+    # fused (one-pass contact tests), split, auto (timed probes switch paths mid-run), and
+    # fused with every particle on the chunked path
+    for fused, fast in ((1, 64), (0, 64), (2, 64), (1, 0)):
         c = dem.Context(w.instances)
         c.set_option(dem.OPT_FUSED_STEP, fused)
+        c.set_option(dem.OPT_FUSED_FAST, fast)
         c.load(w)
         c.set_option(dem.OPT_RECORD_FORCES, 1)
         if mode == "trigger":
