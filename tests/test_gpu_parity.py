@@ -462,6 +462,80 @@ def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
     assert np.all(np.isfinite(out))
 
 
+def pair_keys(pairs, n):
+    """Canonical (min, max) pairs as sorted int64 keys (fast set equality for large lists)."""
+    p = np.sort(np.asarray(pairs, dtype=np.int64), axis=1)
+    return np.sort(p[:, 0] * n + p[:, 1])
+
+
+def test_full_size_2m_bimodal_box(dem, orc, gpu_ok):
+    """The 2M bimodal box workload (d = 1 and 0.5, friction 0.5) at full size, the pair-list
+    imbalance case: the loose fill compressed to 80% of its height and given random
+    displacements and spins, so that contacts of every size combination and the walls are
+    active.  Lists bit-exact after the first rebuild; forces, torques and state within 1e-10
+    of the oracle after one step and after six (rebuilds every 2 steps, so histories are
+    remapped twice at full size); the fused step (one-pass and chunked tests) and the split
+    path bit-identical."""
+    w = dem.config2()
+    n = w.n
+    say(f"[2m] n={n}")
+    rng = np.random.default_rng(41)
+    w.pos[:, 2] = w.pos[:, 2] * 0.8
+    w.pdis = rng.normal(scale=5e-3, size=(n, 3))
+    w.angv = rng.normal(scale=5e-3, size=(n, 3))
+    K = 2
+    ctx, sim = make_pair(dem, orc, w, width=128, interval=K)
+    ctx.rebuild()
+    assert sim.rebuild() == 0
+    assert np.array_equal(ctx.hash(), sim.field("hash"))
+    assert np.array_equal(ctx.sorted(), sim.field("sorted"))
+    ids, _, _ = ctx.pairs()
+    P = sim.scalars()["npairs"]
+    assert len(ids) == P
+    po = np.stack([sim.field("lp")[:P], sim.field("ln")[:P]], axis=1)
+    assert np.array_equal(pair_keys(ids, n), pair_keys(po, n))
+    say(f"[2m] lists equal, P={P} ({P / n:.2f} per sphere)")
+    sim.set_iall(0)
+    ctx.step(1)
+    assert sim.step(1) == 0
+    f, t, c = ctx.forces()
+    assert rel_err(f, sim.field("forcei")) <= TOL
+    assert rel_err(t, sim.field("torquei")) <= TOL
+    assert np.array_equal(c, sim.field("contacti"))
+    assert int(c.sum()) > n // 10, "the compressed bed must be in contact"
+    ctx.step(5)
+    oracle_steps(sim, 5, K, 1)
+    f, t, c = ctx.forces()
+    st = ctx.state()
+    assert rel_err(f, sim.field("forcei")) <= TOL
+    assert rel_err(t, sim.field("torquei")) <= TOL
+    assert np.array_equal(c, sim.field("contacti"))
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+    assert ctx.stats().rebuilds == 3
+    say("[2m] oracle parity after 6 steps")
+    del sim
+    ref = (st, f, t, c)
+    ctx.close()
+    for fused, fast in ((1, 0), (0, 64)):
+        b = dem.Context(w.instances)
+        b.set_state(w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+        b.set_option(dem.OPT_RECORD_FORCES, 1)
+        b.set_option(dem.OPT_REBUILD_INTERVAL, K)
+        b.set_option(dem.OPT_FUSED_STEP, fused)
+        b.set_option(dem.OPT_FUSED_FAST, fast)
+        b.rebuild()
+        b.step(6)
+        sb = b.state()
+        fb, tb, cb = b.forces()
+        for k in ("pos", "pdis", "angv", "qi"):
+            assert np.array_equal(sb[k], ref[0][k]), (fused, fast, k)
+        assert np.array_equal(fb, ref[1]) and np.array_equal(tb, ref[2])
+        assert np.array_equal(cb, ref[3])
+        b.close()
+
+
 def test_full_pour_packing_and_coordination(dem, orc, gpu_ok):
     """Pour of the 2,000-sphere configuration (jittered lattice into a 20d x 20d box, rebuild
     every 20 steps) run until settled on the GPU (hipGraph path) and on the oracle: packing
