@@ -94,6 +94,22 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
     return x * q + (x < r ? x : r) + i;
 }
 
+// XCD-aware order in chunks: XCD x runs chunks x, x + 8, x + 16, ... of `chunk` consecutive
+// logical blocks, so work concentrated in one part of the slot range (the contacts of a bed
+// forming at the floor, in the lowest slots) is shared by all XCDs while each chunk keeps its
+// neighbours on one L2.  Blocks past the last whole round of 8 chunks take xcd_block's order.
+// chunk == 0: one contiguous range per XCD (xcd_block).  A bijection on [0, nb).
+__device__ __forceinline__ unsigned xcd_chunk_block(unsigned b, unsigned nb, unsigned chunk)
+{
+    if (chunk == 0u) return xcd_block(b, nb);
+    const unsigned round = kXcds * chunk;
+    const unsigned full = nb / round;
+    const unsigned x = b % kXcds, i = b / kXcds;
+    if (i / chunk < full) return (i / chunk) * round + x * chunk + i % chunk;
+    const unsigned base = full * round;
+    return base + xcd_block(b - base, nb - base);
+}
+
 // ----------------------------------------------------------------------------------------
 // Rebuild
 // ----------------------------------------------------------------------------------------
@@ -436,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                                                        Hist *hist_out, POut *out,
                                                        const InstDev *inst, LawParams pp,
                                                        double dt, Hist *tape_hist,
-                                                       const unsigned *gate)
+                                                       const unsigned *gate, unsigned xchunk)
 {
     if (gated_off(gate)) return;
     const unsigned np = *npairs;
@@ -444,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
     // first member's slot, so a range touches nearby records)
     const unsigned bs = blockDim.x;
     const unsigned per = (np + gridDim.x - 1) / gridDim.x;
-    const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned p_begin = xcd_chunk_block(blockIdx.x, gridDim.x, xchunk) * per;
     const unsigned p_end = min(p_begin + per, np);
     for (unsigned p = p_begin + threadIdx.x; p < p_end; p += bs) {
         uint2 e = ab[p];
@@ -626,11 +642,11 @@ __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
     const unsigned *toff, const unsigned *jt, const POut *out, double4 *cpos, unsigned n,
     const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, const unsigned *gate)
+    PRec *tape_rec, const unsigned *gate, unsigned xchunk)
 {
     if (gated_off(gate)) return;
     const unsigned np = ctl->npairs;
-    unsigned a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    unsigned a = xcd_chunk_block(blockIdx.x, gridDim.x, xchunk) * blockDim.x + threadIdx.x;
     bool live = a < n;
     unsigned flag = 0;
     if (live) {
@@ -848,7 +864,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
     Hist *whist, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
-    PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate, unsigned fast_max)
+    PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate, unsigned fast_max,
+    unsigned xchunk)
 {
     extern __shared__ double lrec[];
     __shared__ unsigned cbits[kBits / 32];
@@ -859,7 +876,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
     const unsigned gv = *gate;   // never null (a zero word for steps that are not queued)
     const unsigned np = ctl->npairs;
     const unsigned bd = blockDim.x, t = threadIdx.x;
-    const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * bd;
+    const unsigned s0 = xcd_chunk_block(blockIdx.x, gridDim.x, xchunk) * bd;
     const unsigned nb = s0 < n ? min(bd, n - s0) : 0u;
     const unsigned a = s0 + t;
     const bool live = t < nb;

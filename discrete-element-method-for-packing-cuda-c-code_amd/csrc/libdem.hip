@@ -204,6 +204,7 @@ struct dem_ctx {
     bool fused = true;        // step path in use: fused per-particle (k_step) or split
     int fused_mode = 2;       // DEM_OPT_FUSED_STEP: 0 split, 1 fused, 2 auto (timed probes)
     unsigned fused_fast = kFast;   // DEM_OPT_FUSED_FAST: one-pass limit of the fused step
+    unsigned xcd_chunk = 8;        // DEM_OPT_XCD_CHUNK: blocks per XCD chunk of the step kernels
     int probe_left = 0;       // auto: steps left in the current probe (fused, then split;
                               // the first probe runs the pair twice and keeps the second)
     unsigned reb_since_probe = 0, probe_every = 8;
@@ -487,11 +488,11 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         if (c->batched)
             hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist, gate);
+                               dinst, one.pp, one.dt, c->tape_hist, gate, c->xcd_chunk);
         else
             hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist, gate);
+                               dinst, one.pp, one.dt, c->tape_hist, gate, c->xcd_chunk);
     }
     if (!c->fused) {   // the fused step evaluates the wall contacts itself
         if (c->batched)
@@ -520,7 +521,8 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast);
+                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
+                               c->xcd_chunk);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
@@ -528,7 +530,8 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
-                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast);
+                               c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
+                               c->xcd_chunk);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -538,13 +541,13 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
-                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate);
+                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate, c->xcd_chunk);
     } else {
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
-                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate);
+                           one, c->contacti, fdbg, c->ctl, c->tape_rec, gate, c->xcd_chunk);
     }
     if (!c->fused) {
         c->last_path = 1;
@@ -951,6 +954,13 @@ int dem_set_option(dem_ctx *c, int option, double value)
         if (!(value >= 0.0 && value <= (double)kFast) || (double)(unsigned)value != value)
             return DEM_ERR_INVALID;
         c->fused_fast = (unsigned)value;
+        drop_graph(c);
+        return DEM_OK;
+    }
+    case DEM_OPT_XCD_CHUNK: {
+        if (!(value >= 0.0 && value <= 65536.0) || (double)(unsigned)value != value)
+            return DEM_ERR_INVALID;
+        c->xcd_chunk = (unsigned)value;
         drop_graph(c);
         return DEM_OK;
     }
@@ -1733,13 +1743,13 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr, (const unsigned *)nullptr);
+                                   (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
             else
                 hipLaunchKernelGGL(k_pair_force<false>, dim3(g), dim3(c->pair_block), 0, c->stream,
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr, (const unsigned *)nullptr);
+                                   (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
             DEM_HIP(hipGetLastError());
             DEM_HIP(hipStreamSynchronize(c->stream));
         }

@@ -248,12 +248,15 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
     if mode == "trigger":
         w.pdis = w.pdis * 100.0   # fast enough for displacement-triggered rebuilds
     res = []
-    # fused (one-pass contact tests), split, auto (timed probes switch paths mid-run), and
-    # fused with every particle on the chunked path
-    for fused, fast in ((1, 64), (0, 64), (2, 64), (1, 0)):
+    # fused (one-pass contact tests), split, auto (timed probes switch paths mid-run), fused
+    # with every particle on the chunked path, and fused and split with one contiguous range
+    # of blocks per XCD
+    for fused, fast, chunk in ((1, 64, 8), (0, 64, 8), (2, 64, 8), (1, 0, 8), (1, 64, 0),
+                               (0, 64, 0)):
         c = dem.Context(w.instances)
         c.set_option(dem.OPT_FUSED_STEP, fused)
         c.set_option(dem.OPT_FUSED_FAST, fast)
+        c.set_option(dem.OPT_XCD_CHUNK, chunk)
         c.load(w)
         c.set_option(dem.OPT_RECORD_FORCES, 1)
         if mode == "trigger":
