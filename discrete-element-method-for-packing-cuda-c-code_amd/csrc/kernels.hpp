@@ -77,6 +77,32 @@ __device__ __forceinline__ void st2(double *p, double a, double b)
 
 __device__ __forceinline__ bool is_active(const unsigned *active) { return !active || *active != 0u; }
 
+// FP32 copy of a slot's position and radius: the step kernels' contact filter.
+__device__ __forceinline__ float4 pos32(double x, double y, double z, double r)
+{
+    return make_float4((float)x, (float)y, (float)z, (float)r);
+}
+
+// Conservative contact filter on the FP32 copies: false only when the spheres are certainly
+// apart, i.e. when law_touch on the FP64 values is certainly false; a true result sends the
+// pair to the FP64 law, whose own contact test decides (and gives the exact zero result of a
+// pair out of contact).  With u = 2^-24, each FP32 coordinate is within u|x| of the FP64 one
+// and the FP32 difference within u|dx| of the exact difference of the copies, so the FP32
+// separation vector is within u * sum_k (|xa_k| + |xb_k| + |dx_k|) of the exact one; the
+// margin doubles that and adds 8u of the radius sum for the roundings of the sums, products
+// and radii.  NaN compares false and keeps the pair.
+__device__ __forceinline__ bool maybe_touch(float4 a, float4 b)
+{
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float s = a.w + b.w;
+    const float u = 5.9604645e-8f;   // 2^-24
+    const float e = fabsf(a.x) + fabsf(b.x) + fabsf(dx) + fabsf(a.y) + fabsf(b.y) + fabsf(dy) +
+                    fabsf(a.z) + fabsf(b.z) + fabsf(dz);
+    const float cut = s + (2.0f * u) * e + (8.0f * u) * s;
+    return !(d2 > cut * cut);
+}
+
 // Gate of a queued step: a step launched before the host saw the previous step's rebuild
 // trigger does nothing (every kernel of it returns at once) when that trigger was raised.
 __device__ __forceinline__ bool gated_off(const unsigned *gate) { return gate && *gate != 0u; }
@@ -187,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
                                                     const InstDev *inst, InstDev one,
                                                     unsigned *wflag, unsigned *wlist,
                                                     DevCtl *ctl, unsigned *perm,
-                                                    double4 *cpos, unsigned *cid,
+                                                    double4 *cpos, float4 *cp32, unsigned *cid,
                                                     const unsigned *active)
 {
     if (!is_active(active)) return;
@@ -232,6 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     // radius, a 4-byte index: four candidates per cache line instead of one)
     double2 p01 = ld2(src), p23 = ld2(src + 2);
     cpos[s] = make_double4(p01.x, p01.y, p23.x, p23.y);
+    cp32[s] = pos32(p01.x, p01.y, p23.x, p23.y);
     cid[s] = tail.y;
 }
 
@@ -397,9 +424,10 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const uns
     }
 }
 
-// Compact positions of restored records: cpos[s] = (px, py, pz, rad) of rec[s].
+// Compact positions of restored records: cpos[s] = (px, py, pz, rad) of rec[s], and its FP32
+// copy.
 __global__ __launch_bounds__(kBlock) void k_cpos_from_rec(const PRec *rec, unsigned n,
-                                                          double4 *cpos)
+                                                          double4 *cpos, float4 *cp32)
 {
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
@@ -408,6 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_cpos_from_rec(const PRec *rec, unsig
     double *C = reinterpret_cast<double *>(cpos + s);
     st2(C, p01.x, p01.y);
     st2(C + 2, p23.x, p23.y);
+    cp32[s] = pos32(p01.x, p01.y, p23.x, p23.y);
 }
 
 // jf from a restored list: jf[k] = first member of pair jt[k].
@@ -573,7 +602,7 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
                                                     d3 fw, d3 tw, const InstDev *inst,
                                                     const InstDev &one,
                                                     unsigned *contacti, double *fdbg,
-                                                    PRec *tape_rec, double4 *cpos_out)
+                                                    PRec *tape_rec, float4 *cp32_out)
 {
     double2 r0 = ld2(R), r1 = ld2(R + 2), r2 = ld2(R + 4), r3 = ld2(R + 6), r4 = ld2(R + 8),
             r5 = ld2(R + 10), r6 = ld2(R + 12);
@@ -625,9 +654,7 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
     st2(W + 12, qx, qy);
     W[14] = qzz;
     *reinterpret_cast<uint2 *>(W + 15) = tail;
-    double *CP = reinterpret_cast<double *>(cpos_out + a);
-    st2(CP, x.x, x.y);
-    st2(CP + 2, x.z, r1.y);
+    cp32_out[a] = pos32(x.x, x.y, x.z, r1.y);
     contacti[a] = c;
     double q2 = qx * qx + qy * qy + qzz * qzz;
     return q2 > I.ann2 ? 1u : 0u;
@@ -640,7 +667,7 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
-    const unsigned *toff, const unsigned *jt, const POut *out, double4 *cpos, unsigned n,
+    const unsigned *toff, const unsigned *jt, const POut *out, float4 *cp32, unsigned n,
     const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
     PRec *tape_rec, const unsigned *gate, unsigned xchunk)
 {
@@ -689,7 +716,7 @@ __global__ __launch_bounds__(kBlock) void k_particle(
         }
         double *R = reinterpret_cast<double *>(rec + a);
         flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wall, fw, tw, inst,
-                                        one, contacti, fdbg, tape_rec, cpos);
+                                        one, contacti, fdbg, tape_rec, cp32);
     }
     // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
     if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
@@ -728,20 +755,19 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
 // Wall candidates of the last rebuild evaluate their wall contact here too (wall history
 // per slot, in place), before the pair work, while few registers are live.
 //
-// Contact tests first: the partners' positions and radii come from the compact copy cin
-// (four slots per cache line), several candidates at a time (independent loads in flight
-// together), through the law's own contact test (law_touch); pairs out of contact get the
-// all-zero law result (history reset) at once.  The laws of the pairs in contact then run
-// in order, reading the rest of the partner's record and the history only for them.  A pair
-// out of contact contributes +0.0 to every sum; the running sums start at +0.0 and so are
-// never -0.0, for which x + 0.0 == x and x - 0.0 == x bit for bit: leaving those terms out
-// keeps the reference order's result.
+// Contact filter first: the partners' positions and radii come from the 16-byte FP32 copy
+// cin (eight slots per cache line), several candidates at a time (independent loads in
+// flight together), through maybe_touch, which is false only for spheres certainly apart;
+// those pairs get the all-zero law result (history reset) at once.  The FP64 law then runs
+// for every other pair, in order, reading the partner's record and the history: its own
+// contact test decides, and a pair it finds out of contact gets the same all-zero result.  A
+// pair out of contact contributes +0.0 (or -0.0) to every sum; the running sums start at +0.0
+// and so are never -0.0, for which x + 0.0 == x, x + -0.0 == x and x - 0.0 == x bit for bit:
+// skipping those terms or adding them keeps the reference order's result.
 //
-// A lane with at most kFast candidates on each side tests both sides in one pass (own and
-// transposed loads in flight together) and runs one law loop over its contacts, own pairs
-// first: a wave then waits for its longest lane's contact count, not for the longest own
-// count plus the longest transposed count.  Lanes with more candidates go through chunks of
-// kChunk per side, own side first.
+// A lane with at most kFast candidates on each side filters both sides in one pass (own and
+// transposed loads in flight together) and its laws are balanced over the wave (below).
+// Lanes with more candidates go through chunks of kChunk per side, own side first.
 constexpr unsigned kChunk = 32;
 constexpr unsigned kFast = 64;
 constexpr unsigned kLook = 4;
@@ -862,7 +888,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock, 4) void k_step(
     const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
-    Hist *whist, const double4 *cin, double4 *cout, unsigned char *zs, unsigned zmode,
+    Hist *whist, const float4 *cin, float4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
     PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate, unsigned fast_max,
     unsigned xchunk)
@@ -942,15 +968,14 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             }
         fast = no <= fast_max && nt <= fast_max;
         if (fast) {
-            // both sides' contact tests in one pass
+            // both sides' contact filters in one pass
             double2 a0 = ld2(RA), a1 = ld2(RA + 2);
-            const d3 xa = d3{a0.x, a0.y, a1.x};
-            const double ra = a1.y;
+            const float4 xa = pos32(a0.x, a0.y, a1.x, a1.y);
             const unsigned m = no > nt ? no : nt;
             for (unsigned i = 0; i < m; i += kLook) {
                 unsigned b[kLook], f[kLook];
                 bool vt[kLook];
-                double2 c0[kLook], c1[kLook], d0[kLook], d1[kLook];
+                float4 cb[kLook], cf[kLook];
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
                     b[k] = i + k < no ? ab[p0 + i + k].y : a;
@@ -960,22 +985,17 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 }
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
-                    const double *B = reinterpret_cast<const double *>(cin + b[k]);
-                    c0[k] = ld2(B);
-                    c1[k] = ld2(B + 2);
-                    const double *F = reinterpret_cast<const double *>(cin + (vt[k] ? f[k] : a));
-                    d0[k] = ld2(F);
-                    d1[k] = ld2(F + 2);
+                    cb[k] = cin[b[k]];
+                    cf[k] = cin[vt[k] ? f[k] : a];
                 }
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
                     if (i + k < no) {
-                        const bool con = law_touch(xa, d3{c0[k].x, c0[k].y, c1[k].x}, ra, c1[k].y);
+                        const bool con = maybe_touch(xa, cb[k]);
                         if (con) mo |= 1ull << (i + k);
                         own_outcome(coop, cbits, p0 + i + k - pb0, zs, hout, p0 + i + k, con, zmode);
                     }
-                    if (vt[k] && law_touch(d3{d0[k].x, d0[k].y, d1[k].x}, xa, d1[k].y, ra))
-                        mt |= 1ull << (i + k);
+                    if (vt[k] && maybe_touch(cf[k], xa)) mt |= 1ull << (i + k);
                 }
             }
         }
@@ -1088,26 +1108,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
     if (live && !fast) {
         const unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
         double2 a0 = ld2(RA), a1 = ld2(RA + 2);
-        const d3 xa = d3{a0.x, a0.y, a1.x};
-        const double ra = a1.y;
+        const float4 xa = pos32(a0.x, a0.y, a1.x, a1.y);
         for (unsigned base = p0; base < p1; base += kChunk) {
             const unsigned end = min(base + kChunk, p1);
             unsigned mask = 0;
             for (unsigned p = base; p < end; p += kLook) {
                 unsigned b[kLook];
-                double2 c0[kLook], c1[kLook];
+                float4 cb[kLook];
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    const double *B = reinterpret_cast<const double *>(cin + b[i]);
-                    c0[i] = ld2(B);
-                    c1[i] = ld2(B + 2);
-                }
+                for (unsigned i = 0; i < kLook; i++) cb[i] = cin[b[i]];
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) {
                     if (p + i >= end) break;
-                    const bool con = law_touch(xa, d3{c0[i].x, c0[i].y, c1[i].x}, ra, c1[i].y);
+                    const bool con = maybe_touch(xa, cb[i]);
                     if (con) mask |= 1u << (p + i - base);
                     own_outcome(coop, cbits, p + i - pb0, zs, hout, p + i, con, zmode);
                 }
@@ -1135,7 +1150,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             unsigned mask = 0;
             for (unsigned k = base; k < end; k += kLook) {
                 unsigned q[kLook], f[kLook];
-                double2 c0[kLook], c1[kLook];
+                float4 cf[kLook];
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) {
                     bool in = k + i < end;
@@ -1143,15 +1158,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                     f[i] = in ? jf[k + i] : a;
                 }
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    const double *F = reinterpret_cast<const double *>(cin + (q[i] < np ? f[i] : a));
-                    c0[i] = ld2(F);
-                    c1[i] = ld2(F + 2);
-                }
+                for (unsigned i = 0; i < kLook; i++) cf[i] = cin[q[i] < np ? f[i] : a];
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++)
-                    if (q[i] < np && law_touch(d3{c0[i].x, c0[i].y, c1[i].x}, xa, c1[i].y, ra))
-                        mask |= 1u << (k + i - base);
+                    if (q[i] < np && maybe_touch(cf[i], xa)) mask |= 1u << (k + i - base);
             }
             while (mask) {
                 const unsigned k = base + (unsigned)__builtin_ctz(mask);

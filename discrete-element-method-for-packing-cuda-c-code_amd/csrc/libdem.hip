@@ -179,10 +179,12 @@ struct dem_ctx {
     int pc = 0;   // index of the current wall histories and inverse slot map (flips at rebuilds)
     unsigned *contacti = nullptr;
     unsigned *wflag = nullptr, *wlist = nullptr;   // wall candidates of the current rebuild
-    // Position + radius per slot (32 bytes: four slots per cache line), cpos[rc] mirroring
-    // rec[rc]: written by the reorder and by both step paths, read by the candidate searches
-    // and the fused step's contact tests.
+    // Position + radius per slot (32 bytes: four slots per cache line): written by the reorder,
+    // read by the candidate searches of the same rebuild.
     double4 *cpos[2] = {nullptr, nullptr};
+    // Its FP32 copy (16 bytes), cp32[rc] mirroring rec[rc]: written by the reorder and by both
+    // step paths, read by the step kernels' contact filter.
+    float4 *cp32[2] = {nullptr, nullptr};
     unsigned *cid = nullptr;   // particle index per slot, written by the reorder
     double *wforce = nullptr;                      // k_walls output, 8 doubles per slot
     double *fdbg = nullptr;
@@ -362,13 +364,13 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
                            c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
-                           c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cid, active);
+                           c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
                            c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
-                           c->tape_perm, c->cpos[nr], c->cid, active);
+                           c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, active);
     c->rc = nr;
     c->pc = 1 - c->pc;
     if (c->batched)
@@ -519,7 +521,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
                                c->xcd_chunk);
@@ -528,7 +530,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
-                               c->whist[c->pc], (const double4 *)c->cpos[c->rc], c->cpos[nr],
+                               c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
                                c->xcd_chunk);
@@ -540,13 +542,13 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
-                           (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
+                           (const unsigned *)c->jt, (const POut *)c->out, c->cp32[c->rc], c->n, dinst,
                            one, c->contacti, fdbg, c->ctl, c->tape_rec, gate, c->xcd_chunk);
     } else {
         hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
-                           (const unsigned *)c->jt, (const POut *)c->out, c->cpos[c->rc], c->n, dinst,
+                           (const unsigned *)c->jt, (const POut *)c->out, c->cp32[c->rc], c->n, dinst,
                            one, c->contacti, fdbg, c->ctl, c->tape_rec, gate, c->xcd_chunk);
     }
     if (!c->fused) {
@@ -630,7 +632,7 @@ void free_ctx(dem_ctx *c)
     if (!c) return;
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
-                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->jf, c->zs, c->ctl};
     for (void *p : bufs)
@@ -885,6 +887,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->wforce, sizeof(double) * 8 * (size_t)n));
     DEM_HIP(hipMalloc(&c->cpos[0], sizeof(double4) * (size_t)n));
     DEM_HIP(hipMalloc(&c->cpos[1], sizeof(double4) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cp32[0], sizeof(float4) * (size_t)n));
+    DEM_HIP(hipMalloc(&c->cp32[1], sizeof(float4) * (size_t)n));
     DEM_HIP(hipMalloc(&c->cid, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
@@ -2273,7 +2277,7 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
     c->zs_ok = false;
     DEM_HIP(hipMemcpyAsync(c->whist[c->pc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(k_cpos_from_rec, dim3(cdiv((unsigned)n, kBlock)), dim3(kBlock), 0, st,
-                       (const PRec *)c->rec[c->rc], (unsigned)n, c->cpos[c->rc]);
+                       (const PRec *)c->rec[c->rc], (unsigned)n, c->cpos[c->rc], c->cp32[c->rc]);
     DEM_HIP(hipMemcpyAsync(c->inv[c->pc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
     DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
     if (S.have_list) {
