@@ -67,7 +67,14 @@ struct DevCtl {
     unsigned skip;           // copy of need_rebuild made by k_publish: the gate of queued steps
     unsigned zero;           // always 0: k_step's gate when the step is not queued
     unsigned pad;
+    // Triggers of queued fused steps, by step parity: a queued k_step raises trig[s & 1] and
+    // is gated by trig[(s - 1) & 1], so no publish kernel is needed between queued steps (a
+    // gated step sets its own word, so the gate carries on to the steps after it).  Cleared
+    // by every rebuild and before every queued run.
+    unsigned trig[2];
+    unsigned pad2[2];
 };
+static_assert(sizeof(DevCtl) == 48, "DevCtl is 12 words (the per-instance tail is 8-aligned)");
 
 __device__ __forceinline__ double2 ld2(const double *p) { return *reinterpret_cast<const double2 *>(p); }
 __device__ __forceinline__ void st2(double *p, double a, double b)
@@ -154,6 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
         ctl->need_rebuild = 0u;
         ctl->skip = 0u;
         ctl->nwall = 0u;
+        ctl->trig[0] = 0u;
+        ctl->trig[1] = 0u;
     }
     if (s >= n) return;
     const PRec &r = rec[s];
@@ -891,7 +900,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
     Hist *whist, const float4 *cin, float4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
     PRec *tape_rec, Hist *tape_hist, Hist *tape_whist, const unsigned *gate, unsigned fast_max,
-    unsigned xchunk)
+    unsigned xchunk, unsigned *trig, unsigned *slot)
 {
     extern __shared__ double lrec[];
     __shared__ unsigned cbits[kBits / 32];
@@ -939,7 +948,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
         if (k < 8u * nb) st2(lrec + (k >> 3) * kRow + 2 * (k & 7u), v[i].x, v[i].y);
     }
     if (zin) zsl[t] = zword;
-    if (gv != 0u) return;   // a gated-off step: uniform over the grid, no lane reaches the barrier
+    if (gv != 0u) {   // a gated-off step: uniform over the grid, no lane reaches the barrier
+        if (blockIdx.x == 0 && t == 0) *trig = 1u;   // the gate carries on to the next step
+        return;
+    }
     __syncthreads();
     double *RA = lrec + t * kRow;
     unsigned flag = 0;
@@ -1206,7 +1218,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             zs_pair_z(zs, hout, p, (cbits[k >> 5] >> (k & 31u)) & 1u, zmode, z);
         }
     }
-    if (__any(flag) && __lane_id() == 0) atomicOr(&ctl->need_rebuild, 1u);
+    // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare); a
+    // queued step also reports it in its host-mapped outcome slot
+    if (__any(flag) && __lane_id() == 0) {
+        atomicOr(trig, 1u);
+        if (slot) *reinterpret_cast<volatile unsigned *>(slot) = 1u;
+    }
 }
 
 // A bounded busy period of `ticks` wall-clock ticks (one lane; the clock runs at a fixed

@@ -207,6 +207,7 @@ struct dem_ctx {
     int fused_mode = 2;       // DEM_OPT_FUSED_STEP: 0 split, 1 fused, 2 auto (timed probes)
     unsigned fused_fast = kFast;   // DEM_OPT_FUSED_FAST: one-pass limit of the fused step
     unsigned xcd_chunk = 8;        // DEM_OPT_XCD_CHUNK: blocks per XCD chunk of the step kernels
+    unsigned tpar = 0;             // parity of the next queued fused step (DevCtl::trig)
     int probe_left = 0;       // auto: steps left in the current probe (fused, then split;
                               // the first probe runs the pair twice and keeps the second)
     unsigned reb_since_probe = 0, probe_every = 8;
@@ -512,7 +513,14 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         // zero-history bookkeeping: bit nh of zs describes the buffer written now; with
         // valid bookkeeping (lazy) pairs already known to be zero there are not rewritten
         const unsigned zmode = (1u << nh) | (c->zs_ok ? 4u : 0u);
-        const unsigned *kgate = gate ? gate : &c->ctl->zero;
+        // queued (slot given): gated by the previous step's trigger word, raises its own and
+        // reports a trigger in the slot, with no publish kernel; otherwise ungated, raising
+        // need_rebuild, which k_publish copies to the mirror
+        const bool queued = slot != nullptr;
+        const unsigned par = c->tpar;
+        const unsigned *kgate = queued ? &c->ctl->trig[1 - par] : (gate ? gate : &c->ctl->zero);
+        unsigned *ktrig = queued ? &c->ctl->trig[par] : &c->ctl->need_rebuild;
+        if (queued) c->tpar = 1 - par;
         c->zs_ok = true;
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by k_walls, so they move with the records
@@ -524,7 +532,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
-                               c->xcd_chunk);
+                               c->xcd_chunk, ktrig, queued ? slot : nullptr);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
@@ -533,7 +541,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
                                c->zs, zmode, c->n, dinst, one, c->contacti, fdbg, c->ctl,
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
-                               c->xcd_chunk);
+                               c->xcd_chunk, ktrig, queued ? slot : nullptr);
         c->rc = nr;
         c->hc = nh;
         c->out_valid = true;
@@ -556,7 +564,8 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         c->zs_ok = false;   // the split step rewrites hist[hc] in place
     }
     if (end) (void)hipEventRecord(end, st);
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d, gate, slot);
+    if (!(c->fused && slot))   // a queued fused step needs no publish kernel
+        hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d, gate, slot);
 }
 
 InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
@@ -1133,10 +1142,12 @@ int dem_rebuild(dem_ctx *c)
 namespace {
 
 // dem_step under the displacement trigger with steps queued ahead of the host.  A queued
-// step that starts after its predecessor raised the trigger does nothing (its kernels read
-// the gate word k_publish sets); when the host sees the trigger in the predecessor's slot it
-// drains the queue, undoes the host-side buffer flips of the steps that did not run, rebuilds
-// and issues them again.  The steps that run, and their results, are the synchronous loop's.
+// step that starts after its predecessor raised the trigger does nothing (the fused step
+// reads its predecessor's trigger word, the split path's kernels the gate word k_publish
+// sets); a step that raises the trigger writes 1 into its host-mapped slot.  When the host
+// sees the trigger it drains the queue, undoes the host-side buffer flips of the steps that
+// did not run, rebuilds and issues them again.  The steps that run, and their results, are
+// the synchronous loop's.
 int step_queued(dem_ctx *c, int nsteps)
 {
     struct Issued {
@@ -1148,6 +1159,10 @@ int step_queued(dem_ctx *c, int nsteps)
     const unsigned *gate = &c->ctl->skip;
     volatile unsigned *flag = c->qflag_h;
     int done = 0, head = 0, inflight = 0;
+    // queued fused steps gate on each other's trigger words: start from both clear
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, c->stream, c->ctl->trig, 2u, 0u,
+                       (const unsigned *)nullptr);
+    c->tpar = 0;
     while (done < nsteps) {
         if (inflight == 0) {
             if (c->need_rebuild) {
@@ -1183,15 +1198,14 @@ int step_queued(dem_ctx *c, int nsteps)
         while (inflight < depth && done + inflight < nsteps) {
             int k = (head + inflight) % dem_ctx::kQueue;
             q[k] = Issued{c->rc, c->hc, c->last_path, c->out_valid};
-            flag[k] = 2u;
+            flag[k] = 0u;   // set to 1 by the step if it raises the trigger
             launch_force_step(c, pair_grid(c, c->npairs), nullptr, nullptr, gate, c->qflag_d + k);
             DEM_HIP(hipEventRecord(c->evq[k], c->stream));
             inflight++;
         }
         DEM_HIP(hipGetLastError());
         DEM_HIP(hipEventSynchronize(c->evq[head]));
-        unsigned f = flag[head];
-        if (f == 2u) return DEM_ERR_HIP;   // the oldest queued step always runs
+        unsigned f = flag[head];   // the oldest queued step always runs
         head = (head + 1) % dem_ctx::kQueue;
         inflight--;
         done++;
@@ -1650,7 +1664,7 @@ int dem_get_stats(dem_ctx *c, dem_stats *o)
     o->npairs = c->npairs;
     o->pair_capacity = std::min(c->cap[c->lc], c->out_cap);
     o->ninstances = (unsigned)c->ninst;
-    o->last_rebuild_flag = m->need_rebuild;
+    o->last_rebuild_flag = c->last_flag;
     o->overflow = m->overflow;
     const volatile unsigned long long *mz = reinterpret_cast<const volatile unsigned long long *>(c->mirror_h + sizeof(DevCtl));
     const volatile unsigned *mp = reinterpret_cast<const volatile unsigned *>(mz + c->ninst);
