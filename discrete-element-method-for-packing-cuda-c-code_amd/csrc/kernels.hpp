@@ -991,14 +991,13 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
                     b[k] = i + k < no ? ab[p0 + i + k].y : a;
-                    unsigned q = i + k < nt ? jt[q0 + i + k] : kEmpty;
-                    vt[k] = q < np;
-                    f[k] = i + k < nt ? jf[q0 + i + k] : a;
+                    vt[k] = i + k < nt;
+                    f[k] = vt[k] ? jf[q0 + i + k] : a;   // (the pair index jt is read by the law)
                 }
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
                     cb[k] = cin[b[k]];
-                    cf[k] = cin[vt[k] ? f[k] : a];
+                    cf[k] = cin[f[k]];
                 }
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
@@ -1059,6 +1058,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                     pp = jt[q0L + idx];
                     pb = jf[q0L + idx];
                 }
+                // a transposed entry without a pair (jt past the list) takes no part
+                const bool valid = pp < np;
+                if (!valid) pp = 0u;
                 const double *S = lrec + (wbase + L) * kRow;   // the owner's record
                 const double *B = reinterpret_cast<const double *>(rin + pb);
                 const double *H = reinterpret_cast<const double *>(hin + pp);
@@ -1086,6 +1088,10 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                               sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
                               own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
                               d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo);
+                if (!valid) {
+                    rc = false;
+                    fa = ta = tb = d3{0.0, 0.0, 0.0};
+                }
                 if (own) {
                     double *HW = reinterpret_cast<double *>(hout + pp);
                     st2(HW, ho.x, ho.y);
@@ -1161,24 +1167,21 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             const unsigned end = min(base + kChunk, q1);
             unsigned mask = 0;
             for (unsigned k = base; k < end; k += kLook) {
-                unsigned q[kLook], f[kLook];
+                unsigned f[kLook];
                 float4 cf[kLook];
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) {
-                    bool in = k + i < end;
-                    q[i] = in ? jt[k + i] : kEmpty;
-                    f[i] = in ? jf[k + i] : a;
-                }
+                for (unsigned i = 0; i < kLook; i++) f[i] = k + i < end ? jf[k + i] : a;
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) cf[i] = cin[q[i] < np ? f[i] : a];
+                for (unsigned i = 0; i < kLook; i++) cf[i] = cin[f[i]];
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++)
-                    if (q[i] < np && maybe_touch(cf[i], xa)) mask |= 1u << (k + i - base);
+                    if (k + i < end && maybe_touch(cf[i], xa)) mask |= 1u << (k + i - base);
             }
             while (mask) {
                 const unsigned k = base + (unsigned)__builtin_ctz(mask);
                 mask &= mask - 1u;
                 const unsigned q = jt[k];
+                if (q >= np) continue;   // a transposed entry without a pair takes no part
                 const double *F = reinterpret_cast<const double *>(rin + jf[k]);
                 const double *H = reinterpret_cast<const double *>(hin + q);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
