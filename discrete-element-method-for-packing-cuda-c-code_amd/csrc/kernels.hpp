@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const double4 *cpos, const unsigned *cid, const uint2 *sorted, unsigned n,
     const InstDev *inst, InstDev one,
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
-    unsigned cap, uint2 *ab, Hist *hist, unsigned *jt, const unsigned *inv_old,
+    unsigned cap, uint2 *ab, unsigned *bo, Hist *hist, unsigned *jt, const unsigned *inv_old,
     const uint2 *ab_old, const unsigned *off_old, const Hist *hist_old, unsigned n_old,
     unsigned cap_old, DevCtl *ctl, unsigned *src, const unsigned *active)
 {
@@ -390,6 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
         if (idb > ida) {
             if (pg >= cap) { over = true; return; }
             ab[pg] = make_uint2(a, b);
+            bo[pg] = b;
             Hist h = Hist{0.0, 0.0, 0.0, 0.0};
             unsigned q = kEmpty;
             if (oa != kEmpty) {
@@ -450,12 +451,13 @@ __global__ __launch_bounds__(kBlock) void k_cpos_from_rec(const PRec *rec, unsig
 
 // jf from a restored list: jf[k] = first member of pair jt[k].
 __global__ __launch_bounds__(kBlock) void k_owner(const uint2 *ab, const unsigned *jt,
-                                                  unsigned np, unsigned *jf)
+                                                  unsigned np, unsigned *jf, unsigned *bo)
 {
     unsigned k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= np) return;
     unsigned q = jt[k];
     jf[k] = q < np ? ab[q].x : kEmpty;
+    bo[k] = ab[k].y;
 }
 
 // Pair-capacity check after the scan: npairs = min(total, cap); overflow when total > cap.
@@ -895,7 +897,7 @@ __device__ __forceinline__ d3 sel3(bool c, d3 u, d3 v)
 
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock, 4) void k_step(
-    const PRec *rin, PRec *rout, const uint2 *ab, const unsigned *off, const unsigned *toff,
+    const PRec *rin, PRec *rout, const unsigned *bo, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
     Hist *whist, const float4 *cin, float4 *cout, unsigned char *zs, unsigned zmode,
     unsigned n, const InstDev *inst, InstDev one, unsigned *contacti, double *fdbg, DevCtl *ctl,
@@ -990,7 +992,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 float4 cb[kLook], cf[kLook];
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
-                    b[k] = i + k < no ? ab[p0 + i + k].y : a;
+                    b[k] = i + k < no ? bo[p0 + i + k] : a;
                     vt[k] = i + k < nt;
                     f[k] = vt[k] ? jf[q0 + i + k] : a;   // (the pair index jt is read by the law)
                 }
@@ -1053,7 +1055,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 unsigned pp, pb;
                 if (own) {
                     pp = p0L + idx;
-                    pb = ab[pp].y;
+                    pb = bo[pp];
                 } else {
                     pp = jt[q0L + idx];
                     pb = jf[q0L + idx];
@@ -1134,7 +1136,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 unsigned b[kLook];
                 float4 cb[kLook];
 #pragma unroll
-                for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? ab[p + i].y : a;
+                for (unsigned i = 0; i < kLook; i++) b[i] = p + i < end ? bo[p + i] : a;
 #pragma unroll
                 for (unsigned i = 0; i < kLook; i++) cb[i] = cin[b[i]];
 #pragma unroll
@@ -1148,7 +1150,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             while (mask) {
                 const unsigned p = base + (unsigned)__builtin_ctz(mask);
                 mask &= mask - 1u;
-                const double *B = reinterpret_cast<const double *>(rin + ab[p].y);
+                const double *B = reinterpret_cast<const double *>(rin + bo[p]);
                 const double *H = reinterpret_cast<const double *>(hin + p);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
                 d3 fa, ta, tb, ho;

@@ -239,6 +239,7 @@ struct dem_ctx {
     POut *out = nullptr;
     unsigned *jt = nullptr;
     unsigned *jf = nullptr;   // jt's companion: the slot of the pair's first member
+    unsigned *bo = nullptr;   // the second member's slot of each pair of the current list
     // Per pair of the current list, bit k: hist[k] holds an exact zero for the pair.  Valid
     // (zs_ok) only across consecutive fused steps: a rebuild, a split step or a restore of the
     // histories leaves it stale, and the next fused step then rewrites it from scratch.
@@ -311,15 +312,18 @@ int alloc_out(dem_ctx *c, unsigned cap)
     if (c->out) DEM_HIP(hipFree(c->out));
     if (c->jt) DEM_HIP(hipFree(c->jt));
     if (c->jf) DEM_HIP(hipFree(c->jf));
+    if (c->bo) DEM_HIP(hipFree(c->bo));
     if (c->zs) DEM_HIP(hipFree(c->zs));
     c->out = nullptr;
     c->jt = nullptr;
     c->jf = nullptr;
+    c->bo = nullptr;
     c->zs = nullptr;
     c->zs_ok = false;
     DEM_HIP(hipMalloc(&c->out, (size_t)std::max(cap, 1u) * sizeof(POut)));
     DEM_HIP(hipMalloc(&c->jt, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
     DEM_HIP(hipMalloc(&c->jf, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
+    DEM_HIP(hipMalloc(&c->bo, (size_t)std::max(cap, 1u) * sizeof(unsigned)));
     DEM_HIP(hipMalloc(&c->zs, zs_bytes(std::max(cap, 1u))));
     c->out_cap = cap;
     c->buf_gen++;
@@ -410,13 +414,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->hist[nh], c->jt, inv_old,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->hist[nh], c->jt, inv_old,
                            c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
                            c->tape_src, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
@@ -526,7 +530,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         // slot and updated in place by k_walls, so they move with the records
         if (c->batched)
             hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
-                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
+                               (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
@@ -535,7 +539,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                c->xcd_chunk, ktrig, queued ? slot : nullptr);
         else
             hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
-                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->off[c->lc],
+                               (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
                                c->whist[c->pc], (const float4 *)c->cp32[c->rc], c->cp32[nr],
@@ -643,7 +647,7 @@ void free_ctx(dem_ctx *c)
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
                     c->nl, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
-                    c->hist[1], c->out, c->jt, c->jf, c->zs, c->ctl};
+                    c->hist[1], c->out, c->jt, c->jf, c->bo, c->zs, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     c->S.release();
@@ -1338,22 +1342,26 @@ int dem_step_graph(dem_ctx *c, int nsteps)
     if (c->hcap[1 - c->hc] < want && (rc = alloc_hist(c, 1 - c->hc, want))) return rc;
     if ((rc = grow_hist_keep(c, c->hc, want))) return rc;
     if (c->out_cap < want) {
-        // out / jt / jf hold the live transposed index: keep jt's and jf's contents
+        // out / jt / jf / bo hold the live transposed index and partner slots: keep their contents
         POut *no = nullptr;
-        unsigned *nj = nullptr, *nf = nullptr;
+        unsigned *nj = nullptr, *nf = nullptr, *nb = nullptr;
         DEM_HIP(hipMalloc(&no, (size_t)want * sizeof(POut)));
         DEM_HIP(hipMalloc(&nj, (size_t)want * sizeof(unsigned)));
         DEM_HIP(hipMalloc(&nf, (size_t)want * sizeof(unsigned)));
+        DEM_HIP(hipMalloc(&nb, (size_t)want * sizeof(unsigned)));
         DEM_HIP(hipMemcpyAsync(nj, c->jt, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
         DEM_HIP(hipMemcpyAsync(nf, c->jf, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
+        DEM_HIP(hipMemcpyAsync(nb, c->bo, (size_t)c->out_cap * sizeof(unsigned), hipMemcpyDeviceToDevice, c->stream));
         DEM_HIP(hipStreamSynchronize(c->stream));
         DEM_HIP(hipFree(c->out));
         DEM_HIP(hipFree(c->jt));
         DEM_HIP(hipFree(c->jf));
+        DEM_HIP(hipFree(c->bo));
         DEM_HIP(hipFree(c->zs));
         c->out = no;
         c->jt = nj;
         c->jf = nf;
+        c->bo = nb;
         c->zs = nullptr;
         c->zs_ok = false;
         DEM_HIP(hipMalloc(&c->zs, zs_bytes(want)));
@@ -2305,7 +2313,7 @@ int snap_restore(dem_ctx *c, const AdjSnap &S)
             DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
             hipLaunchKernelGGL(k_owner, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, st,
                                (const uint2 *)c->ab[c->lc], (const unsigned *)c->jt, (unsigned)np,
-                               c->jf);
+                               c->jf, c->bo);
             DEM_HIP(hipGetLastError());
         }
     }
