@@ -163,7 +163,7 @@ EXPORTS = [
     "dem_deck_material", "dem_write_restart", "dem_damping_for_restitution",
     "dem_restitution_for_damping", "dem_gen_diameters", "dem_gen_insert", "dem_gen_lattice",
     "dem_particle_props", "dem_make_grid", "dem_dev_alloc", "dem_dev_free", "dem_memcpy",
-    "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_law_pair_vjp",
+    "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_contact_filter", "dem_law_pair_vjp",
     "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
     "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
     "dem_adj_release", "dem_get_step_timing",
@@ -214,6 +214,7 @@ def lib() -> ctypes.CDLL:
         "dem_memzero": ([P, ctypes.c_size_t], c_int),
         "dem_device_sync": ([], c_int),
         "dem_law_pair_eval": ([P, P, P], c_int),
+        "dem_contact_filter": ([P, c_int, P], c_int),
         "dem_law_pair_vjp": ([P, P, P, P, P], c_int),
         "dem_law_walls_eval": ([P, P, P, P], c_int),
         "dem_law_walls_vjp": ([P, P, P, P, P, P], c_int),

@@ -124,6 +124,32 @@ DEM_HD bool law_touch(d3 xa, d3 xb, double ra, double rb)
     return dn > 0.0;
 }
 
+// FP32 copy of a slot's position and radius: the step kernels' contact filter.
+DEM_HD float4 pos32(double x, double y, double z, double r)
+{
+    return make_float4((float)x, (float)y, (float)z, (float)r);
+}
+
+// Conservative contact filter on the FP32 copies: false only when the spheres are certainly
+// apart, i.e. when law_touch on the FP64 values is certainly false; a true result sends the
+// pair to the FP64 law, whose own contact test decides (and gives the exact zero result of a
+// pair out of contact).  With u = 2^-24, each FP32 coordinate is within u|x| of the FP64 one
+// and the FP32 difference within u|dx| of the exact difference of the copies, so the FP32
+// separation vector is within u * sum_k (|xa_k| + |xb_k| + |dx_k|) of the exact one; the
+// margin doubles that and adds 8u of the radius sum for the roundings of the sums, products
+// and radii.  NaN compares false and keeps the pair.
+DEM_HD bool maybe_touch(float4 a, float4 b)
+{
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    const float s = a.w + b.w;
+    const float u = 5.9604645e-8f;   // 2^-24
+    const float e = fabsf(a.x) + fabsf(b.x) + fabsf(dx) + fabsf(a.y) + fabsf(b.y) + fabsf(dy) +
+                    fabsf(a.z) + fabsf(b.z) + fabsf(dz);
+    const float cut = s + (2.0f * u) * e + (8.0f * u) * s;
+    return !(d2 > cut * cut);
+}
+
 // Pair law between bodies a and b.  Returns true on contact; on no contact every output
 // is zero (history reset), matching the reference's no-contact branch (neiglist.cu:220-227).
 DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,

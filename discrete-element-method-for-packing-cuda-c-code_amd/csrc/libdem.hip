@@ -1851,6 +1851,16 @@ int dem_law_pair_eval(const double *in, const double *mat, double *out)
     return c ? 1 : 0;
 }
 
+int dem_contact_filter(const double *in, int count, int *keep)
+{
+    if (!in || !keep || count < 0) return DEM_ERR_INVALID;
+    for (int i = 0; i < count; i++) {
+        const double *q = in + 8 * (size_t)i;
+        keep[i] = maybe_touch(pos32(q[0], q[1], q[2], q[3]), pos32(q[4], q[5], q[6], q[7])) ? 1 : 0;
+    }
+    return DEM_OK;
+}
+
 int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
                      double *matbar)
 {

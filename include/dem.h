@@ -321,6 +321,9 @@ int dem_device_sync(void);
  * contact.  vjp: outbar = adjoints of (F, Ta, Tb, h_out) (12); inbar = adjoints of xa, xb, va,
  * vb, wa, wb, hin (21); matbar = adjoints of (E*, mu, gamma_n, gamma_t, mu_r). */
 int dem_law_pair_eval(const double *in, const double *mat, double *out);
+/* The step kernels' FP32 contact filter on the host: in = count x (xa (3), ra, xb (3), rb);
+ * keep[i] = 0 only if the spheres are certainly apart (the FP64 law then finds no contact). */
+int dem_contact_filter(const double *in, int count, int *keep);
 int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
                      double *matbar);
 /* Walls: in = x, v, w (9), r, m, hin (3), dt (15); out = F, T, h_out, |Ft| (10), returns the
