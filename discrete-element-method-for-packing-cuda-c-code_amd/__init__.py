@@ -156,7 +156,8 @@ EXPORTS = [
     "calcHash", "RadixSort", "findCellStart", "neighborarray", "prefixsum", "pairlist",
     "calculateforcepair", "sumallforcei", "calculateforceiw", "updateposition",
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
-    "dem_get_state", "dem_get_wall_history", "dem_step", "dem_step_graph",
+    "dem_get_state", "dem_get_wall_history", "dem_save_checkpoint", "dem_load_checkpoint",
+    "dem_read_restart", "dem_write_restart_resume", "dem_step", "dem_step_graph",
     "dem_request_rebuild", "dem_rebuild", "dem_autotune", "dem_set_active", "dem_active_count", "dem_get_stats", "dem_get_instance_stats",
     "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
     "dem_get_pairs", "dem_get_contacts", "dem_get_forces", "dem_device_info", "dem_read_deck",
@@ -188,6 +189,10 @@ def lib() -> ctypes.CDLL:
         "dem_set_state": ([P, P, P, P, P, P, P, c_int], c_int),
         "dem_get_state": ([P, P, P, P, P, c_int], c_int),
         "dem_get_wall_history": ([P, P, P], c_int),
+        "dem_save_checkpoint": ([P, ctypes.c_char_p], c_int),
+        "dem_load_checkpoint": ([P, ctypes.c_char_p], c_int),
+        "dem_read_restart": ([ctypes.c_char_p, P, P, P], c_int),
+        "dem_write_restart_resume": ([ctypes.c_char_p, c_int, c_int, c_int], c_int),
         "dem_step": ([P, c_int], c_int),
         "dem_step_graph": ([P, c_int], c_int),
         "dem_request_rebuild": ([P], c_int),
@@ -591,6 +596,14 @@ class Context:
         _check(self._L.dem_get_wall_history(self._h, _ptr(d), _ptr(f)), "dem_get_wall_history")
         return d, f
 
+    def save_checkpoint(self, path):
+        """Full-state checkpoint (state, pair list and histories, grids, counters) to `path`."""
+        _check(self._L.dem_save_checkpoint(self._h, os.fsencode(path)), "dem_save_checkpoint")
+
+    def load_checkpoint(self, path):
+        """Replace this context's state by a checkpoint made for the same instances."""
+        _check(self._L.dem_load_checkpoint(self._h, os.fsencode(path)), "dem_load_checkpoint")
+
     def step(self, nsteps: int = 1):
         _check(self._L.dem_step(self._h, int(nsteps)), "dem_step")
 
@@ -608,6 +621,9 @@ class Context:
 
     def set_active(self, n: int):
         _check(self._L.dem_set_active(self._h, int(n)), "dem_set_active")
+
+    def active_count(self) -> int:
+        return int(self._L.dem_active_count(self._h))
 
     def request_rebuild(self):
         _check(self._L.dem_request_rebuild(self._h), "dem_request_rebuild")
@@ -818,6 +834,22 @@ def deck_material(d: Deck) -> Material:
 
 def write_restart(path: str):
     _check(lib().dem_write_restart(path.encode()), "dem_write_restart")
+
+
+def read_restart(path: str):
+    """(nrestart, npregas, nparticledat) from restart.dat (dempacking.cu:361-366), or None when
+    the file does not start with three integers (the end-of-run file: start fresh)."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    code = lib().dem_read_restart(path.encode(), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    if code == DEM_ERR_INVALID:
+        return None
+    _check(code, "dem_read_restart")
+    return a.value, b.value, c.value
+
+
+def write_restart_resume(path: str, nrestart: int, npregas: int, nparticledat: int):
+    _check(lib().dem_write_restart_resume(path.encode(), nrestart, npregas, nparticledat),
+           "dem_write_restart_resume")
 
 
 def __getattr__(name):

@@ -1,13 +1,23 @@
 // gpusphere.cpp -- DEM packing driver: the reference's main() flow (dempacking.cu:28-918) on
 // top of libdem.so's context API.
 //
-//   gpusphere [hopp3d.inp] [--tstop N] [--quiet]
+//   gpusphere [hopp3d.inp] [--tstop N] [--quiet] [--checkpoint-every N] [--max-cpu-seconds S]
 //
 // Reads the positional deck, converts to reduced units, builds the layer lattice with the
 // reference's glibc rand() velocities, inserts one layer every tnewpar = 0.5 time units,
 // steps with the displacement-triggered list rebuild, and writes view.dat, ppor.dat (every
 // 1000 steps), timesteps.dat, particle.dat (step 1 and every 50000), jobfinished and
 // restart.dat in the reference's formats.  State stays in HBM; no per-layer reallocation.
+//
+// Checkpoint and resume (the reference's preflow.dat / restart.dat scheme, dempacking.cu:
+// 358-367, 794-875, finished): a checkpoint writes preflow.dat in the reference's text format
+// and preflow.bin, the library's lossless full-state checkpoint (pair list and contact
+// histories included), then restart.dat as "0 0 nparticledat".  It is taken every N steps
+// with --checkpoint-every N, at the reference's CPU-time marks (3600 s, then every 7200 s,
+// checked every 1000 steps), and before the reference's CPU-time stop (41400 s, or
+// --max-cpu-seconds), after which the driver exits with status 3.  A run that finds
+// restart.dat starting with nrestart = 0 loads preflow.bin and continues from the step after
+// it; the steps it runs are the uninterrupted run's steps, bit for bit.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -41,15 +51,40 @@ void write_particles(const char *path, int zone, const std::vector<double> &pos,
     std::fclose(fp);
 }
 
+// preflow.dat (dempacking.cu:812-820), preflow.bin and restart.dat for a resume after step it.
+int write_checkpoint(dem_ctx *ctx, unsigned long long it, unsigned nparticles, int nparticledat,
+                     std::vector<double> &pos, std::vector<double> &pdis, std::vector<double> &angv)
+{
+    int rc = dem_get_state(ctx, pos.data(), pdis.data(), angv.data(), nullptr, 0);
+    if (rc) return fail("dem_get_state", rc);
+    if (std::FILE *fp = std::fopen("preflow.dat", "wb+")) {
+        std::fprintf(fp, "%8d,%8d,%8d\n", (int)it, (int)nparticles, 0);
+        for (unsigned i = 0; i < nparticles; i++)
+            std::fprintf(fp, "%15.6f %15.6f %15.6f %15.6f %15.6f %15.6f %15.6f %15.6f %15.6f\n",
+                         pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], pdis[3 * i], pdis[3 * i + 1],
+                         pdis[3 * i + 2], angv[3 * i], angv[3 * i + 1], angv[3 * i + 2]);
+        std::fclose(fp);
+    } else {
+        std::printf("Cannot open file preflow.dat!");
+    }
+    if ((rc = dem_save_checkpoint(ctx, "preflow.bin"))) return fail("dem_save_checkpoint", rc);
+    if ((rc = dem_write_restart_resume("restart.dat", 0, 0, nparticledat)))
+        return fail("dem_write_restart_resume", rc);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     const char *deck_path = "hopp3d.inp";
-    long long tstop_override = -1;
+    long long tstop_override = -1, ckpt_every = 0;
+    double max_cpu = 41400.0;   // dempacking.cu:841
     bool quiet = false;
     for (int a = 1; a < argc; a++) {
         if (!std::strcmp(argv[a], "--tstop") && a + 1 < argc) tstop_override = std::atoll(argv[++a]);
+        else if (!std::strcmp(argv[a], "--checkpoint-every") && a + 1 < argc) ckpt_every = std::atoll(argv[++a]);
+        else if (!std::strcmp(argv[a], "--max-cpu-seconds") && a + 1 < argc) max_cpu = std::atof(argv[++a]);
         else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
         else deck_path = argv[a];
     }
@@ -173,9 +208,23 @@ int main(int argc, char **argv)
 
     unsigned nparticles = 0;
     int nparticledat = 0;
+    unsigned long long it0 = 1;
+    {
+        int nrestart = 1, npregas = 0, ndat = 0;
+        if (dem_read_restart("restart.dat", &nrestart, &npregas, &ndat) == DEM_OK && nrestart == 0) {
+            if ((rc = dem_load_checkpoint(ctx, "preflow.bin"))) return fail("dem_load_checkpoint", rc);
+            dem_stats st;
+            if ((rc = dem_get_stats(ctx, &st))) return fail("dem_get_stats", rc);
+            it0 = st.steps + 1;
+            nparticles = dem_active_count(ctx);
+            nparticledat = ndat;
+            std::printf("resuming at step %llu with %u particles\n", it0, nparticles);
+        }
+    }
     const double tnewpar = 0.50;
+    double realtime = 3600.0;   // dempacking.cu:161
     std::clock_t start = std::clock();
-    for (unsigned long long it = 1; it <= tstop; it++) {
+    for (unsigned long long it = it0; it <= tstop; it++) {
         double time = dt * (double)it;
         if (nparticles < ntot && time > (float)nparticles * tnewpar / float(nplayer)) {   // :378-381
             unsigned nn = nparticles + (unsigned)nplayer;
@@ -210,6 +259,22 @@ int main(int argc, char **argv)
                 return fail("dem_get_state", rc);
             nparticledat++;
             write_particles("particle.dat", nparticledat, pos, rad, nparticles, diam);
+        }
+        bool ckpt = ckpt_every > 0 && it % (unsigned long long)ckpt_every == 0 && it < tstop;
+        bool stop = false;
+        if (it % 1000 == 0) {   // dempacking.cu:805-823, 837-841
+            double tret = (double)(std::clock() - start) / CLOCKS_PER_SEC;
+            if (tret >= realtime) {
+                ckpt = true;
+                realtime += 7200.0;
+            }
+            if (tret > max_cpu && it < tstop) ckpt = stop = true;
+        }
+        if (ckpt && write_checkpoint(ctx, it, nparticles, nparticledat, pos, pdis, angv)) return 1;
+        if (stop) {
+            std::printf("CPU-time limit reached after step %llu: checkpoint written\n", it);
+            dem_destroy(ctx);
+            return 3;
         }
     }
     if (std::FILE *fp = std::fopen("jobfinished", "wb")) {   // dempacking.cu:904-907

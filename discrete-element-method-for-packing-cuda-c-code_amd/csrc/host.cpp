@@ -178,6 +178,30 @@ int dem_write_restart(const char *path)
     return DEM_OK;
 }
 
+int dem_read_restart(const char *path, int *nrestart, int *npregas, int *nparticledat)
+{
+    if (!path || !nrestart || !npregas || !nparticledat) return DEM_ERR_INVALID;
+    std::FILE *fp = std::fopen(path, "rb");   // dempacking.cu:361-366
+    if (!fp) return DEM_ERR_IO;
+    int a = 0, b = 0, c = 0;
+    int got = std::fscanf(fp, "%d %d %d", &a, &b, &c);
+    std::fclose(fp);
+    if (got != 3) return DEM_ERR_INVALID;   // e.g. the end-of-run text
+    *nrestart = a;
+    *npregas = b;
+    *nparticledat = c;
+    return DEM_OK;
+}
+
+int dem_write_restart_resume(const char *path, int nrestart, int npregas, int nparticledat)
+{
+    if (!path) return DEM_ERR_INVALID;
+    std::FILE *fp = std::fopen(path, "wb");   // the write left commented at dempacking.cu:829
+    if (!fp) return DEM_ERR_IO;
+    std::fprintf(fp, "%8d %8d %8d", nrestart, npregas, nparticledat);
+    return std::fclose(fp) == 0 ? DEM_OK : DEM_ERR_IO;
+}
+
 double dem_restitution_for_damping(double g)
 {
     if (!(g >= 0.0)) return -1.0;

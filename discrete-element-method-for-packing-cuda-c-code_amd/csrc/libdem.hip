@@ -601,6 +601,7 @@ InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
 // skin, i.e. the 2 r_max part is kept) and rebuild the grids over the same extents.  The
 // physics does not depend on the skin (pairs outside contact contribute nothing); the
 // rebuild trigger |q| > skin/2 does.
+int install_grid(dem_ctx *c, unsigned cell_off);
 int regrid(dem_ctx *c, double skin)
 {
     if (!(skin > 0.0)) return DEM_ERR_INVALID;
@@ -622,6 +623,12 @@ int regrid(dem_ctx *c, double skin)
         I.cell_off = cell_off;
         cell_off += (unsigned)cells;
     }
+    return install_grid(c, cell_off);
+}
+
+// Upload the host instance table and size the cell arrays for `cell_off` cells in total.
+int install_grid(dem_ctx *c, unsigned cell_off)
+{
     DEM_HIP(hipStreamSynchronize(c->stream));
     DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
     if (cell_off != c->ncells) {
@@ -1141,6 +1148,335 @@ int dem_rebuild(dem_ctx *c)
     int rc = rebuild_host(c);
     if (rc) return rc;
     return DEM_OK;
+}
+
+// =========================================================================================
+// Full dynamic state: restore from a snapshot (device) or a checkpoint file (host)
+// =========================================================================================
+
+namespace {
+
+// The dynamic state of a context between steps: records, wall histories, inverse slot map
+// and control block, and the live pair list with its transposed index, wall candidates and
+// tangential histories.  Restoring it (and nothing else) makes the following steps repeat
+// the original run bit for bit: the slot order, the list and the summation order are the
+// original's.
+struct StateSrc {
+    const PRec *rec = nullptr;
+    const Hist *whist = nullptr, *hist = nullptr;
+    const unsigned *inv = nullptr, *off = nullptr, *toff = nullptr, *jt = nullptr,
+                   *wflag = nullptr, *wlist = nullptr;
+    const uint2 *ab = nullptr;
+    const DevCtl *ctl = nullptr;
+    size_t nrec = 0;          // records, wall histories and inverse-map entries to copy
+    bool both = false;        // copy them into both buffers (slots of inactive particles)
+    unsigned npairs = 0, n_list = 0, since = 0, last_flag = 0;
+    bool have_list = false, need_rebuild = false;
+    unsigned long long steps = 0, rebuilds = 0;
+};
+
+int restore_state(dem_ctx *c, const StateSrc &S, hipMemcpyKind kind)
+{
+    hipStream_t st = c->stream;
+    size_t n = c->n, np = S.have_list ? S.npairs : 0;
+    int rc;
+    DEM_HIP(hipStreamSynchronize(st));
+    unsigned grow = (unsigned)(np * 3 / 2 + 1024);
+    if (np > c->cap[c->lc] && (rc = alloc_list(c, c->lc, grow))) return rc;
+    for (int k = 0; k < 2; k++)
+        if (np > c->hcap[k] && (rc = alloc_hist(c, k, grow))) return rc;
+    if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
+    for (int k = 0; k < (S.both ? 2 : 1); k++) {
+        int r = S.both ? k : c->rc, p = S.both ? k : c->pc;
+        DEM_HIP(hipMemcpyAsync(c->rec[r], S.rec, sizeof(PRec) * S.nrec, kind, st));
+        DEM_HIP(hipMemcpyAsync(c->whist[p], S.whist, sizeof(Hist) * S.nrec, kind, st));
+        DEM_HIP(hipMemcpyAsync(c->inv[p], S.inv, sizeof(unsigned) * S.nrec, kind, st));
+    }
+    c->zs_ok = false;
+    c->out_valid = false;
+    hipLaunchKernelGGL(k_cpos_from_rec, dim3(cdiv(S.nrec, kBlock)), dim3(kBlock), 0, st,
+                       (const PRec *)c->rec[c->rc], (unsigned)S.nrec, c->cpos[c->rc], c->cp32[c->rc]);
+    DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), kind, st));
+    if (S.have_list) {
+        DEM_HIP(hipMemcpyAsync(c->off[c->lc], S.off, sizeof(unsigned) * (n + 1), kind, st));
+        DEM_HIP(hipMemcpyAsync(c->toff, S.toff, sizeof(unsigned) * (n + 1), kind, st));
+        DEM_HIP(hipMemcpyAsync(c->wflag, S.wflag, sizeof(unsigned) * n, kind, st));
+        DEM_HIP(hipMemcpyAsync(c->wlist, S.wlist, sizeof(unsigned) * n, kind, st));
+        if (np) {
+            DEM_HIP(hipMemcpyAsync(c->ab[c->lc], S.ab, sizeof(uint2) * np, kind, st));
+            DEM_HIP(hipMemcpyAsync(c->hist[c->hc], S.hist, sizeof(Hist) * np, kind, st));
+            DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, kind, st));
+            hipLaunchKernelGGL(k_owner, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, st,
+                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->jt, (unsigned)np,
+                               c->jf, c->bo);
+        }
+    }
+    DEM_HIP(hipGetLastError());
+    // host sources must outlive the copies
+    if (kind == hipMemcpyHostToDevice) DEM_HIP(hipStreamSynchronize(st));
+    c->npairs = S.npairs;
+    c->n_list = S.n_list;
+    c->since = S.since;
+    c->last_flag = S.last_flag;
+    c->have_list = S.have_list;
+    c->need_rebuild = S.need_rebuild;
+    c->steps = S.steps;
+    c->rebuilds = S.rebuilds;
+    drop_graph(c);
+    return DEM_OK;
+}
+
+// Checkpoint file: a fixed header, the instance table, then the state arrays in the
+// library's slot layout, then an FNV-1a 64-bit checksum of everything before it.
+constexpr char kCkptMagic[8] = {'D', 'E', 'M', 'C', 'K', 'P', 'T', '1'};
+constexpr unsigned kCkptVersion = 1, kCkptEndian = 0x01020304u;
+
+struct CkptHeader {
+    char magic[8];
+    unsigned version, endian;
+    unsigned sz_rec, sz_hist, sz_inst, sz_ctl;
+    unsigned ninst, cap_n, n, npairs;
+    unsigned have_list, need_rebuild, since, last_flag;
+    unsigned n_list, interval, pad0, pad1;
+    unsigned long long steps, rebuilds;
+};
+static_assert(sizeof(CkptHeader) == 96, "checkpoint header layout");
+
+struct Fnv64 {
+    unsigned long long h = 1469598103934665603ull;
+    void add(const void *p, size_t bytes)
+    {
+        const unsigned char *b = static_cast<const unsigned char *>(p);
+        for (size_t k = 0; k < bytes; k++) {
+            h ^= b[k];
+            h *= 1099511628211ull;
+        }
+    }
+};
+
+struct CkptWriter {
+    std::FILE *fp;
+    Fnv64 sum;
+    bool ok = true;
+    void put(const void *p, size_t bytes)
+    {
+        if (!bytes) return;
+        sum.add(p, bytes);
+        ok = ok && std::fwrite(p, 1, bytes, fp) == bytes;
+    }
+};
+
+struct CkptReader {
+    std::FILE *fp;
+    Fnv64 sum;
+    bool ok = true;
+    void get(void *p, size_t bytes)
+    {
+        if (!bytes || !ok) return;
+        ok = std::fread(p, 1, bytes, fp) == bytes;
+        if (ok) sum.add(p, bytes);
+    }
+};
+
+// Host-side consistency of a list read from a file, so a damaged file can never make a
+// kernel index out of bounds: CSR offsets monotone and ending at npairs, slots < n, transposed
+// entries pairs or empty, wall candidates < n.
+bool ckpt_list_ok(unsigned n, unsigned np, const std::vector<unsigned> &off,
+                  const std::vector<unsigned> &toff, const std::vector<uint2> &ab,
+                  const std::vector<unsigned> &jt, const std::vector<unsigned> &wlist,
+                  unsigned nwall)
+{
+    if (off[0] != 0 || toff[0] != 0 || off[n] != np || toff[n] != np || nwall > n) return false;
+    for (unsigned s = 0; s < n; s++)
+        if (off[s] > off[s + 1] || toff[s] > toff[s + 1]) return false;
+    for (unsigned k = 0; k < np; k++) {
+        if (ab[k].x >= n || ab[k].y >= n) return false;
+        if (jt[k] >= np && jt[k] != kEmpty) return false;
+    }
+    for (unsigned k = 0; k < nwall; k++)
+        if (wlist[k] >= n) return false;
+    return true;
+}
+
+}  // namespace
+
+int dem_save_checkpoint(dem_ctx *c, const char *path)
+{
+    if (!c || !path || !*path) return DEM_ERR_INVALID;
+    DEM_HIP(hipStreamSynchronize(c->stream));
+    const size_t N = c->cap_n, n = c->n;
+    const size_t np = c->have_list ? c->npairs : 0;
+    std::vector<PRec> rec(N);
+    std::vector<Hist> wh(N), hist(np);
+    std::vector<unsigned> inv(N), off, toff, wflag, wlist, jt(np);
+    std::vector<uint2> ab(np);
+    DevCtl ctl;
+    DEM_HIP(hipMemcpy(rec.data(), c->rec[c->rc], sizeof(PRec) * N, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(wh.data(), c->whist[c->pc], sizeof(Hist) * N, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(inv.data(), c->inv[c->pc], sizeof(unsigned) * N, hipMemcpyDeviceToHost));
+    DEM_HIP(hipMemcpy(&ctl, c->ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+    if (c->have_list) {
+        off.resize(n + 1);
+        toff.resize(n + 1);
+        wflag.resize(n);
+        wlist.resize(n);
+        DEM_HIP(hipMemcpy(off.data(), c->off[c->lc], sizeof(unsigned) * (n + 1), hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(toff.data(), c->toff, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(wflag.data(), c->wflag, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+        DEM_HIP(hipMemcpy(wlist.data(), c->wlist, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+        if (np) {
+            DEM_HIP(hipMemcpy(ab.data(), c->ab[c->lc], sizeof(uint2) * np, hipMemcpyDeviceToHost));
+            DEM_HIP(hipMemcpy(hist.data(), c->hist[c->hc], sizeof(Hist) * np, hipMemcpyDeviceToHost));
+            DEM_HIP(hipMemcpy(jt.data(), c->jt, sizeof(unsigned) * np, hipMemcpyDeviceToHost));
+        }
+    }
+    CkptHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kCkptMagic, sizeof(h.magic));
+    h.version = kCkptVersion;
+    h.endian = kCkptEndian;
+    h.sz_rec = sizeof(PRec);
+    h.sz_hist = sizeof(Hist);
+    h.sz_inst = sizeof(InstDev);
+    h.sz_ctl = sizeof(DevCtl);
+    h.ninst = (unsigned)c->ninst;
+    h.cap_n = (unsigned)N;
+    h.n = (unsigned)n;
+    h.npairs = (unsigned)np;
+    h.have_list = c->have_list;
+    h.need_rebuild = c->need_rebuild;
+    h.since = c->since;
+    h.last_flag = c->last_flag;
+    h.n_list = c->n_list;
+    h.interval = (unsigned)c->interval;
+    h.steps = c->steps;
+    h.rebuilds = c->rebuilds;
+    // write to a side file and rename it over the target: an interrupted save leaves the
+    // previous checkpoint intact
+    std::vector<char> tmp(std::strlen(path) + 8);
+    std::snprintf(tmp.data(), tmp.size(), "%s.part", path);
+    std::FILE *fp = std::fopen(tmp.data(), "wb");
+    if (!fp) return DEM_ERR_IO;
+    CkptWriter w{fp};
+    w.put(&h, sizeof(h));
+    w.put(c->hinst.data(), sizeof(InstDev) * c->hinst.size());
+    w.put(rec.data(), sizeof(PRec) * N);
+    w.put(wh.data(), sizeof(Hist) * N);
+    w.put(inv.data(), sizeof(unsigned) * N);
+    w.put(&ctl, sizeof(ctl));
+    if (c->have_list) {
+        w.put(off.data(), sizeof(unsigned) * (n + 1));
+        w.put(toff.data(), sizeof(unsigned) * (n + 1));
+        w.put(wflag.data(), sizeof(unsigned) * n);
+        w.put(wlist.data(), sizeof(unsigned) * n);
+        w.put(ab.data(), sizeof(uint2) * np);
+        w.put(hist.data(), sizeof(Hist) * np);
+        w.put(jt.data(), sizeof(unsigned) * np);
+    }
+    unsigned long long sum = w.sum.h;
+    bool ok = w.ok && std::fwrite(&sum, sizeof(sum), 1, fp) == 1;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.data(), path) != 0) {
+        std::remove(tmp.data());
+        return DEM_ERR_IO;
+    }
+    return DEM_OK;
+}
+
+int dem_load_checkpoint(dem_ctx *c, const char *path)
+{
+    if (!c || !path) return DEM_ERR_INVALID;
+    std::FILE *fp = std::fopen(path, "rb");
+    if (!fp) return DEM_ERR_IO;
+    CkptReader r{fp};
+    CkptHeader h;
+    r.get(&h, sizeof(h));
+    auto bad = [&](int code) {
+        std::fclose(fp);
+        return code;
+    };
+    if (!r.ok || std::memcmp(h.magic, kCkptMagic, sizeof(h.magic)) != 0 ||
+        h.version != kCkptVersion || h.endian != kCkptEndian || h.sz_rec != sizeof(PRec) ||
+        h.sz_hist != sizeof(Hist) || h.sz_inst != sizeof(InstDev) || h.sz_ctl != sizeof(DevCtl))
+        return bad(DEM_ERR_IO);
+    // the context must have been created for the same instances (particle counts)
+    if (h.ninst != (unsigned)c->ninst || h.cap_n != c->cap_n || h.n == 0 || h.n > h.cap_n ||
+        (c->ninst > 1 && h.n != h.cap_n) || h.npairs > 0xfffffff0u || (!h.have_list && h.npairs))
+        return bad(DEM_ERR_INVALID);
+    std::vector<InstDev> inst(h.ninst);
+    r.get(inst.data(), sizeof(InstDev) * h.ninst);
+    if (!r.ok) return bad(DEM_ERR_IO);
+    unsigned long long cells = 0;
+    for (unsigned i = 0; i < h.ninst; i++) {
+        const InstDev &a = inst[i], &b = c->hinst[i];
+        if (a.p_off != b.p_off || a.n != b.n || a.cell_off != cells) return bad(DEM_ERR_INVALID);
+        cells += (unsigned long long)a.gsize[0] * a.gsize[1] * a.gsize[2];
+        if (cells >= 0xffffffffull) return bad(DEM_ERR_INVALID);
+    }
+    const size_t N = h.cap_n, n = h.n, np = h.npairs;
+    std::vector<PRec> rec(N);
+    std::vector<Hist> wh(N), hist(np);
+    std::vector<unsigned> inv(N), off, toff, wflag, wlist, jt(np);
+    std::vector<uint2> ab(np);
+    DevCtl ctl;
+    r.get(rec.data(), sizeof(PRec) * N);
+    r.get(wh.data(), sizeof(Hist) * N);
+    r.get(inv.data(), sizeof(unsigned) * N);
+    r.get(&ctl, sizeof(ctl));
+    if (h.have_list) {
+        off.resize(n + 1);
+        toff.resize(n + 1);
+        wflag.resize(n);
+        wlist.resize(n);
+        r.get(off.data(), sizeof(unsigned) * (n + 1));
+        r.get(toff.data(), sizeof(unsigned) * (n + 1));
+        r.get(wflag.data(), sizeof(unsigned) * n);
+        r.get(wlist.data(), sizeof(unsigned) * n);
+        r.get(ab.data(), sizeof(uint2) * np);
+        r.get(hist.data(), sizeof(Hist) * np);
+        r.get(jt.data(), sizeof(unsigned) * np);
+    }
+    unsigned long long want = r.sum.h, sum = 0;
+    bool ok = r.ok && std::fread(&sum, sizeof(sum), 1, fp) == 1 && sum == want;
+    std::fclose(fp);
+    if (!ok) return DEM_ERR_IO;
+    for (size_t s = 0; s < N; s++)
+        if (rec[s].id >= N || inv[s] >= N || rec[s].inst >= h.ninst) return DEM_ERR_IO;
+    if (h.have_list && !ckpt_list_ok((unsigned)n, (unsigned)np, off, toff, ab, jt, wlist, ctl.nwall))
+        return DEM_ERR_IO;
+
+    adj_free(c);   // tapes and checkpoints of another trajectory
+    c->hinst = inst;
+    int rc = install_grid(c, (unsigned)cells);
+    if (rc) return rc;
+    c->n = h.n;
+    c->interval = (int)h.interval;
+    c->last_path = 0;
+    StateSrc S;
+    S.rec = rec.data();
+    S.whist = wh.data();
+    S.inv = inv.data();
+    S.ctl = &ctl;
+    S.nrec = N;
+    S.both = true;
+    if (h.have_list) {
+        S.off = off.data();
+        S.toff = toff.data();
+        S.wflag = wflag.data();
+        S.wlist = wlist.data();
+        S.ab = ab.data();
+        S.hist = hist.data();
+        S.jt = jt.data();
+    }
+    S.npairs = h.npairs;
+    S.n_list = h.n_list;
+    S.since = h.since;
+    S.last_flag = h.last_flag;
+    S.have_list = h.have_list != 0;
+    S.need_rebuild = h.need_rebuild != 0;
+    S.steps = h.steps;
+    S.rebuilds = h.rebuilds;
+    return restore_state(c, S, hipMemcpyHostToDevice);
 }
 
 namespace {
@@ -2296,47 +2632,28 @@ int snap_save(dem_ctx *c, AdjSnap &S)
 
 int snap_restore(dem_ctx *c, const AdjSnap &S)
 {
-    hipStream_t st = c->stream;
-    size_t n = c->n, np = S.npairs;
-    int rc;
-    DEM_HIP(hipStreamSynchronize(st));
-    unsigned grow = (unsigned)(np * 3 / 2 + 1024);
-    if (np > c->cap[c->lc] && (rc = alloc_list(c, c->lc, grow))) return rc;
-    for (int k = 0; k < 2; k++)
-        if (np > c->hcap[k] && (rc = alloc_hist(c, k, grow))) return rc;
-    if (np > c->out_cap && (rc = alloc_out(c, (unsigned)(np * 3 / 2 + 1024)))) return rc;
-    DEM_HIP(hipMemcpyAsync(c->rec[c->rc], S.rec, sizeof(PRec) * n, hipMemcpyDeviceToDevice, st));
-    c->zs_ok = false;
-    DEM_HIP(hipMemcpyAsync(c->whist[c->pc], S.whist, sizeof(Hist) * n, hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_cpos_from_rec, dim3(cdiv((unsigned)n, kBlock)), dim3(kBlock), 0, st,
-                       (const PRec *)c->rec[c->rc], (unsigned)n, c->cpos[c->rc], c->cp32[c->rc]);
-    DEM_HIP(hipMemcpyAsync(c->inv[c->pc], S.inv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
-    DEM_HIP(hipMemcpyAsync(c->ctl, S.ctl, sizeof(DevCtl), hipMemcpyDeviceToDevice, st));
-    if (S.have_list) {
-        DEM_HIP(hipMemcpyAsync(c->off[c->lc], S.off, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
-        DEM_HIP(hipMemcpyAsync(c->toff, S.toff, sizeof(unsigned) * (n + 1), hipMemcpyDeviceToDevice, st));
-        DEM_HIP(hipMemcpyAsync(c->wflag, S.wflag, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
-        DEM_HIP(hipMemcpyAsync(c->wlist, S.wlist, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, st));
-        if (np) {
-            DEM_HIP(hipMemcpyAsync(c->ab[c->lc], S.ab, sizeof(uint2) * np, hipMemcpyDeviceToDevice, st));
-            DEM_HIP(hipMemcpyAsync(c->hist[c->hc], S.hist, sizeof(Hist) * np, hipMemcpyDeviceToDevice, st));
-            DEM_HIP(hipMemcpyAsync(c->jt, S.jt, sizeof(unsigned) * np, hipMemcpyDeviceToDevice, st));
-            hipLaunchKernelGGL(k_owner, dim3(cdiv(np, kBlock)), dim3(kBlock), 0, st,
-                               (const uint2 *)c->ab[c->lc], (const unsigned *)c->jt, (unsigned)np,
-                               c->jf, c->bo);
-            DEM_HIP(hipGetLastError());
-        }
-    }
-    c->npairs = S.npairs;
-    c->n_list = S.n_list;
-    c->since = S.since;
-    c->last_flag = S.last_flag;
-    c->have_list = S.have_list;
-    c->need_rebuild = S.need_rebuild;
-    c->steps = S.steps;
-    c->rebuilds = S.rebuilds;
-    drop_graph(c);
-    return DEM_OK;
+    StateSrc V;
+    V.rec = S.rec;
+    V.whist = S.whist;
+    V.inv = S.inv;
+    V.ctl = S.ctl;
+    V.nrec = c->n;
+    V.off = S.off;
+    V.toff = S.toff;
+    V.wflag = S.wflag;
+    V.wlist = S.wlist;
+    V.ab = S.ab;
+    V.hist = S.hist;
+    V.jt = S.jt;
+    V.npairs = S.npairs;
+    V.n_list = S.n_list;
+    V.since = S.since;
+    V.last_flag = S.last_flag;
+    V.have_list = S.have_list;
+    V.need_rebuild = S.need_rebuild;
+    V.steps = S.steps;
+    V.rebuilds = S.rebuilds;
+    return restore_state(c, V, hipMemcpyDeviceToDevice);
 }
 
 int list_reserve(AdjList &L, size_t n, size_t np)

@@ -245,6 +245,19 @@ int dem_get_state(dem_ctx *ctx, double *pos, double *pdis, double *angv, double 
 /* Per-particle wall history (disptw AoS double3, fricpw) in index order. */
 int dem_get_wall_history(dem_ctx *ctx, double *disptw, double *fricpw);
 
+/* Full-state checkpoint, the binary, lossless replacement of the reference's preflow.dat
+ * (dempacking.cu:805-823, which keeps pos/pdis/angv at 6 decimals and no contact history):
+ * records of every particle (active or waiting), wall histories, the live pair list with
+ * its tangential histories, the instance table (grids, materials), step counters, the
+ * rebuild state and the rebuild interval.  Steps after dem_load_checkpoint repeat the
+ * original run's steps bit for bit.  Save writes `path`.part and renames it over `path`.
+ * Load needs a context created for the same instances (same particle counts); it replaces
+ * the state, the grids and materials, and drops any adjoint recording.  Returns DEM_ERR_IO
+ * for a missing, truncated or corrupted file (checksum), DEM_ERR_INVALID for a context of
+ * other sizes. */
+int dem_save_checkpoint(dem_ctx *ctx, const char *path);
+int dem_load_checkpoint(dem_ctx *ctx, const char *path);
+
 /* Single-instance contexts: make particles [0, n) active; the rest of the state set by
  * dem_set_state waits, as the reference's not-yet-inserted layers do (dempacking.cu:378-462).
  * Any n before the first step after dem_set_state, only growth afterwards.  The next step
@@ -402,6 +415,14 @@ int dem_read_deck(const char *path, dem_deck *deck);
 int dem_deck_material(const dem_deck *deck, dem_material *mat);
 /* Writes restart.dat exactly as dempacking.cu:908-913 (40 bytes). */
 int dem_write_restart(const char *path);
+/* Reads restart.dat as the reference's disabled reader does (fscanf "%d %d %d",
+ * dempacking.cu:361-366): nrestart 0 means "resume from the checkpoint".  DEM_ERR_IO if the
+ * file is missing, DEM_ERR_INVALID if it does not start with three integers (the end-of-run
+ * file written by dem_write_restart), in which case a run starts fresh. */
+int dem_read_restart(const char *path, int *nrestart, int *npregas, int *nparticledat);
+/* Writes restart.dat in that format ("%8d %8d %8d", the write left commented out at
+ * dempacking.cu:829), so that the next run resumes when nrestart is 0. */
+int dem_write_restart_resume(const char *path, int nrestart, int npregas, int nparticledat);
 /* Normal damping coefficient giving restitution e for the Hertz + dashpot law (bisection
  * on the head-on collision ODE).  Returns the coefficient, or -1 on bad input. */
 double dem_damping_for_restitution(double e);

@@ -68,6 +68,24 @@ def test_restart_file_bytes(dem, tmp_path):
     assert len(p.read_bytes()) == 40
 
 
+def test_restart_reader_end_of_run_file_means_fresh_start(dem):
+    """The shipped end-of-run restart.dat does not start with three integers: the reference's
+    fscanf("%d %d %d") (dempacking.cu:365) matches nothing, so no resume."""
+    assert dem.read_restart(os.path.join(GOLDEN, "restart.dat")) is None
+
+
+def test_restart_resume_round_trip(dem, tmp_path):
+    p = tmp_path / "restart.dat"
+    dem.write_restart_resume(str(p), 0, 7, 12)
+    assert p.read_bytes() == b"       0        7       12"   # "%8d %8d %8d"
+    assert dem.read_restart(str(p)) == (0, 7, 12)
+    p.write_text("1 2 3\n")
+    assert dem.read_restart(str(p)) == (1, 2, 3)
+    with pytest.raises(dem.DemError) as ei:
+        dem.read_restart(str(tmp_path / "missing.dat"))
+    assert ei.value.code == dem.DEM_ERR_IO
+
+
 def test_lognormal_diameters(dem):
     d = dem.gen_diameters(1, 200000, 0.1, 0.7, 1.3, seed=2)
     assert d.min() >= 0.7 and d.max() <= 1.3
