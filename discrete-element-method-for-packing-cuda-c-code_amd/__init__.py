@@ -50,6 +50,7 @@ OPT_TIME_STEPS = 10
 OPT_PIPELINE = 11
 OPT_FUSED_FAST = 12
 OPT_XCD_CHUNK = 13
+OPT_PIPELINE_BATCH = 14
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0

@@ -228,8 +228,9 @@ struct dem_ctx {
     bool timing = false;
     // queued steps under the displacement trigger: up to `pipeline` steps in flight, each
     // reporting its outcome in a host-mapped slot (2 = not run)
-    static constexpr int kQueue = 8;
-    int pipeline = 3;
+    static constexpr int kQueue = 16;
+    int pipeline = 16;
+    int pipeline_batch = 8;   // DEM_OPT_PIPELINE_BATCH: queued steps per recorded event
     hipEvent_t evq[kQueue] = {};
     unsigned *qflag_h = nullptr, *qflag_d = nullptr;
     double t_acc[2][2] = {{0, 0}, {0, 0}};   // [path][first kernel group, per-particle kernel]
@@ -994,6 +995,12 @@ int dem_set_option(dem_ctx *c, int option, double value)
         c->pipeline = d;
         return DEM_OK;
     }
+    case DEM_OPT_PIPELINE_BATCH: {
+        int b = (int)value;
+        if (b < 1 || b > dem_ctx::kQueue || (double)b != value) return DEM_ERR_INVALID;
+        c->pipeline_batch = b;
+        return DEM_OK;
+    }
     case DEM_OPT_TIME_STEPS:
         if (value != 0.0 && !c->ring[0].e[0]) {
             for (auto &T : c->ring) {
@@ -1487,15 +1494,19 @@ namespace {
 // sets); a step that raises the trigger writes 1 into its host-mapped slot.  When the host
 // sees the trigger it drains the queue, undoes the host-side buffer flips of the steps that
 // did not run, rebuilds and issues them again.  The steps that run, and their results, are
-// the synchronous loop's.
+// the synchronous loop's.  Steps are issued in batches of `pipeline_batch` with one event
+// after each batch (every packet between two dependent kernels lengthens the gap between
+// them); the queue is refilled when a whole batch fits.
 int step_queued(dem_ctx *c, int nsteps)
 {
     struct Issued {
         int rc, hc, last_path;
         bool out_valid;
+        int ev;   // queue slot of the event recorded after this step's batch
     };
     Issued q[dem_ctx::kQueue];
     const int depth = c->pipeline;
+    const int batch = std::min(c->pipeline_batch, depth);
     const unsigned *gate = &c->ctl->skip;
     volatile unsigned *flag = c->qflag_h;
     int done = 0, head = 0, inflight = 0;
@@ -1535,16 +1546,20 @@ int step_queued(dem_ctx *c, int nsteps)
                 continue;
             }
         }
-        while (inflight < depth && done + inflight < nsteps) {
-            int k = (head + inflight) % dem_ctx::kQueue;
-            q[k] = Issued{c->rc, c->hc, c->last_path, c->out_valid};
-            flag[k] = 0u;   // set to 1 by the step if it raises the trigger
-            launch_force_step(c, pair_grid(c, c->npairs), nullptr, nullptr, gate, c->qflag_d + k);
-            DEM_HIP(hipEventRecord(c->evq[k], c->stream));
-            inflight++;
+        while (inflight + batch <= depth && done + inflight < nsteps) {
+            const int nb = std::min(batch, nsteps - done - inflight);
+            const int kl = (head + inflight + nb - 1) % dem_ctx::kQueue;   // the batch's last
+            for (int i = 0; i < nb; i++) {
+                int k = (head + inflight) % dem_ctx::kQueue;
+                q[k] = Issued{c->rc, c->hc, c->last_path, c->out_valid, kl};
+                flag[k] = 0u;   // set to 1 by the step if it raises the trigger
+                launch_force_step(c, pair_grid(c, c->npairs), nullptr, nullptr, gate, c->qflag_d + k);
+                inflight++;
+            }
+            DEM_HIP(hipEventRecord(c->evq[kl], c->stream));
         }
         DEM_HIP(hipGetLastError());
-        DEM_HIP(hipEventSynchronize(c->evq[head]));
+        DEM_HIP(hipEventSynchronize(c->evq[q[head].ev]));
         unsigned f = flag[head];   // the oldest queued step always runs
         head = (head + 1) % dem_ctx::kQueue;
         inflight--;
@@ -1555,8 +1570,8 @@ int step_queued(dem_ctx *c, int nsteps)
             // the trigger: the queued steps after this one did nothing -- drain them, undo
             // their host-side buffer flips, and rebuild before issuing them again
             if (inflight > 0) {
-                int last = (head + inflight - 1) % dem_ctx::kQueue;
-                DEM_HIP(hipEventSynchronize(c->evq[last]));
+                int last = (head + inflight - 1) % dem_ctx::kQueue;   // ends a batch
+                DEM_HIP(hipEventSynchronize(c->evq[q[last].ev]));
                 const Issued &u = q[head];
                 c->rc = u.rc;
                 c->hc = u.hc;

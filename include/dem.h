@@ -205,15 +205,19 @@ enum {
     DEM_OPT_TIME_STEPS = 10,       /* 1: dem_step records HIP events around its kernels
                                       (read with dem_get_step_timing) */
     DEM_OPT_PIPELINE = 11,         /* steps dem_step keeps in flight under the displacement
-                                      trigger (1..8, default 3; 1 = wait for each step) */
+                                      trigger (1..16, default 16; 1 = wait for each step) */
     DEM_OPT_FUSED_FAST = 12,       /* fused step: a particle with at most this many candidates
                                       on each side (own pairs, pairs where it is second) tests
                                       both sides in one pass and runs one law loop (0..64,
                                       default 64); others go through 32-candidate chunks, own
                                       side first.  Same results bit for bit */
-    DEM_OPT_XCD_CHUNK = 13         /* step kernels: the 8 XCDs take chunks of this many
+    DEM_OPT_XCD_CHUNK = 13,        /* step kernels: the 8 XCDs take chunks of this many
                                       blocks in turn (default 8; 0: one contiguous eighth of
                                       the particles or pairs each).  Same results bit for bit */
+    DEM_OPT_PIPELINE_BATCH = 14    /* queued steps issued together behind one HIP event
+                                      (1..16, at most DEM_OPT_PIPELINE; default 8): an event
+                                      between two kernels adds ~4 us to the gap between them.
+                                      Same results bit for bit */
 };
 
 /* Result of dem_autotune. */

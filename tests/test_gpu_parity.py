@@ -293,13 +293,16 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
 def test_queued_steps_equal_synchronous(dem, gpu_ok):
     """Under the displacement trigger dem_step queues steps ahead of the host (a queued step
     after a trigger does nothing and is issued again after the rebuild); the steps run, the
-    rebuilds and the results equal the synchronous loop's bit for bit, whatever the depth."""
+    rebuilds and the results equal the synchronous loop's bit for bit, whatever the depth and
+    however many steps share one event."""
     w = dense_workload(dem, n=4000, seed=52)
     w.pdis = w.pdis * 100.0   # frequent triggers
     res = []
-    for depth in (1, 2, 3, 8):
+    for depth, batch in ((1, 1), (2, 1), (3, 1), (8, 1), (8, 4), (6, 3), (8, 8), (5, 2), (16, 8),
+                         (16, 16)):
         c = dem.Context(w.instances)
         c.set_option(dem.OPT_PIPELINE, depth)
+        c.set_option(dem.OPT_PIPELINE_BATCH, batch)
         c.load(w)
         for k in (1, 7, 50, 142):   # calls of several lengths
             c.step(k)
