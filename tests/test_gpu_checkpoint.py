@@ -223,3 +223,16 @@ def test_gpusphere_resume_through_restart_dat(dem, gpu_ok, tmp_path):
     for f in ("ppor.dat", "particle.dat", "restart.dat", "jobfinished"):
         assert (split / f).read_bytes() == (full / f).read_bytes(), f
     assert (full / "restart.dat").read_bytes() == open(os.path.join(golden, "restart.dat"), "rb").read()
+    # periodic checkpoints (--checkpoint-every) leave the run unchanged; the last one is of
+    # step 2100 (all 10,000 particles in), and the finished run's restart.dat is the end-of-run
+    # text again
+    every = tmp_path / "every"
+    every.mkdir()
+    shutil.copy(os.path.join(golden, "hopp3d.inp"), every / "hopp3d.inp")
+    r = subprocess.run([exe, "--tstop", "2500", "--quiet", "--checkpoint-every", "700"], cwd=every,
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    for f in ("ppor.dat", "particle.dat", "restart.dat", "jobfinished"):
+        assert (every / f).read_bytes() == (full / f).read_bytes(), f
+    assert (every / "preflow.dat").read_text().startswith("    2100,   10000,       0\n")
+    assert (every / "preflow.bin").stat().st_size > 0

@@ -345,6 +345,38 @@ def test_batched_equals_plain(dem, gpu_ok):
     assert np.all(cnt > 0)
 
 
+def test_full_size_batched_256_equals_per_instance(dem, gpu_ok):
+    """The batched configuration at full size (256 instances, N log-uniform in [1K, 64K],
+    per-instance materials, sum N = 4.5M): 10 graph groups of 20 steps in one batched context
+    end, instance by instance, bit-identical to each instance stepped in a plain context of
+    its own on the same fixed rebuild interval (the batched kernels read per-instance
+    constants; the slot order inside an instance is the plain context's)."""
+    K, steps = 20, 200
+    w = dem.config3(rebuild_interval=K)
+    say(f"[c3] {len(w.instances)} instances, n={w.n}")
+    big = dem.Context(w.instances)
+    big.load(w)
+    big.set_option(dem.OPT_REBUILD_INTERVAL, K)
+    big.step_graph(steps)
+    sb = big.state()
+    z, cnt = big.instance_stats()
+    big.close()
+    assert np.all(cnt > 0), "every instance must carry contacts"
+    off = 0
+    for k, inst in enumerate(w.instances):
+        sl = slice(off, off + inst.n)
+        c = dem.Context([inst])
+        c.set_state(w.pos[sl], w.pdis[sl], w.angv[sl], w.rad[sl], w.rmass[sl], w.inert[sl])
+        c.set_option(dem.OPT_REBUILD_INTERVAL, K)
+        c.step_graph(steps)
+        s = c.state()
+        c.close()
+        for key in ("pos", "pdis", "angv", "qi"):
+            assert np.array_equal(sb[key][sl], s[key]), (k, key)
+        off += inst.n
+    say(f"[c3] all {len(w.instances)} instances bit-identical")
+
+
 def test_pair_capacity_grows(dem, orc, gpu_ok):
     w = dense_workload(dem, n=1500, seed=17)
     ctx = dem.Context(w.instances, pair_capacity=64)
