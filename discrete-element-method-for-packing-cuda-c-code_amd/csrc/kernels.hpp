@@ -339,18 +339,14 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
 // Pair list of slot a: its partners with a larger particle index, in ascending slot order,
 // at [off[a], off[a+1]) -- each pair is stored once, first member = smaller particle index,
 // exactly the reference's Lp < Ln orientation (dempacking.cu:640-644), while segments follow
-// the cell-sorted slot order.  Tangential history is carried over by pair identity (the
-// reference's pairlistD remap, particleSystem.cu:143-191): the old list is searched under
-// the members' old slots.  Partners with a smaller index go to the transposed list, to be
-// resolved to pair indices by k_transpose.
+// the cell-sorted slot order.  Partners with a smaller index go to the transposed list, to be
+// resolved to pair indices by k_transpose.  The tangential histories are filled by k_remap.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const double4 *cpos, const unsigned *cid, const uint2 *sorted, unsigned n,
     const InstDev *inst, InstDev one,
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
-    unsigned cap, uint2 *ab, unsigned *bo, Hist *hist, unsigned *jt, const unsigned *inv_old,
-    const uint2 *ab_old, const unsigned *off_old, const Hist *hist_old, unsigned n_old,
-    unsigned cap_old, DevCtl *ctl, unsigned *src, const unsigned *active)
+    unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, DevCtl *ctl, const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -359,26 +355,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     unsigned pg = off[a];
     unsigned pl = toff[a];
     unsigned ida = cid[a];
-    unsigned oa = (inv_old && ida < n_old) ? inv_old[ida] : kEmpty;
     bool over = false;
     for_candidates(cpos, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
-        unsigned idb = cid[b];
-        if (idb > ida) {
+        if (cid[b] > ida) {
             if (pg >= cap) { over = true; return; }
             ab[pg] = make_uint2(a, b);
             bo[pg] = b;
-            Hist h = Hist{0.0, 0.0, 0.0, 0.0};
-            unsigned q = kEmpty;
-            if (oa != kEmpty) {
-                unsigned ob = idb < n_old ? inv_old[idb] : kEmpty;
-                if (ob != kEmpty) {
-                    unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
-                    q = find_second(ab_old, lo, hi, ob);
-                    if (q != kEmpty) h = hist_old[q];
-                }
-            }
-            hist[pg] = h;
-            if (src) src[pg] = q;   // recorded for the adjoint of the remap
             pg++;
         } else {
             if (pl < cap) jt[pl] = b;
@@ -386,6 +368,46 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
         }
     });
     if (over && ctl) atomicOr(&ctl->overflow, 1u);
+}
+
+// Tangential history of each pair of a new list, one lane per pair: the history of the same
+// particle pair in the old list, searched under the members' old slots, else zero -- the
+// reference's pairlistD remap (particleSystem.cu:143-191; new particles have empty old
+// segments, dempacking.cu:204,521-522).  Consecutive lanes take consecutive pairs, mostly
+// of one first member and so of one old segment, and write whole history lines.  Each
+// block takes a contiguous, XCD-aware range of the device pair count (clamped to the
+// capacity), so a fixed grid serves a captured rebuild.  src (optional) records each pair's
+// old index for the adjoint of the remap.
+__global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl *ctl, unsigned cap,
+                                                  const unsigned *cid, Hist *hist,
+                                                  const unsigned *inv_old, const uint2 *ab_old,
+                                                  const unsigned *off_old, const Hist *hist_old,
+                                                  unsigned n_old, unsigned cap_old, unsigned *src,
+                                                  const unsigned *active)
+{
+    if (!is_active(active)) return;
+    const unsigned np = min(ctl->npairs, cap);
+    const unsigned per = (np + gridDim.x - 1) / gridDim.x;
+    const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned p_end = min(p_begin + per, np);
+    for (unsigned p = p_begin + threadIdx.x; p < p_end; p += blockDim.x) {
+        const uint2 e = ab[p];
+        Hist h = Hist{0.0, 0.0, 0.0, 0.0};
+        unsigned q = kEmpty;
+        if (inv_old) {
+            const unsigned ida = cid[e.x], idb = cid[e.y];
+            if (ida < n_old && idb < n_old) {
+                const unsigned oa = inv_old[ida], ob = inv_old[idb];
+                if (oa < n_old && ob != kEmpty) {
+                    const unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
+                    q = find_second(ab_old, lo, hi, ob);
+                    if (q != kEmpty) h = hist_old[q];
+                }
+            }
+        }
+        hist[p] = h;
+        if (src) src[p] = q;
+    }
 }
 
 // Replace each transposed entry (partner slot b, smaller particle index) by the index of

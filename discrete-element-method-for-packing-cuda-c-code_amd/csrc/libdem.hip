@@ -414,16 +414,20 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
     unsigned capo = std::min(std::min(c->cap[ol], c->out_cap), c->hcap[oh]);
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->hist[nh], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
-                           c->tape_src, active);
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->ctl, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
-                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid, (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->hist[nh], c->jt, inv_old,
-                           c->ab[ol], c->off[ol], c->hist[oh], c->n_list, capo, c->ctl,
-                           c->tape_src, active);
+                           (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->ctl, active);
+    // histories by pair identity, one lane per pair over the device pair count
+    const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
+    hipLaunchKernelGGL(k_remap, dim3(rg), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
+                       (const DevCtl *)c->ctl, capn, (const unsigned *)c->cid, c->hist[nh], inv_old,
+                       (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
+                       (const Hist *)c->hist[oh], c->n_list, capo, c->tape_src, active);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
                        c->jf, active);
