@@ -116,10 +116,12 @@ def cpu_baseline(dem, w, budget_s):
                       f"step (list build {t_first:.1f} s, excluded), oracle C/OpenMP"}
 
 
-def bench_batched(dem, steps, interval, tune_steps=48):
+def bench_batched(dem, steps, interval, tune_steps=48, graph=True):
     """Batched form (256 instances, hipGraph-captured, autotuned skin / launch shapes / rebuild
     interval) against the same instances stepped as per-instance plain contexts (same skin and
-    fixed rebuild interval): host loop, and hipGraph replays."""
+    fixed rebuild interval): host loop, and hipGraph replays.  graph=False steps the batched
+    context with the host loop too and skips the per-instance graph replays (the same kernels,
+    for profilers that cannot trace graph launches)."""
     t0 = time.perf_counter()
     w = dem.config3(rebuild_interval=interval)
     log(f"batched workload: {len(w.instances)} instances, n={w.n} ({time.perf_counter() - t0:.1f} s)")
@@ -135,10 +137,11 @@ def bench_batched(dem, steps, interval, tune_steps=48):
         big.set_option(dem.OPT_REBUILD_INTERVAL, interval)
         log(f"autotune: {tune}")
     steps = max(interval, (steps // interval) * interval)
-    big.step_graph(2 * interval)
+    adv = big.step_graph if graph else big.step
+    adv(2 * interval)
     big.sync()
     t0 = time.perf_counter()
-    big.step_graph(steps)
+    adv(steps)
     big.sync()
     tb = time.perf_counter() - t0
     nb = w.n
@@ -159,25 +162,29 @@ def bench_batched(dem, steps, interval, tune_steps=48):
         ctxs.append(c)
     for c in ctxs:
         c.sync()
+    log(f"batched {nb * steps / tb:.4e} p-steps/s; {len(ctxs)} per-instance contexts ready")
     t0 = time.perf_counter()
     for c in ctxs:
         c.step(steps)
     tp = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for c in ctxs:
-        c.step_graph(steps)
-    tg = time.perf_counter() - t0
-    vb, vp, vg = nb * steps / tb, nb * steps / tp, nb * steps / tg
-    log(f"batched {vb:.4e} p-steps/s; per-instance plain {vp:.4e} (graph {vg:.4e}); "
-        f"ratio {vb / vp:.2f}x ({vb / vg:.2f}x vs per-instance graphs)")
+    tg = None
+    if graph:
+        t0 = time.perf_counter()
+        for c in ctxs:
+            c.step_graph(steps)
+        tg = time.perf_counter() - t0
+    vb, vp = nb * steps / tb, nb * steps / tp
+    vg = nb * steps / tg if tg else None
+    log(f"batched {vb:.4e} p-steps/s; per-instance plain {vp:.4e}; ratio {vb / vp:.2f}x"
+        + (f" (per-instance graphs {vg:.4e}: {vb / vg:.2f}x)" if vg else ""))
     return {"workload": w.name, "instances": len(w.instances), "n": nb, "steps": steps,
             "rebuild_interval": interval, "value": vb, "per_instance_plain": vp,
-            "per_instance_graph": vg, "ratio_vs_plain": vb / vp, "ratio_vs_plain_graph": vb / vg,
+            "per_instance_graph": vg, "ratio_vs_plain": vb / vp, "ratio_vs_plain_graph": vb / vg if vg else None,
             "ms_per_step": tb / steps * 1e3, "trigger_in_last_group": int(flag),
             "autotune": tune}
 
 
-def dense_bed(dem, w, settle_steps, load=None, save=None):
+def dense_bed(dem, w, settle_steps, load=None, save=None, graph=True):
     """The 500K workload poured into a bed on the GPU (graph steps), or loaded from a file
     a previous run saved (so a profiler can run the dense window alone)."""
     if load:
@@ -188,7 +195,7 @@ def dense_bed(dem, w, settle_steps, load=None, save=None):
         log(f"dense window: bed loaded from {load}")
         return ws
     t0 = time.perf_counter()
-    ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0)
+    ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0, graph=graph)
     log(f"dense window: bed poured in {time.perf_counter() - t0:.1f} s (top {ws.meta['bed_top']:.1f})")
     if save:
         np.savez(save, pos=ws.pos, pdis=ws.pdis, angv=ws.angv, settle_steps=settle_steps)
@@ -196,9 +203,10 @@ def dense_bed(dem, w, settle_steps, load=None, save=None):
     return ws
 
 
-def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False, load=None, save=None):
+def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False, load=None, save=None,
+                graph=True):
     """Dense window: `steps` plain steps, displacement trigger, from the poured bed."""
-    ws = dense_bed(dem, w, settle_steps, load, save)
+    ws = dense_bed(dem, w, settle_steps, load, save, graph)
     ctx = dem.Context(ws.instances)
     if split is not None:
         ctx.set_option(dem.OPT_FUSED_STEP, 0 if split else 1)
@@ -224,14 +232,14 @@ def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False, load=None
     return out
 
 
-def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4):
+def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True):
     """Differentiable form on the 20K-sphere bed: checkpointed forward + adjoint backward
     against the plain forward (cost ratio), and the adjoint gradient of phi and CN in
     (mu, log E, e, mu_r) against central differences of the plain forward."""
     import copy
     A = dem.adjoint
     t0 = time.perf_counter()
-    w = dem.settle_bed(dem.config4(), tap=0.0)
+    w = dem.settle_bed(dem.config4(), tap=0.0, graph=graph)
     log(f"adjoint workload: n={w.n}, bed top {w.meta['bed_top']:.2f} ({time.perf_counter() - t0:.1f} s)")
     obs = dem.make_obs(zhi=w.meta["bed_top"] - 3.0, margin=2.0, zlo=2.0, eps=0.5, eps_c=1e-3)
     prob = A.PackingProblem(w, nsteps, every, obs)
@@ -246,6 +254,7 @@ def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4):
     info = prob.ctx.adj_info()
     g_cn = prob.backward([0.0], [1.0])[0]
     assert phi[0] == phi_p[0] and cn[0] == cn_p[0]
+    log(f"adjoint: forward {t_fwd:.3f} s, backward {t_bwd:.3f} s; central differences next")
     r = A.fd_check(prob, [th0], rel=fd_rel, which=(0, 1, 3))
     base = prob.mmap.material(th0)
     gam = base.p.dampnp
@@ -292,13 +301,13 @@ def norm_err(adj, fd, theta):
 
 
 def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, iters=16,
-                      seed=7):
+                      seed=7, graph=True):
     """Calibration of (mu, log E, e, mu_r) to the observables of a bed run by the plain method
     at seeded hidden parameters: `ninst` seeded starts in one batched context (replicas of one
     settled bed), one forward + adjoint backward per iteration."""
     A = dem.adjoint
     t0 = time.perf_counter()
-    w1 = dem.settle_bed(dem.config4(n=n, side=side, seed=seed), tap=0.0)
+    w1 = dem.settle_bed(dem.config4(n=n, side=side, seed=seed), tap=0.0, graph=graph)
     obs = dem.make_obs(zhi=w1.meta["bed_top"] - 2.0, margin=1.5, zlo=1.5, eps=0.5, eps_c=1e-3)
     rng = np.random.default_rng(seed)
 
@@ -371,6 +380,9 @@ def main():
     ap.add_argument("--split", action="store_true",
                     help="force the split path (pair kernel + gather); default: auto")
     ap.add_argument("--fused", action="store_true", help="force the fused step; default: auto")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="host-loop steps wherever the legs use hipGraph replays (same kernels, "
+                         "same results; for profilers that cannot trace graph launches)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run 3 device-to-device copies of this many bytes (the known-size "
                          "kernel that scales the HBM traffic counters)")
@@ -398,7 +410,8 @@ def main():
     split_opt = True if args.split else (False if args.fused else None)
     if args.dense_only:
         d = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
-                        split=split_opt, load=args.dense_load, save=args.dense_save)
+                        split=split_opt, load=args.dense_load, save=args.dense_save,
+                        graph=not args.no_graph)
         log(json.dumps(d))
         return
     ctx = dem.Context(w.instances)
@@ -478,10 +491,12 @@ def main():
     dense = None
     if not args.no_dense:
         dense = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
-                            split=split_opt, load=args.dense_load, save=args.dense_save)
+                            split=split_opt, load=args.dense_load, save=args.dense_save,
+                            graph=not args.no_graph)
     batched = None
     if not args.no_batched:
-        batched = bench_batched(dem, args.batched_steps, args.batched_interval)
+        batched = bench_batched(dem, args.batched_steps, args.batched_interval,
+                                graph=not args.no_graph)
     adjoint = None
     if not args.no_adjoint:
         log("importing torch for the autograd layer (the first import on a fresh machine can "
@@ -489,10 +504,10 @@ def main():
         t0 = time.perf_counter()
         import torch  # noqa: F401
         log(f"torch imported ({time.perf_counter() - t0:.1f} s)")
-        adjoint = bench_adjoint(dem)
+        adjoint = bench_adjoint(dem, graph=not args.no_graph)
         if not args.no_calibration:
             adjoint["calibration"] = bench_calibration(dem, ninst=args.calib_instances,
-                                                       n=args.calib_n)
+                                                       n=args.calib_n, graph=not args.no_graph)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dem, w, args.cpu_seconds)

@@ -512,11 +512,12 @@ def config4(n: int = 20_000, seed: int = 5, phi0: float = 0.3, side: float = 30.
 
 
 def settle_bed(w: Workload, steps: int = 100_000, interval: int = 50, tap: float = 0.5,
-               seed: int = 6) -> Workload:
-    """Pour `w` into a bed on the GPU (hipGraph steps, fixed rebuild interval: no particle
-    moves half the skin in `interval` steps of free fall from the fill height), then 'tap'
-    it: every particle gets a random velocity of scale `tap` (in sqrt(g d)) and contact
-    histories restart at zero.  Returns the tapped bed as a new workload."""
+               seed: int = 6, graph: bool = True) -> Workload:
+    """Pour `w` into a bed on the GPU (hipGraph steps, or the host loop with graph=False:
+    the same steps bit for bit; fixed rebuild interval: no particle moves half the skin in
+    `interval` steps of free fall from the fill height), then 'tap' it: every particle gets
+    a random velocity of scale `tap` (in sqrt(g d)) and contact histories restart at zero.
+    Returns the tapped bed as a new workload."""
     ctx = Context(w.instances)
     ctx.set_state(w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
     ctx.set_option(OPT_REBUILD_INTERVAL, interval)
@@ -525,7 +526,10 @@ def settle_bed(w: Workload, steps: int = 100_000, interval: int = 50, tap: float
     left = (steps // interval) * interval
     while left > 0:
         k = min(chunk, left)
-        ctx.step_graph(k)
+        if graph:
+            ctx.step_graph(k)
+        else:
+            ctx.step(k)
         left -= k
     st = ctx.state()
     ctx.close()

@@ -204,12 +204,21 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     if (!is_active(active)) return;
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
     bool cand = false;
+    unsigned id = 0, o = 0;
+    double2 v[8];   // the record, eight 16-byte chunks
+    Hist w = Hist{0.0, 0.0, 0.0, 0.0};
     if (s < n) {
-        unsigned id = sorted[s].y;
-        unsigned o = inv_old[id];
-        const PRec &r = rec_old[o];
-        const InstDev &I = BATCHED ? inst[r.inst] : one;
-        cand = wall_reach(I.walls, r.px, r.py, r.pz, r.rad, 0.5 * I.skin) > 0.0;
+        id = sorted[s].y;
+        o = inv_old[id];
+        // every load of the record in flight before any store (the stores cannot be moved
+        // above loads of a buffer the compiler cannot tell apart from theirs)
+        const double *src = reinterpret_cast<const double *>(rec_old + o);
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = ld2(src + 2 * k);
+        w = wh_old[o];
+        const unsigned ri = (unsigned)(unsigned long long)__double_as_longlong(v[7].y);   // inst
+        const InstDev &I = BATCHED ? inst[ri] : one;
+        cand = wall_reach(I.walls, v[0].x, v[0].y, v[1].x, v[1].y, 0.5 * I.skin) > 0.0;
         wflag[s] = cand ? 1u : 0u;
     }
     // wave-aggregated append to the wall-candidate list
@@ -223,28 +232,19 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
         if (cand) wlist[base + __popcll(m & ((1ull << lane) - 1ull))] = s;
     }
     if (s >= n) return;
-    unsigned id = sorted[s].y;
-    unsigned o = inv_old[id];
     if (perm) perm[s] = o;   // recorded for the adjoint of the reorder
-    const double *src = reinterpret_cast<const double *>(rec_old + o);
     double *dst = reinterpret_cast<double *>(rec_new + s);
 #pragma unroll
-    for (int k = 0; k < 12; k += 2) {
-        double2 v = ld2(src + k);
-        st2(dst + k, v.x, v.y);
-    }
-    st2(dst + 12, 0.0, 0.0);
-    uint2 tail = *reinterpret_cast<const uint2 *>(src + 15);
-    *reinterpret_cast<double *>(dst + 14) = 0.0;
-    *reinterpret_cast<uint2 *>(dst + 15) = tail;
-    wh_new[s] = wh_old[o];
+    for (int k = 0; k < 6; k++) st2(dst + 2 * k, v[k].x, v[k].y);
+    st2(dst + 12, 0.0, 0.0);     // q restarts at zero
+    st2(dst + 14, 0.0, v[7].y);  // (instance, index) copied as raw bits
+    wh_new[s] = w;
     inv_new[id] = s;
     // compact copies for the candidate searches of this rebuild (a 32-byte position and
     // radius, a 4-byte index: four candidates per cache line instead of one)
-    double2 p01 = ld2(src), p23 = ld2(src + 2);
-    cpos[s] = make_double4(p01.x, p01.y, p23.x, p23.y);
-    cp32[s] = pos32(p01.x, p01.y, p23.x, p23.y);
-    cid[s] = tail.y;
+    cpos[s] = make_double4(v[0].x, v[0].y, v[1].x, v[1].y);
+    cp32[s] = pos32(v[0].x, v[0].y, v[1].x, v[1].y);
+    cid[s] = (unsigned)((unsigned long long)__double_as_longlong(v[7].y) >> 32);
 }
 
 // Slot range [s0, s1) of the up-to-three consecutive cells of one stencil row.  Cells of a
