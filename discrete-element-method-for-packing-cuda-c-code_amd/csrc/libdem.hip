@@ -1453,6 +1453,8 @@ int dem_load_checkpoint(dem_ctx *c, const char *path)
     if (!ok) return DEM_ERR_IO;
     for (size_t s = 0; s < N; s++)
         if (rec[s].id >= N || inv[s] >= N || rec[s].inst >= h.ninst) return DEM_ERR_IO;
+    // the kernels bound their pair loops by the control block's count and gate on its zero word
+    if (ctl.npairs != h.npairs || ctl.overflow != 0u || ctl.zero != 0u) return DEM_ERR_IO;
     if (h.have_list && !ckpt_list_ok((unsigned)n, (unsigned)np, off, toff, ab, jt, wlist, ctl.nwall))
         return DEM_ERR_IO;
 

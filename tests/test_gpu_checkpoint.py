@@ -91,6 +91,24 @@ def test_resume_repeats_the_run(dem, gpu_ok, tmp_path, mode):
     assert_same(ref, snapshot(c))
 
 
+def test_checkpoint_before_any_step_into_a_fresh_context(dem, gpu_ok, tmp_path):
+    """A checkpoint taken before the first step (no list yet) loaded into a context that was
+    never given a state: the file alone initialises it, and the run that follows is the
+    original's."""
+    w = compressed_lattice(dem, n=700, seed=8)
+    path = str(tmp_path / "start.bin")
+    a = dem.Context(w.instances)
+    a.load(w)
+    a.save_checkpoint(path)
+    a.step(120)
+    ref = snapshot(a)
+    b = dem.Context(w.instances)
+    b.load_checkpoint(path)
+    assert b.stats().steps == 0
+    b.step(120)
+    assert_same(ref, snapshot(b))
+
+
 def test_resume_with_waiting_layers(dem, gpu_ok, tmp_path):
     """A checkpoint taken while layers still wait (dem_set_active) keeps their state; the
     resumed run inserts them on the same schedule and ends as the uninterrupted one."""
