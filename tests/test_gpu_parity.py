@@ -479,6 +479,82 @@ def test_dropin_reference_sequence(dem, orc, gpu_ok):
     assert int(cpair.host()[:Pn].sum()) > n, "drop-in workload must be in contact"
 
 
+def test_invalid_arguments_and_capacity_errors(dem, gpu_ok):
+    """Error paths return a status instead of ending the process (the reference's gpuErrchk
+    exit()s, particleSystem.cu:15-23): bad instance descriptions, option values, step counts
+    and active counts give DEM_ERR_INVALID and leave the context usable; a neighbour row too
+    short for a particle's candidates gives DEM_ERR_CAPACITY through dem_last_error, which the
+    read clears, and no entry is written past the row."""
+    import ctypes
+    w = dense_workload(dem, n=800, seed=71)
+    inst = w.instances[0]
+
+    def variant(f):
+        b = type(inst)()
+        ctypes.pointer(b)[0] = inst
+        f(b)
+        return b
+
+    edits = (lambda b: setattr(b, "n", 0), lambda b: setattr(b, "dt", 0.0),
+             lambda b: setattr(b, "skin", -0.1), lambda b: b.gsize.__setitem__(0, 0),
+             lambda b: b.cell.__setitem__(1, 0.0))
+    for f in edits:
+        with pytest.raises(dem.DemError) as ei:
+            dem.Context([variant(f)])
+        assert ei.value.code == dem.DEM_ERR_INVALID
+    c = dem.Context([inst])
+    c.load(w)
+    c.step(5)
+    for opt, val in ((dem.OPT_PAIR_BLOCK, 100), (dem.OPT_PARTICLE_BLOCK, 32),
+                     (dem.OPT_FUSED_STEP, 3), (dem.OPT_FUSED_STEP, 0.5), (dem.OPT_PIPELINE, 0),
+                     (dem.OPT_PIPELINE, 17), (dem.OPT_PIPELINE_BATCH, 0),
+                     (dem.OPT_PIPELINE_BATCH, 17), (dem.OPT_XCD_CHUNK, -1),
+                     (dem.OPT_FUSED_FAST, 65), (dem.OPT_REBUILD_INTERVAL, -1),
+                     (dem.OPT_SKIN, 0.0), (dem.OPT_PAIR_GRID, 0), (99, 1.0)):
+        with pytest.raises(dem.DemError) as ei:
+            c.set_option(opt, val)
+        assert ei.value.code == dem.DEM_ERR_INVALID, (opt, val)
+    calls = (lambda: c.step(-1), lambda: c.step_graph(10), lambda: c.set_active(0),
+             lambda: c.set_active(inst.n + 1), lambda: c.set_active(inst.n - 100))
+    for f in calls:
+        with pytest.raises(dem.DemError) as ei:
+            f()
+        assert ei.value.code == dem.DEM_ERR_INVALID
+    c.set_option(dem.OPT_REBUILD_INTERVAL, 20)
+    with pytest.raises(dem.DemError) as ei:
+        c.step_graph(30)
+    assert ei.value.code == dem.DEM_ERR_INVALID
+    c.step(10)
+    assert c.stats().steps == 15 and np.all(np.isfinite(c.state()["pos"]))
+    two = dem.Context([inst, inst])
+    with pytest.raises(dem.DemError) as ei:
+        two.set_active(10)
+    assert ei.value.code == dem.DEM_ERR_INVALID
+    # drop-in neighbour rows of width 2 on a compressed lattice (about 12 candidates each)
+    L = dem.lib()
+    D = dem.DeviceArray
+    n = w.n
+    C = inst.gsize[0] * inst.gsize[1] * inst.gsize[2]
+    pos = D.from_host(w.pos)
+    hash0, hash1 = D(2 * n, np.uint32), D(2 * n, np.uint32)
+    cstart = D(C, np.uint32)
+    guard = D(2 * n + 64, np.uint32)   # the rows, then a guard band that must stay zero
+    njgi, njli = D(n, np.uint32), D(n, np.uint32)
+    G = (ctypes.c_uint * 3)(*inst.gsize)
+    cell = (ctypes.c_double * 3)(*inst.cell)
+    orig = (ctypes.c_double * 3)(*inst.origin)
+    L.dem_last_error()   # start from a clear error word
+    L.calcHash(pos.p, hash0.p, G, cell, orig, n)
+    L.RadixSort(hash0.p, hash1.p, n, 32)
+    L.findCellStart(hash0.p, cstart.p, n, C)
+    L.neighborarray(pos.p, hash0.p, cstart.p, G, cell, orig, 2, n, guard.p, njgi.p, njli.p)
+    assert L.dem_last_error() == dem.DEM_ERR_CAPACITY
+    assert L.dem_last_error() == 0
+    g = guard.host()
+    assert np.all(g[2 * n:] == 0), "neighborarray wrote past its rows"
+    assert int(njgi.host().sum() + njli.host().sum()) > 4 * n
+
+
 def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
     """The 500K lognormal cylinder workload at full size: lists bit-exact, one step's forces
     and state within 1e-10 of the oracle, pair-force sum zero (Newton's third law)."""
