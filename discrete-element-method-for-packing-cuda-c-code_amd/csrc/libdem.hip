@@ -479,9 +479,9 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
 }
 
 // One step.  Split path: k_pair_force (one lane per pair, history in place, 80-byte output
-// per pair) -> k_walls -> k_particle (gather, integrate in place).  Fused path: k_walls ->
-// k_step (one lane per particle, both sides of each pair evaluated, records and histories
-// from the current buffers into the other ones; the buffer indices flip).  `mid` is
+// per pair) -> k_walls -> k_particle (gather, integrate in place).  Fused path: k_step alone
+// (one lane per particle, wall contacts included, both sides of each pair evaluated, records
+// and histories from the current buffers into the other ones; the buffer indices flip).  `mid` is
 // recorded before the per-particle kernel, `end` after it.
 void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                        hipEvent_t end = nullptr, const unsigned *gate = nullptr,
@@ -532,7 +532,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         if (queued) c->tpar = 1 - par;
         c->zs_ok = true;
         // the other record buffer takes the whole updated record; wall histories are per
-        // slot and updated in place by k_walls, so they move with the records
+        // slot and updated in place by the step's wall contact, so they move with the records
         if (c->batched)
             hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
