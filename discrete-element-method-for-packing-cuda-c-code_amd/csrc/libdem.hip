@@ -258,7 +258,7 @@ struct dem_ctx {
     unsigned char *mirror_h = nullptr, *mirror_d = nullptr;
 
     int interval = 0;
-    unsigned pair_block = 256, pair_grid_per_cu = 16, particle_block = 256;   // launch shapes
+    unsigned pair_block = 256, pair_grid_per_cu = 16, particle_block = 128;   // launch shapes
     unsigned long long steps = 0, rebuilds = 0;
     unsigned since = 0;
     bool need_rebuild = true;

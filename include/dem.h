@@ -193,7 +193,8 @@ enum {
     DEM_OPT_SKIN = 4,              /* list skin (cell edge follows); requests a rebuild */
     DEM_OPT_PAIR_BLOCK = 5,        /* threads per block of the pair kernel: 64, 128, 256 */
     DEM_OPT_PAIR_GRID = 6,         /* pair kernel blocks per CU (grid-stride) */
-    DEM_OPT_PARTICLE_BLOCK = 7,    /* threads per block of the per-particle kernel */
+    DEM_OPT_PARTICLE_BLOCK = 7,    /* threads per block of the per-particle kernels: 64, 128
+                                      (default) or 256 */
     DEM_OPT_ADJ_ROLL_REG = 8,      /* adjoint: floor of |w_rel| in the rolling moment's
                                       linearisation, in units of the one-step chatter scale
                                       mu_r fcn dt^2 (1/I_a + 1/I_b); 0 = exact, default 1 */
