@@ -138,3 +138,29 @@ def test_config0_workload(dem):
         d = np.linalg.norm(p - p[i], axis=1)
         d[i] = 9
         assert d.min() >= 1.0
+
+
+def test_header_constants_match_the_binding(dem):
+    """Option numbers and status codes of include/dem.h are the ones the Python binding uses."""
+    hdr = open(os.path.join(ROOT, "include", "dem.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    opts = dict((m.group(1), int(m.group(2)))
+                for m in re.finditer(r"\bDEM_OPT_([A-Z_]+)\s*=\s*(\d+)", hdr))
+    assert len(opts) >= 13
+    for name, val in opts.items():
+        assert getattr(dem, "OPT_" + name) == val, name
+    errs = dict((m.group(1), int(m.group(2)))
+                for m in re.finditer(r"\bDEM_(OK|ERR_[A-Z_]+)\s*=\s*(\d+)", hdr))
+    assert errs["OK"] == 0 and len(errs) >= 6
+    for name, val in errs.items():
+        assert getattr(dem, "DEM_" + name) == val, name
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with a message before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
