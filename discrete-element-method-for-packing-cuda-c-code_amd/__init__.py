@@ -158,7 +158,7 @@ EXPORTS = [
     "calculateforcepair", "sumallforcei", "calculateforceiw", "updateposition",
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
     "dem_get_state", "dem_get_wall_history", "dem_save_checkpoint", "dem_load_checkpoint",
-    "dem_read_restart", "dem_write_restart_resume", "dem_step", "dem_step_graph",
+    "dem_read_restart", "dem_write_restart_resume", "dem_read_sidecar", "dem_step", "dem_step_graph",
     "dem_request_rebuild", "dem_rebuild", "dem_autotune", "dem_set_active", "dem_active_count", "dem_get_stats", "dem_get_instance_stats",
     "dem_sync", "dem_last_ms", "dem_time_kernels", "dem_get_hash", "dem_get_sorted", "dem_get_cell_start",
     "dem_get_pairs", "dem_get_contacts", "dem_get_forces", "dem_device_info", "dem_read_deck",
@@ -193,6 +193,7 @@ def lib() -> ctypes.CDLL:
         "dem_save_checkpoint": ([P, ctypes.c_char_p], c_int),
         "dem_load_checkpoint": ([P, ctypes.c_char_p], c_int),
         "dem_read_restart": ([ctypes.c_char_p, P, P, P], c_int),
+        "dem_read_sidecar": ([ctypes.c_char_p, P], c_int),
         "dem_write_restart_resume": ([ctypes.c_char_p, c_int, c_int, c_int], c_int),
         "dem_step": ([P, c_int], c_int),
         "dem_step_graph": ([P, c_int], c_int),
@@ -839,6 +840,25 @@ def deck_material(d: Deck) -> Material:
 
 def write_restart(path: str):
     _check(lib().dem_write_restart(path.encode()), "dem_write_restart")
+
+
+class Sidecar(ctypes.Structure):
+    """dem_sidecar: the deck side-car's knobs (include/dem.h)."""
+    _fields_ = [
+        ("walls", c_int), ("particles", c_int), ("box", c_double * 3), ("radius", c_double),
+        ("height", c_double), ("diameters", c_int), ("insertion", c_int),
+        ("dpar", c_double * 3), ("ipar", c_double * 3), ("seed", c_ull),
+        ("restitution", c_double), ("rebuild_interval", c_int), ("pad", c_int),
+    ]
+
+
+DECKS = os.path.join(HERE, "decks")
+
+
+def read_sidecar(path: str) -> Sidecar:
+    sc = Sidecar()
+    _check(lib().dem_read_sidecar(os.fsencode(path), ctypes.byref(sc)), "dem_read_sidecar")
+    return sc
 
 
 def read_restart(path: str):

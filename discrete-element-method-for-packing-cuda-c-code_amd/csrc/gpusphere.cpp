@@ -1,7 +1,8 @@
 // gpusphere.cpp -- DEM packing driver: the reference's main() flow (dempacking.cu:28-918) on
 // top of libdem.so's context API.
 //
-//   gpusphere [hopp3d.inp] [--tstop N] [--quiet] [--checkpoint-every N] [--max-cpu-seconds S]
+//   gpusphere [hopp3d.inp] [--side FILE] [--tstop N] [--quiet] [--checkpoint-every N]
+//             [--max-cpu-seconds S]
 //
 // Reads the positional deck, converts to reduced units, builds the layer lattice with the
 // reference's glibc rand() velocities, inserts one layer every tnewpar = 0.5 time units,
@@ -18,6 +19,12 @@
 // --max-cpu-seconds), after which the driver exits with status 3.  A run that finds
 // restart.dat starting with nrestart = 0 loads preflow.bin and continues from the step after
 // it; the steps it runs are the uninterrupted run's steps, bit for bit.
+//
+// --side FILE reads a deck side-car (dem_read_sidecar): container, particle count, diameter
+// distribution, insertion (the reference's layers, a jittered lattice or random insertion),
+// seed, restitution and rebuild policy -- the knobs the 12-line deck has no line for, so the
+// deck keeps the reference's layout.  Without it the run is the reference's.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -78,6 +85,7 @@ int write_checkpoint(dem_ctx *ctx, unsigned long long it, unsigned nparticles, i
 int main(int argc, char **argv)
 {
     const char *deck_path = "hopp3d.inp";
+    const char *side_path = nullptr;
     long long tstop_override = -1, ckpt_every = 0;
     double max_cpu = 41400.0;   // dempacking.cu:841
     bool quiet = false;
@@ -85,6 +93,7 @@ int main(int argc, char **argv)
         if (!std::strcmp(argv[a], "--tstop") && a + 1 < argc) tstop_override = std::atoll(argv[++a]);
         else if (!std::strcmp(argv[a], "--checkpoint-every") && a + 1 < argc) ckpt_every = std::atoll(argv[++a]);
         else if (!std::strcmp(argv[a], "--max-cpu-seconds") && a + 1 < argc) max_cpu = std::atof(argv[++a]);
+        else if (!std::strcmp(argv[a], "--side") && a + 1 < argc) side_path = argv[++a];
         else if (!std::strcmp(argv[a], "--quiet")) quiet = true;
         else deck_path = argv[a];
     }
@@ -108,15 +117,28 @@ int main(int argc, char **argv)
     std::printf("%lf %lf %lf\n", d.stiffness, d.dampcoeffn, d.dampcoefft);
     std::printf("%lf %lf\n", d.upp, d.upw);
     unsigned long long tstop = tstop_override >= 0 ? (unsigned long long)tstop_override : d.tstop;
+    dem_sidecar side;
+    std::memset(&side, 0, sizeof(side));
+    side.walls = -1;
+    side.dpar[0] = 1.0;
+    side.seed = 1;
+    side.restitution = -1.0;
+    if (side_path && (rc = dem_read_sidecar(side_path, &side))) return fail("dem_read_sidecar", rc);
+    if (side.particles > 0) d.ntot = side.particles;
 
     dem_material mat;
     dem_deck_material(&d, &mat);
+    if (side.restitution > 0.0) {   // both laws' dashpots from the restitution coefficient
+        double g = dem_damping_for_restitution(side.restitution);
+        if (g < 0.0) return fail("dem_damping_for_restitution", DEM_ERR_INVALID);
+        mat.p.dampnp = mat.p.damptp = mat.w.dampnw = mat.w.damptw = g;
+    }
     const double diam = d.diam;
     const double pi = std::atan(1.0) * 4.0;   // dempacking.cu:160
     const double dt = d.dt;
     // geometry in units of d (dempacking.cu:213-223)
-    const double thick = d.thick / diam, thick5 = thick * 0.5;
-    const double dmt2 = d.dmt2 / diam, dmt25 = dmt2 * 0.5;
+    const double thick = d.thick / diam;
+    const double dmt2 = d.dmt2 / diam;
     const double dmt1 = d.dmt1 / diam;
     const double hgt1 = d.zht1 / diam, hgt2 = hgt1 + d.zht2 / diam, hopzt = hgt2;
     (void)dmt1;
@@ -129,34 +151,73 @@ int main(int argc, char **argv)
         std::fclose(fp6);
     }
 
-    // layer lattice and initial velocities (dempacking.cu:242-316)
+    // container (dempacking.cu:213-223, or the side-car's) and diameters
     const unsigned ntot = (unsigned)d.ntot;
-    std::vector<double> xp, yp;
-    long nplayer;
-    double nrings = (unsigned)(dmt2) - 3;
-    double nringsy = (unsigned)(thick) - 3;
-    if (dmt25 > 1.99) {
-        double dringsx = dmt2 / (float)(nrings);
-        std::vector<double> rring((size_t)nrings), rringy((size_t)nringsy);
-        for (long ir = 0; ir < (long)nrings; ir++) rring[ir] = ((float)(ir) + 0.5) * dringsx - 0.5 * dmt2;
-        double dringsy = thick / (float)(nringsy);
-        for (long ir = 0; ir < (long)nringsy; ir++) rringy[ir] = ((float)(ir) + 0.5) * dringsy - 0.5 * thick;
-        for (long ir = 0; ir < (long)nrings; ir++)
-            for (long ij = 0; ij < (long)nringsy; ij++) {
-                xp.push_back(rring[ir]);
-                yp.push_back(rringy[ij]);
-            }
-        nplayer = (long)xp.size();
-    } else {
-        xp.push_back(0.0);
-        yp.push_back(0.0);
-        nplayer = 1;
+    if (ntot == 0) return fail("deck", DEM_ERR_INVALID);
+    double world[3] = {dmt2, thick, hopzt};
+    dem_walls walls;
+    std::memset(&walls, 0, sizeof(walls));
+    walls.kind = DEM_WALLS_BOX;
+    if (side.walls == DEM_WALLS_BOX) {
+        for (int k = 0; k < 3; k++) world[k] = side.box[k];
+    } else if (side.walls == DEM_WALLS_CYLINDER) {
+        walls.kind = DEM_WALLS_CYLINDER;
+        walls.radius = side.radius;
+        world[0] = world[1] = 2.0 * side.radius;
     }
+    if (side.height > 0.0) world[2] = side.height;
+    std::vector<double> diamv(ntot, side.dpar[0]);
+    if (side.diameters != 0 &&
+        (rc = dem_gen_diameters(side.diameters, ntot, side.dpar[0], side.dpar[1], side.dpar[2],
+                                side.seed, diamv.data())))
+        return fail("dem_gen_diameters", rc);
+    double dmax = 0.0, vol = 0.0;
+    for (double x : diamv) {
+        dmax = std::max(dmax, x);
+        vol += pi / 6.0 * x * x * x;
+    }
+    const double area = walls.kind == DEM_WALLS_BOX ? world[0] * world[1] : pi * side.radius * side.radius;
+    double fill_height = 0.0;   // random insertion: the height that gives the asked fill
+    if (side.insertion == 2) {
+        fill_height = vol / side.ipar[0] / area;
+        if (side.height > 0.0) fill_height = std::min(fill_height, side.height);
+        else world[2] = fill_height;
+    }
+    walls.size[0] = world[0];
+    walls.size[1] = world[1];
+    walls.size[2] = world[2];
+
     std::vector<double> pos(3 * (size_t)ntot, 0.0), pdis(3 * (size_t)ntot, 0.0), angv(3 * (size_t)ntot, 0.0);
     std::vector<double> rad(ntot), rmass(ntot), inert(ntot);
-    {
+    if ((rc = dem_particle_props(ntot, diamv.data(), rad.data(), rmass.data(), inert.data())))
+        return fail("dem_particle_props", rc);
+    long nplayer = 1;
+    if (side.insertion == 0) {
+        // layer lattice and initial velocities (dempacking.cu:242-316)
+        if (walls.kind != DEM_WALLS_BOX) return fail("insertion layers needs a box", DEM_ERR_INVALID);
+        const double wx = world[0], wy = world[1], wx5 = wx * 0.5;
+        std::vector<double> xp, yp;
+        double nrings = (unsigned)(wx) - 3;
+        double nringsy = (unsigned)(wy) - 3;
+        if (wx5 > 1.99) {
+            double dringsx = wx / (float)(nrings);
+            std::vector<double> rring((size_t)nrings), rringy((size_t)nringsy);
+            for (long ir = 0; ir < (long)nrings; ir++) rring[ir] = ((float)(ir) + 0.5) * dringsx - 0.5 * wx;
+            double dringsy = wy / (float)(nringsy);
+            for (long ir = 0; ir < (long)nringsy; ir++) rringy[ir] = ((float)(ir) + 0.5) * dringsy - 0.5 * wy;
+            for (long ir = 0; ir < (long)nrings; ir++)
+                for (long ij = 0; ij < (long)nringsy; ij++) {
+                    xp.push_back(rring[ir]);
+                    yp.push_back(rringy[ij]);
+                }
+            nplayer = (long)xp.size();
+        } else {
+            xp.push_back(0.0);
+            yp.push_back(0.0);
+            nplayer = 1;
+        }
         long nlayers = (long)(ntot / nplayer) + 1;
-        double zp0 = hopzt - 5.0;
+        double zp0 = world[2] - 5.0;
         unsigned i = 0;
         for (long ilay = 0; ilay < nlayers; ilay++)
             for (long ip = 0; ip < nplayer; ip++) {
@@ -172,41 +233,46 @@ int main(int argc, char **argv)
                 ang = 2.0 * pi * rno;
                 pdis[3 * i] = dxy * std::cos(ang);
                 pdis[3 * i + 1] = dxy * std::sin(ang);
-                rad[i] = 0.5;
-                rmass[i] = 8 * rad[i] * rad[i] * rad[i];
-                inert[i] = 0.4 * rmass[i] * (rad[i] * rad[i]);
                 i++;
             }
+    } else if (side.insertion == 1) {
+        if ((rc = dem_gen_lattice(&walls, side.ipar[2], side.ipar[0], side.ipar[1], ntot, side.seed,
+                                  pos.data())))
+            return fail("dem_gen_lattice", rc);
+    } else {
+        if ((rc = dem_gen_insert(&walls, 0.0, fill_height, ntot, diamv.data(), side.seed + 1,
+                                 pos.data())))
+            return fail("dem_gen_insert", rc);
     }
 
-    // grid (dempacking.cu:329-344)
+    // grid (dempacking.cu:329-344): cell = largest diameter + skin, 1.55 for the deck's spheres
     dem_instance inst;
     std::memset(&inst, 0, sizeof(inst));
     inst.n = ntot;
-    double cell = 1.55;
-    double world[3] = {dmt2, thick, hopzt};
-    inst.origin[0] = -dmt25;
-    inst.origin[1] = -thick5;
-    inst.origin[2] = 0.0;
+    const double skin = 1.55 - 1.0;   // cub - cua (neiglist.cu:27-28)
+    const double cell = dmax + skin;
+    double origin[3];
+    if ((rc = dem_make_grid(&walls, world[2], cell, inst.gsize, origin))) return fail("dem_make_grid", rc);
     for (int k = 0; k < 3; k++) {
         inst.cell[k] = cell;
-        inst.gsize[k] = (uint)((int)(world[k] / cell) + 1);
+        inst.origin[k] = origin[k];
     }
-    inst.walls.kind = DEM_WALLS_BOX;
-    inst.walls.size[0] = world[0];
-    inst.walls.size[1] = world[1];
-    inst.walls.size[2] = world[2];
+    inst.walls = walls;
     inst.mat = mat;
     inst.dt = dt;
-    inst.skin = 1.55 - 1.0;   // cub - cua (neiglist.cu:27-28)
+    inst.skin = skin;
 
     dem_ctx *ctx = nullptr;
     if ((rc = dem_create(&ctx, 1, &inst, 0))) return fail("dem_create", rc);
     if ((rc = dem_set_state(ctx, pos.data(), pdis.data(), angv.data(), rad.data(), rmass.data(),
                             inert.data(), 0)))
         return fail("dem_set_state", rc);
+    if (side.rebuild_interval > 0 &&
+        (rc = dem_set_option(ctx, DEM_OPT_REBUILD_INTERVAL, side.rebuild_interval)))
+        return fail("dem_set_option", rc);
 
-    unsigned nparticles = 0;
+    // the reference's layers enter one by one; lattice and random insertion start complete
+    unsigned nparticles = side.insertion == 0 ? 0u : ntot;
     int nparticledat = 0;
     unsigned long long it0 = 1;
     {
@@ -237,7 +303,7 @@ int main(int argc, char **argv)
             dem_stats st;
             if ((rc = dem_get_stats(ctx, &st))) return fail("dem_get_stats", rc);
             double avercn = st.pcntot / (nparticles * 1.0 + 1.0e-30);
-            double xvolumespace = world[0] * world[1] * st.maxhz;
+            double xvolumespace = area * st.maxhz;
             double porosity5 = 1.0 - nparticles * pi / 6 / xvolumespace;
             double bedaver = 1.0 - porosity5;
             if (std::FILE *fp = std::fopen("ppor.dat", "ab+")) {

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -142,6 +143,104 @@ int dem_read_deck(const char *path, dem_deck *d)
     d->stiffness = p[21]; d->dampcoeffn = p[22]; d->dampcoefft = p[23];
     d->upp = p[24]; d->upw = p[25];
     if (!(d->diam > 0.0) || !(d->denp > 0.0) || d->ntot < 0) return DEM_ERR_IO;
+    return DEM_OK;
+}
+
+int dem_read_sidecar(const char *path, dem_sidecar *sc)
+{
+    if (!path || !sc) return DEM_ERR_INVALID;
+    std::memset(sc, 0, sizeof(*sc));
+    sc->walls = -1;
+    sc->dpar[0] = 1.0;
+    sc->seed = 1;
+    sc->restitution = -1.0;
+    std::FILE *fp = std::fopen(path, "rb");
+    if (!fp) return DEM_ERR_IO;
+    std::string txt;
+    char buf[4096];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), fp)) > 0) txt.append(buf, got);
+    std::fclose(fp);
+    std::istringstream lines(txt);
+    std::string line;
+    while (std::getline(lines, line)) {
+        size_t hash = line.find('#');
+        if (hash != std::string::npos) line.erase(hash);
+        std::istringstream is(line);
+        std::vector<std::string> tok;
+        for (std::string t; is >> t;) tok.push_back(t);
+        if (tok.empty()) continue;
+        const std::string &key = tok[0];
+        std::vector<double> v;
+        size_t first = 1;
+        std::string word;
+        if (key == "walls" || key == "diameters" || key == "insertion" || key == "rebuild") {
+            if (tok.size() < 2) return DEM_ERR_IO;
+            word = tok[1];
+            first = 2;
+        }
+        for (size_t k = first; k < tok.size(); k++) {
+            double x;
+            if (!parse_number(tok[k], x) || !std::isfinite(x)) return DEM_ERR_IO;
+            v.push_back(x);
+        }
+        auto count = [&](size_t c) { return v.size() == c; };
+        auto whole = [](double x, double lo, double hi) {
+            return x >= lo && x <= hi && x == std::floor(x);
+        };
+        if (key == "walls") {
+            if (word == "box" && count(3) && v[0] > 0.0 && v[1] > 0.0 && v[2] > 0.0) {
+                sc->walls = DEM_WALLS_BOX;
+                for (int k = 0; k < 3; k++) sc->box[k] = v[k];
+            } else if (word == "cylinder" && count(1) && v[0] > 0.0) {
+                sc->walls = DEM_WALLS_CYLINDER;
+                sc->radius = v[0];
+            } else {
+                return DEM_ERR_IO;
+            }
+        } else if (key == "height") {
+            if (!count(1) || !(v[0] > 0.0)) return DEM_ERR_IO;
+            sc->height = v[0];
+        } else if (key == "particles") {
+            if (!count(1) || !whole(v[0], 1.0, 2147483647.0)) return DEM_ERR_IO;
+            sc->particles = (int)v[0];
+        } else if (key == "diameters") {
+            if (word == "mono" && count(1) && v[0] > 0.0) {
+                sc->diameters = 0;
+            } else if (word == "lognormal" && count(3) && v[0] >= 0.0 && v[1] > 0.0 && v[2] >= v[1]) {
+                sc->diameters = 1;
+            } else if (word == "bimodal" && count(3) && v[0] > 0.0 && v[1] > 0.0 && v[2] >= 0.0 &&
+                       v[2] <= 1.0) {
+                sc->diameters = 2;
+            } else {
+                return DEM_ERR_IO;
+            }
+            for (size_t k = 0; k < 3; k++) sc->dpar[k] = k < v.size() ? v[k] : 0.0;
+        } else if (key == "insertion") {
+            if (word == "layers" && count(0)) {
+                sc->insertion = 0;
+            } else if (word == "lattice" && count(3) && v[0] > 0.0 && v[1] >= 0.0) {
+                sc->insertion = 1;
+            } else if (word == "random" && count(1) && v[0] > 0.0 && v[0] < 0.6) {
+                sc->insertion = 2;
+            } else {
+                return DEM_ERR_IO;
+            }
+            for (size_t k = 0; k < 3; k++) sc->ipar[k] = k < v.size() ? v[k] : 0.0;
+        } else if (key == "seed") {
+            if (!count(1) || !whole(v[0], 0.0, 9007199254740992.0)) return DEM_ERR_IO;
+            sc->seed = (unsigned long long)v[0];
+        } else if (key == "restitution") {
+            if (!count(1) || !(v[0] > 0.0 && v[0] <= 1.0)) return DEM_ERR_IO;
+            sc->restitution = v[0];
+        } else if (key == "rebuild") {
+            if (word == "trigger" && count(0)) sc->rebuild_interval = 0;
+            else if (word == "interval" && count(1) && whole(v[0], 1.0, 1e9)) sc->rebuild_interval = (int)v[0];
+            else return DEM_ERR_IO;
+        } else {
+            return DEM_ERR_IO;
+        }
+    }
     return DEM_OK;
 }
 

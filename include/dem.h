@@ -416,6 +416,42 @@ typedef struct dem_deck {
 } dem_deck;
 
 int dem_read_deck(const char *path, dem_deck *deck);
+
+/* Deck side-car: the knobs the 12-line deck has no line for, in a file of its own so that
+ * hopp3d.inp keeps the reference's layout.  One "key values" per line, '#' starts a
+ * comment; lengths in units of the deck's particle diameter:
+ *   walls box X Y Z | walls cylinder R      container (default: the deck's box)
+ *   height H                                container height (default: from the deck)
+ *   particles N                             particle count (default: the deck's ntot)
+ *   diameters mono D | lognormal SIGMA DMIN DMAX | bimodal DBIG DSMALL FRACTION_BIG
+ *   insertion layers | lattice SPACING JITTER Z0 | random FILL
+ *                                           layers: the reference's layer pour (default);
+ *                                           lattice / random: every particle placed at the
+ *                                           start, at rest (random: seeded non-overlapping
+ *                                           insertion up to the height giving volume fraction
+ *                                           FILL, at most the container height)
+ *   seed S                                  diameters use S, positions S + 1 (lattice: S)
+ *   restitution E                           damping of both laws from e (the deck's
+ *                                           damping coefficients otherwise)
+ *   rebuild trigger | rebuild interval K    list rebuild policy (default: the trigger)
+ * Unknown keys and bad values are DEM_ERR_IO. */
+typedef struct dem_sidecar {
+    int walls;                   /* -1: the deck's box; DEM_WALLS_BOX; DEM_WALLS_CYLINDER */
+    int particles;               /* 0: the deck's ntot */
+    double box[3];
+    double radius;
+    double height;               /* 0: from the deck */
+    int diameters;               /* dem_gen_diameters kind: 0 mono, 1 lognormal, 2 bimodal */
+    int insertion;               /* 0 layers, 1 lattice, 2 random */
+    double dpar[3];              /* diameter parameters (mono: dpar[0] = D, default 1) */
+    double ipar[3];              /* lattice: spacing, jitter, z0; random: fill */
+    unsigned long long seed;     /* default 1 */
+    double restitution;          /* < 0: keep the deck's damping coefficients */
+    int rebuild_interval;        /* 0: displacement trigger */
+    int pad;
+} dem_sidecar;
+
+int dem_read_sidecar(const char *path, dem_sidecar *side);
 /* Reduced material from the deck (dempacking.cu:141-149, 224-225). */
 int dem_deck_material(const dem_deck *deck, dem_material *mat);
 /* Writes restart.dat exactly as dempacking.cu:908-913 (40 bytes). */

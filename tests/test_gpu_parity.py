@@ -747,6 +747,47 @@ def test_gpusphere_driver_on_shipped_deck(dem, gpu_ok, tmp_path):
     assert (tmp_path / "view.dat").read_text().startswith("number of particles=10000\n")
 
 
+def test_gpusphere_with_sidecar_runs_configuration_0(dem, gpu_ok, tmp_path):
+    """The driver on the shipped configuration-0 deck and side-car (jittered lattice,
+    restitution 0.6, rebuild every 20 steps) runs the same steps as the context API on the
+    package's configuration-0 workload: its checkpoint after step 200 holds the same state bit
+    for bit."""
+    import os
+    import shutil
+    import subprocess
+    for f in ("config0.inp", "config0.side"):
+        shutil.copy(os.path.join(dem.DECKS, f), tmp_path / f)
+    exe = os.path.join(dem.HERE, "gpusphere")
+    r = subprocess.run([exe, "config0.inp", "--side", "config0.side", "--tstop", "400",
+                        "--checkpoint-every", "200", "--quiet"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    pre = (tmp_path / "preflow.dat").read_text().splitlines()
+    assert pre[0] == "     200,    2000,       0" and len(pre) == 2001
+    w = dem.config0()
+    a = dem.Context(w.instances)
+    a.load(w)
+    a.step(200)
+    ref = a.state()
+    b = dem.Context(w.instances)
+    b.load_checkpoint(str(tmp_path / "preflow.bin"))
+    assert b.stats().steps == 200
+    got = b.state()
+    for k in ("pos", "pdis", "angv", "qi"):
+        assert np.array_equal(got[k], ref[k]), k
+    for x, y in zip(b.wall_history(), a.wall_history()):
+        assert np.array_equal(x, y)
+    # the loaded context steps with the driver's instance table (materials from the deck and
+    # the side-car's restitution): once the lattice reaches the floor and the spheres touch,
+    # any difference from the package's material would show
+    a.step(16000)
+    b.step(16000)
+    assert a.contacts().sum() > 500, "the spheres must be in contact by then"
+    got, ref = b.state(), a.state()
+    for k in ("pos", "pdis", "angv", "qi"):
+        assert np.array_equal(got[k], ref[k]), k
+
+
 def test_autotune_then_graph(dem, gpu_ok):
     """The autotuner's picks are from its candidate sets, and a graph run at the chosen fixed
     rebuild interval never raises the displacement trigger (no particle outran the skin)."""

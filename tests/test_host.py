@@ -164,3 +164,45 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_sidecar_defaults_and_shipped_configs(dem, tmp_path):
+    """The deck side-car: an empty file keeps the reference's run; the shipped side-cars of
+    configurations 0-2 parse to their documented settings."""
+    p = tmp_path / "empty.side"
+    p.write_text("# nothing but a comment\n\n")
+    sc = dem.read_sidecar(str(p))
+    assert (sc.walls, sc.particles, sc.height, sc.diameters, sc.insertion) == (-1, 0, 0.0, 0, 0)
+    assert (sc.dpar[0], sc.seed, sc.restitution, sc.rebuild_interval) == (1.0, 1, -1.0, 0)
+    c0 = dem.read_sidecar(os.path.join(dem.DECKS, "config0.side"))
+    assert c0.insertion == 1 and list(c0.ipar) == [1.2, 0.05, 1.0]
+    assert (c0.seed, c0.restitution, c0.rebuild_interval, c0.walls) == (1, 0.6, 20, -1)
+    c1 = dem.read_sidecar(os.path.join(dem.DECKS, "config1.side"))
+    assert (c1.walls, c1.radius, c1.diameters, list(c1.dpar)) == (
+        dem.WALLS_CYLINDER, 40.0, 1, [0.1, 0.7, 1.3])
+    assert (c1.insertion, c1.ipar[0], c1.seed, c1.rebuild_interval) == (2, 0.25, 2, 0)
+    c2 = dem.read_sidecar(os.path.join(dem.DECKS, "config2.side"))
+    assert (c2.diameters, list(c2.dpar), c2.insertion, c2.ipar[0], c2.height) == (
+        2, [1.0, 0.5, 0.3], 2, 0.3, 200.0)
+    for k in range(3):
+        d = dem.read_deck(os.path.join(dem.DECKS, f"config{k}.inp"))
+        assert d.ntot == (2000, 500000, 2000000)[k] and d.emod == 1e7 and d.diam == 0.001
+    # configuration 0's deck gives configuration 0's material and time step bit for bit
+    d0 = dem.read_deck(os.path.join(dem.DECKS, "config0.inp"))
+    assert d0.dt == dem.reduced_dt(dem.DT0, 1e-3) and d0.upp / d0.diam == dem.MAT0["mu_r"]
+
+
+def test_sidecar_rejects_unknown_keys_and_bad_values(dem, tmp_path):
+    bad = ["walls sphere 3", "walls box 1 2", "walls cylinder -1", "height 0", "particles 2.5",
+           "diameters lognormal 0.1 1.3 0.7", "diameters bimodal 1 0.5 1.5", "insertion random 0.7",
+           "insertion lattice 0 0 0", "seed -1", "restitution 1.5", "rebuild interval 0",
+           "rebuild sometimes", "colour blue", "height 10 20", "walls"]
+    for i, line in enumerate(bad):
+        p = tmp_path / f"bad{i}.side"
+        p.write_text(line + "\n")
+        with pytest.raises(dem.DemError) as ei:
+            dem.read_sidecar(str(p))
+        assert ei.value.code == dem.DEM_ERR_IO, line
+    with pytest.raises(dem.DemError) as ei:
+        dem.read_sidecar(str(tmp_path / "missing.side"))
+    assert ei.value.code == dem.DEM_ERR_IO
