@@ -172,9 +172,14 @@ int main(int argc, char **argv)
                                 side.seed, diamv.data())))
         return fail("dem_gen_diameters", rc);
     double dmax = 0.0, vol = 0.0;
-    for (double x : diamv) {
+    std::vector<double> cumvol(ntot + 1, 0.0);   // volume of particles [0, i)
+    bool unit_spheres = true;
+    for (unsigned i = 0; i < ntot; i++) {
+        const double x = diamv[i];
         dmax = std::max(dmax, x);
         vol += pi / 6.0 * x * x * x;
+        cumvol[i + 1] = vol;
+        unit_spheres = unit_spheres && x == 1.0;
     }
     const double area = walls.kind == DEM_WALLS_BOX ? world[0] * world[1] : pi * side.radius * side.radius;
     double fill_height = 0.0;   // random insertion: the height that gives the asked fill
@@ -304,7 +309,10 @@ int main(int argc, char **argv)
             if ((rc = dem_get_stats(ctx, &st))) return fail("dem_get_stats", rc);
             double avercn = st.pcntot / (nparticles * 1.0 + 1.0e-30);
             double xvolumespace = area * st.maxhz;
-            double porosity5 = 1.0 - nparticles * pi / 6 / xvolumespace;
+            // the reference counts unit spheres (N pi / 6, dempacking.cu:769); other diameters
+            // count their own volume
+            double pvol = unit_spheres ? nparticles * pi / 6 : cumvol[nparticles];
+            double porosity5 = 1.0 - pvol / xvolumespace;
             double bedaver = 1.0 - porosity5;
             if (std::FILE *fp = std::fopen("ppor.dat", "ab+")) {
                 std::fprintf(fp, "%8d %8d %8d %15.6f %15.6f %15.6f %15.6f\n", (int)it, (int)nparticles,
