@@ -788,6 +788,40 @@ def test_gpusphere_with_sidecar_runs_configuration_0(dem, gpu_ok, tmp_path):
         assert np.array_equal(got[k], ref[k]), k
 
 
+@pytest.mark.parametrize("config", ["config1", "config2"])
+def test_gpusphere_sidecar_cylinder_and_bimodal(dem, gpu_ok, tmp_path, config):
+    """The driver on the shipped configuration-1 (cylinder, lognormal, random insertion) and
+    configuration-2 (bimodal, random insertion capped at the container height) decks and
+    side-cars, with the particle count cut to 20,000 by one more side-car line (and the
+    configuration-2 box narrowed to 20 d to keep a bed several diameters deep): 1,000 steps,
+    all particles in from the first step, the radii of the distribution in particle.dat, a
+    packing fraction from each sphere's own volume in ppor.dat."""
+    import os
+    import shutil
+    import subprocess
+    shutil.copy(os.path.join(dem.DECKS, config + ".inp"), tmp_path / "deck.inp")
+    side = open(os.path.join(dem.DECKS, config + ".side")).read() + "particles 20000\n"
+    if config == "config2":   # 20,000 spheres would fill the 120 d box to under one diameter
+        side += "walls box 20 20 200\n"
+    (tmp_path / "deck.side").write_text(side)
+    exe = os.path.join(dem.HERE, "gpusphere")
+    r = subprocess.run([exe, "deck.inp", "--side", "deck.side", "--tstop", "1000", "--quiet"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    ppor = (tmp_path / "ppor.dat").read_text().split()
+    it, n, half_contacts = int(ppor[0]), int(ppor[1]), int(ppor[2])
+    phi, maxhz = float(ppor[4]), float(ppor[6])
+    assert (it, n) == (1000, 20000) and half_contacts >= 0
+    assert 0.05 < phi < 0.7 and maxhz > 5.0
+    zone = (tmp_path / "particle.dat").read_text().splitlines()
+    assert zone[0] == 'ZONE T="ZONE001"' and len(zone) == 1 + 20000
+    r_mm = np.array([float(ln.split()[3]) for ln in zone[1:]]) * 1e3   # radius in mm
+    if config == "config1":
+        assert 0.35 <= r_mm.min() and r_mm.max() <= 0.65 and r_mm.std() > 0.02
+    else:
+        assert set(np.round(r_mm, 6)) == {0.25, 0.5}
+
+
 def test_autotune_then_graph(dem, gpu_ok):
     """The autotuner's picks are from its candidate sets, and a graph run at the chosen fixed
     rebuild interval never raises the displacement trigger (no particle outran the skin)."""
