@@ -366,6 +366,7 @@ class Workload:
 
 def _props(diam: np.ndarray):
     L = lib()
+    diam = np.ascontiguousarray(diam, dtype=np.float64).reshape(-1)
     n = diam.shape[0]
     rad = np.empty(n)
     rmass = np.empty(n)
@@ -381,6 +382,7 @@ def gen_diameters(kind: int, n: int, p0: float, p1: float = 0.0, p2: float = 0.0
 
 
 def gen_insert(walls: Walls, z0: float, z1: float, diam: np.ndarray, seed: int):
+    diam = np.ascontiguousarray(diam, dtype=np.float64).reshape(-1)
     pos = np.empty((diam.shape[0], 3))
     _check(lib().dem_gen_insert(ctypes.byref(walls), z0, z1, diam.shape[0], _ptr(diam), seed,
                                 _ptr(pos)), "dem_gen_insert")

@@ -201,6 +201,10 @@ int main(int argc, char **argv)
         // layer lattice and initial velocities (dempacking.cu:242-316)
         if (walls.kind != DEM_WALLS_BOX) return fail("insertion layers needs a box", DEM_ERR_INVALID);
         const double wx = world[0], wy = world[1], wx5 = wx * 0.5;
+        // the reference's ring counts (unsigned)(w) - 3 need both sides >= 4 d once the
+        // lattice is used (wx > 3.98): narrower boxes would wrap or divide by zero
+        if (wx5 > 1.99 && (wx < 4.0 || wy < 4.0))
+            return fail("insertion layers needs a box at least 4 d wide on both sides", DEM_ERR_INVALID);
         std::vector<double> xp, yp;
         double nrings = (unsigned)(wx) - 3;
         double nringsy = (unsigned)(wy) - 3;

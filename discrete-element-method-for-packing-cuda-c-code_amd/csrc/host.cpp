@@ -14,6 +14,24 @@
 
 namespace {
 
+// Probability that a draw of ln d ~ N(-sigma^2/2, sigma) falls in [lo, hi] (sigma = 0: d = 1
+// exactly, so 1 or 0).
+double lognormal_window_mass(double sg, double lo, double hi)
+{
+    if (sg == 0.0) return (lo <= 1.0 && 1.0 <= hi) ? 1.0 : 0.0;
+    const double m = -0.5 * sg * sg;
+    auto cdf = [&](double d) { return 0.5 * std::erfc(-(std::log(d) - m) / (sg * std::sqrt(2.0))); };
+    return cdf(hi) - cdf(lo);
+}
+
+// Redraws allowed per sphere of the clipped lognormal before the window is declared empty.
+const long kMaxRedraws = 1000000;
+
+// Smallest share of the lognormal a side-car clip window must hold (at most ~1e4 redraws per
+// sphere on average).
+const double kMinWindowMass = 1e-4;
+
+
 // splitmix64: small, portable, seedable.
 struct Rng {
     unsigned long long s;
@@ -207,7 +225,9 @@ int dem_read_sidecar(const char *path, dem_sidecar *sc)
         } else if (key == "diameters") {
             if (word == "mono" && count(1) && v[0] > 0.0) {
                 sc->diameters = 0;
-            } else if (word == "lognormal" && count(3) && v[0] >= 0.0 && v[1] > 0.0 && v[2] >= v[1]) {
+            } else if (word == "lognormal" && count(3) && v[0] >= 0.0 && v[1] > 0.0 && v[2] >= v[1] &&
+                       lognormal_window_mass(v[0], v[1], v[2]) >= kMinWindowMass) {
+                // the clip window must hold a drawable share of the distribution
                 sc->diameters = 1;
             } else if (word == "bimodal" && count(3) && v[0] > 0.0 && v[1] > 0.0 && v[2] >= 0.0 &&
                        v[2] <= 1.0) {
@@ -357,9 +377,14 @@ int dem_gen_diameters(int kind, uint n, double p0, double p1, double p2, unsigne
         // ln d ~ N(-sigma^2/2, sigma): mean diameter 1, rejection-clipped to [p1, p2]
         double sg = p0, lo = p1, hi = p2;
         if (!(sg >= 0.0) || !(lo > 0.0) || !(hi >= lo)) return DEM_ERR_INVALID;
+        if (lognormal_window_mass(sg, lo, hi) <= 0.0) return DEM_ERR_INVALID;
         for (uint i = 0; i < n; i++) {
             double d;
-            do { d = std::exp(-0.5 * sg * sg + sg * r.normal()); } while (d < lo || d > hi);
+            long draws = 0;
+            do {
+                if (++draws > kMaxRedraws) return DEM_ERR_INVALID;   // a window too far out
+                d = std::exp(-0.5 * sg * sg + sg * r.normal());
+            } while (d < lo || d > hi);
             diam[i] = d;
         }
         return DEM_OK;

@@ -952,7 +952,9 @@ int dem_set_option(dem_ctx *c, int option, double value)
         drop_graph(c);
         return DEM_OK;
     case DEM_OPT_REBUILD_INTERVAL:
-        if (value < 0.0) return DEM_ERR_INVALID;
+        // a finite whole number in [0, INT_MAX] (NaN fails the first test)
+        if (!(value >= 0.0 && value <= 2147483647.0) || (double)(long long)value != value)
+            return DEM_ERR_INVALID;
         c->interval = (int)value;
         drop_graph(c);
         return DEM_OK;

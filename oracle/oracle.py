@@ -229,3 +229,20 @@ def law_pair(xi, xj, vi, vj, wi, wj, ri, rj, mi, mj, hin, dt, mat):
                        mj, ptr(a[6]), 0.0, dt, ctypes.byref(m), ptr(hout), ctypes.byref(fout),
                        ptr(fi), ptr(ti), ptr(tj))
     return c, fi, ti, tj, hout, fout.value
+
+
+def law_wall(walls, x, v, om, r, m, hin, dt, mat):
+    """Oracle wall law for one sphere against every wall of `walls` (a dem_amd.Walls or an
+    OrcWalls); returns (walls hit, F, T, h_out, |Ft|) -- the force and torque summed over
+    the walls the sphere overlaps, the history of the deepest one."""
+    L = lib()
+    a = [np.ascontiguousarray(q, dtype=np.float64) for q in (x, v, om, hin)]
+    w = walls if isinstance(walls, OrcWalls) else walls_from(walls)
+    mm = mat_from(mat) if not isinstance(mat, OrcMat) else mat
+    hout = np.zeros(3)
+    fout = c_double()
+    fw = np.zeros(3)
+    tw = np.zeros(3)
+    hits = L.orc_law_wall(ctypes.byref(w), ptr(a[0]), ptr(a[1]), ptr(a[2]), r, m, ptr(a[3]), 0.0,
+                          dt, ctypes.byref(mm), ptr(hout), ctypes.byref(fout), ptr(fw), ptr(tw))
+    return hits, fw, tw, hout, fout.value

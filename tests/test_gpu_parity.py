@@ -510,6 +510,8 @@ def test_invalid_arguments_and_capacity_errors(dem, gpu_ok):
                      (dem.OPT_PIPELINE, 17), (dem.OPT_PIPELINE_BATCH, 0),
                      (dem.OPT_PIPELINE_BATCH, 17), (dem.OPT_XCD_CHUNK, -1),
                      (dem.OPT_FUSED_FAST, 65), (dem.OPT_REBUILD_INTERVAL, -1),
+                     (dem.OPT_REBUILD_INTERVAL, float("nan")), (dem.OPT_REBUILD_INTERVAL, 1e20),
+                     (dem.OPT_REBUILD_INTERVAL, 2.5),
                      (dem.OPT_SKIN, 0.0), (dem.OPT_PAIR_GRID, 0), (99, 1.0)):
         with pytest.raises(dem.DemError) as ei:
             c.set_option(opt, val)
@@ -556,10 +558,19 @@ def test_invalid_arguments_and_capacity_errors(dem, gpu_ok):
 
 
 def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
-    """The 500K lognormal cylinder workload at full size: lists bit-exact, one step's forces
-    and state within 1e-10 of the oracle, pair-force sum zero (Newton's third law)."""
+    """The 500K lognormal cylinder workload at full size, its loose fill compressed to 55% of
+    its height and given random displacements and spins so that most spheres are in
+    contact: lists bit-exact, one step's forces and state within 1e-10 of the oracle, and
+    the pair forces cancel (Newton's third law): the total force over all spheres is
+    gravity plus the wall forces, which the oracle's wall law gives for the spheres that
+    touch a wall."""
     w = dem.config1()
-    say(f"[500k] n={w.n}")
+    n = w.n
+    say(f"[500k] n={n}")
+    rng = np.random.default_rng(51)
+    w.pos[:, 2] = w.pos[:, 2] * 0.55
+    w.pdis = rng.normal(scale=1e-4, size=(n, 3))
+    w.angv = rng.normal(scale=1e-4, size=(n, 3))
     ctx, sim = make_pair(dem, orc, w, width=64)
     ctx.rebuild()
     assert sim.rebuild() == 0
@@ -572,11 +583,26 @@ def test_full_size_500k_rebuild_and_step(dem, orc, gpu_ok):
     assert sim.step(1) == 0
     f, t, c = ctx.forces()
     assert rel_err(f, sim.field("forcei")) <= TOL
+    assert rel_err(t, sim.field("torquei")) <= TOL
+    assert np.array_equal(c, sim.field("contacti"))
+    assert int(c.sum()) > n, "the compressed bed must be in contact"
     st = ctx.state()
     assert rel_err(st["pdis"], sim.field("pdis")) <= TOL
     # internal forces cancel: total force = gravity + walls
-    _, _, out = ctx.pairs()
-    assert np.all(np.isfinite(out))
+    inst = w.instances[0]
+    R = inst.walls.radius
+    touch = np.nonzero((w.pos[:, 2] < w.rad) |
+                       (np.hypot(w.pos[:, 0], w.pos[:, 1]) + w.rad > R))[0]
+    fw = np.zeros(3)
+    for i in touch:
+        hits, fwi, _, _, _ = orc.law_wall(inst.walls, w.pos[i], w.pdis[i], w.angv[i], w.rad[i],
+                                          w.rmass[i], np.zeros(3), inst.dt, inst.mat)
+        fw += fwi
+    resid = f.sum(axis=0) - fw - np.array([0.0, 0.0, -w.rmass.sum()])
+    scale = float(np.max(np.linalg.norm(f, axis=1)))
+    say(f"[500k] {len(touch)} spheres on a wall; pair-force residual {np.max(np.abs(resid)) / scale:.2e}")
+    assert len(touch) > 0
+    assert np.max(np.abs(resid)) <= 1e-10 * scale
 
 
 def pair_keys(pairs, n):

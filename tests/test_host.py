@@ -206,3 +206,62 @@ def test_sidecar_rejects_unknown_keys_and_bad_values(dem, tmp_path):
     with pytest.raises(dem.DemError) as ei:
         dem.read_sidecar(str(tmp_path / "missing.side"))
     assert ei.value.code == dem.DEM_ERR_IO
+
+
+def test_sidecar_rejects_lognormal_windows_without_mass(dem, tmp_path):
+    """A clip window the lognormal cannot draw from (sigma 0 with 1 outside the window, or a
+    window far out in the tail) is refused by the parser, and the generator refuses it too
+    instead of redrawing forever."""
+    for i, line in enumerate(["diameters lognormal 0 1.1 1.3", "diameters lognormal 0.1 1.5 2.0",
+                              "diameters lognormal 0.1 1.2 1.2"]):
+        p = tmp_path / f"w{i}.side"
+        p.write_text(line + "\n")
+        with pytest.raises(dem.DemError) as ei:
+            dem.read_sidecar(str(p))
+        assert ei.value.code == dem.DEM_ERR_IO, line
+    # windows that do hold mass still parse (sigma 0 around 1, and the shipped window)
+    for i, line in enumerate(["diameters lognormal 0 0.9 1.1", "diameters lognormal 0.1 0.7 1.3"]):
+        p = tmp_path / f"ok{i}.side"
+        p.write_text(line + "\n")
+        assert dem.read_sidecar(str(p)).diameters == 1
+    for args in ((0.0, 1.1, 1.3), (0.1, 1.2, 1.2)):
+        with pytest.raises(dem.DemError) as ei:
+            dem.gen_diameters(1, 10, *args)
+        assert ei.value.code == dem.DEM_ERR_INVALID
+    d = dem.gen_diameters(1, 1000, 0.0, 0.9, 1.1)
+    assert np.all(d == 1.0)
+
+
+def test_generators_take_any_diameter_array(dem):
+    """Float32 and strided diameter arrays give the same radii, masses and insertion as the
+    contiguous float64 ones (the binding converts before passing a pointer)."""
+    d64 = dem.gen_diameters(1, 400, 0.1, 0.7, 1.3, seed=9).astype(np.float32).astype(np.float64)
+    d32 = d64.astype(np.float32)
+    strided = np.repeat(d64, 2)[::2]
+    ref = dem._props(d64)
+    for d in (d32, strided):
+        got = dem._props(d)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+    walls = dem.make_walls(dem.WALLS_BOX, (12.0, 12.0, 40.0))
+    p_ref = dem.gen_insert(walls, 0.0, 20.0, d64, 3)
+    assert np.array_equal(dem.gen_insert(walls, 0.0, 20.0, d32, 3), p_ref)
+    assert np.array_equal(dem.gen_insert(walls, 0.0, 20.0, strided, 3), p_ref)
+
+
+@pytest.mark.parametrize("box", ["20 3.9 30", "20 3 30", "3.99 20 30"])
+def test_driver_refuses_layer_insertion_in_a_narrow_box(dem, tmp_path, box):
+    """The reference's layer lattice has (unsigned)(w) - 3 rings per side: a side-car box
+    under 4 d on either side is refused before any device work, instead of wrapping the ring
+    count or dividing by zero."""
+    import shutil
+    import subprocess
+    exe = os.path.join(dem.HERE, "gpusphere")
+    assert os.path.exists(exe), "build the driver first"
+    shutil.copy(os.path.join(GOLDEN, "hopp3d.inp"), tmp_path / "hopp3d.inp")
+    side = tmp_path / "narrow.side"
+    side.write_text(f"walls box {box}\nparticles 50\n")
+    r = subprocess.run([exe, "--side", str(side)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert "at least 4 d" in (r.stdout + r.stderr)
