@@ -154,7 +154,7 @@ _lib = None
 # Exported symbols of libdem.so (every function include/dem.h declares).
 EXPORTS = [
     "dem_last_error", "dem_status_string", "setMaterialProperties", "setWallGeometry",
-    "calcHash", "RadixSort", "findCellStart", "neighborarray", "prefixsum", "pairlist",
+    "calcHash", "RadixSort", "RadixSortNoAlloc", "findCellStart", "neighborarray", "prefixsum", "pairlist",
     "calculateforcepair", "sumallforcei", "calculateforceiw", "updateposition",
     "outputporositycn", "dem_create", "dem_destroy", "dem_set_option", "dem_set_state",
     "dem_get_state", "dem_get_wall_history", "dem_save_checkpoint", "dem_load_checkpoint",
@@ -247,6 +247,7 @@ def lib() -> ctypes.CDLL:
         "setWallGeometry": ([P], None),
         "calcHash": ([P, P, P, P, P, c_int], None),
         "RadixSort": ([P, P, c_uint, c_uint], None),
+        "RadixSortNoAlloc": ([P, P, c_uint, c_uint], c_int),
         "findCellStart": ([P, P, c_int, c_uint], None),
         "neighborarray": ([P, P, P, P, P, P, c_int, c_int, P, P, P], None),
         "prefixsum": ([c_uint, P, P, P, P], None),

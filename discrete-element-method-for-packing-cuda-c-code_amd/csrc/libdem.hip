@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -69,8 +70,12 @@ struct Scratch {
     unsigned *offs = nullptr;     // scanned counts (+1)
     size_t counts_cap = 0;
     unsigned *total = nullptr;    // 4 device words
+    bool fixed = false;           // a view of preallocated memory: never (re)allocates
     int ensure(size_t ntiles_scan, size_t ncounts)
     {
+        if (fixed)
+            return (total && ntiles_scan <= tiles_cap && ncounts <= counts_cap) ? DEM_OK
+                                                                               : DEM_ERR_CAPACITY;
         if (!total) DEM_HIP(hipMalloc(&total, 4 * sizeof(unsigned)));
         if (ntiles_scan > tiles_cap) {
             if (tiles) DEM_HIP(hipFree(tiles));
@@ -2334,6 +2339,96 @@ void RadixSort(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint b
     if (rc) { g_last_error = rc; return; }
     if (res != a) DEM_HIP_V(hipMemcpyAsync(a, res, sizeof(uint2) * elements, hipMemcpyDeviceToDevice, 0));
     check_launch("RadixSort");
+}
+
+// ---- RadixSort without allocation --------------------------------------------------------
+// The sort's scratch (digit counts, their scan, the scan's tile sums) lives in one file-scope
+// device array, sized for kNoAllocMaxElements keys; each device holds its own copy, whose
+// address is resolved once per device (hipGetSymbolAddress) and kept in g_noalloc_ptr.
+// Every word the sort reads is written earlier in the same call (the histogram pass writes
+// every count of every pass, the scan every tile sum and offset), so nothing needs clearing.
+constexpr unsigned kNoAllocMaxElements = 1u << 23;                                // 8,388,608
+constexpr size_t kNaSortTiles = kNoAllocMaxElements / kSortTile;                  // 4,096
+constexpr size_t kNaCounts = 256 * kNaSortTiles;                                 // 1,048,576
+constexpr size_t kNaScanTiles = (kNaCounts + kScanTile - 1) / kScanTile;         // 512
+constexpr size_t kNaOffs = kNaCounts;                                            // word offsets
+constexpr size_t kNaTiles = kNaOffs + kNaCounts + 64;
+constexpr size_t kNaTotal = kNaTiles + kNaScanTiles;
+constexpr size_t kNaWords = kNaTotal + 64;                                       // 8.0 MiB + 2.5 KiB
+static_assert(kNoAllocMaxElements % kSortTile == 0, "whole sort tiles");
+
+static __device__ __align__(256) unsigned scratch[kNaWords];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+unsigned *g_noalloc_ptr[kNaMaxDevices] = {};
+std::mutex g_noalloc_mu;
+
+// Device address of this device's copy of `scratch` (resolved on the first call per device).
+int noalloc_scratch(unsigned **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return DEM_ERR_NO_DEVICE;
+    if (dev < 0 || dev >= kNaMaxDevices) return DEM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        unsigned *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            std::fprintf(stderr, "RadixSortNoAlloc: hipGetSymbolAddress failed (%s)\n",
+                         hipGetErrorString(e));
+            return DEM_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return DEM_OK;
+}
+}  // namespace
+
+int RadixSortNoAlloc(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint bits)
+{
+    if (elements == 0) return DEM_OK;
+    if (!pData0 || !pData1 || pData0 == pData1) {
+        g_last_error = DEM_ERR_INVALID;
+        return DEM_ERR_INVALID;
+    }
+    if (elements > kNoAllocMaxElements) {
+        std::fprintf(stderr, "RadixSortNoAlloc: %u elements exceed its fixed scratch (at most %u)\n",
+                     elements, kNoAllocMaxElements);
+        g_last_error = DEM_ERR_CAPACITY;
+        return DEM_ERR_CAPACITY;
+    }
+    unsigned *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc) {
+        g_last_error = rc;
+        return rc;
+    }
+    Scratch S;   // a view of the device array; ensure() checks sizes and allocates nothing
+    S.fixed = true;
+    S.counts = base;
+    S.offs = base + kNaOffs;
+    S.counts_cap = kNaCounts;
+    S.tiles = base + kNaTiles;
+    S.tiles_cap = kNaScanTiles;
+    S.total = base + kNaTotal;
+    uint2 *a = reinterpret_cast<uint2 *>(pData0), *b = reinterpret_cast<uint2 *>(pData1);
+    uint2 *res = radix_sort(S, a, b, a, elements, (int)std::min(bits, 32u), 0, nullptr, &rc);
+    if (rc) {
+        g_last_error = rc;
+        return rc;
+    }
+    if (res != a &&
+        hipMemcpyAsync(a, res, sizeof(uint2) * elements, hipMemcpyDeviceToDevice, 0) != hipSuccess) {
+        g_last_error = DEM_ERR_HIP;
+        return DEM_ERR_HIP;
+    }
+    if (hipGetLastError() != hipSuccess) {
+        g_last_error = DEM_ERR_HIP;
+        return DEM_ERR_HIP;
+    }
+    return DEM_OK;
 }
 
 void findCellStart(uint *particleHash, uint *cellStart, int numBodies, uint numGridCells)

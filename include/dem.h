@@ -117,6 +117,15 @@ void calcHash(double *Pos, uint *particleHash, uint gridSize[3], double cellSize
  * `bits` key bits, pData1 is scratch of the same size, result in pData0. */
 void RadixSort(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint bits);
 
+/* RadixSort without allocation -- no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same stable sort, with RadixSort's scratch (digit counts, their
+ * scan) in a fixed device array of 8 MiB that each device holds until the process ends.
+ * At most 8,388,608 (2^23) elements: larger counts return DEM_ERR_CAPACITY untouched.
+ * Returns a dem_status.  Calls on one device share that array: calls that overlap on one
+ * device (several host threads or streams) race. */
+int RadixSortNoAlloc(KeyValuePair *pData0, KeyValuePair *pData1, uint elements, uint bits);
+
 /* particleSystem.cu:49 -- cellStart[c] = first sorted slot of cell c, 0xffffffff if empty. */
 void findCellStart(uint *particleHash, uint *cellStart, int numBodies, uint numGridCells);
 
