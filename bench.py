@@ -1,25 +1,33 @@
-"""Benchmark: particle-steps/s of the plain DEM step at 500K spheres on one MI355X, over the
-workload's full 50,000-step run.
+"""Benchmark: particle-steps/s of the plain DEM step at 500K spheres on one MI355X.
 
 Workload: 500,000 lognormal spheres (mean d = 1 mm, sigma 0.1, clipped to [0.7, 1.3] mm) inserted
 at random without overlap into a cylinder of radius 40 mean-d with a flat floor; E = 1e7 Pa,
 nu = 0.29, rho = 2500 kg/m^3, sliding friction 0.3, rolling friction 0.002 d, restitution 0.6,
-dt = 1e-6 s, FP64, reference rebuild policy (displacement trigger).  Inputs are resident in HBM
-before timing.  One step = [list rebuild when triggered] + pair forces + gather/walls/gravity/
-integration.
+dt = 1e-6 s, FP64, reference rebuild policy (displacement trigger).
 
-The step path is chosen by the library (auto: one timed step of each path after the first and
-every 8th rebuild); --fused / --split force one.
+The timed steps (`--steps`, the headline `value`) run on that workload poured into a bed
+(200,000 steps on the GPU before timing): the packed regime, P/N = 7.6 candidate pairs and about
+2.5 contacts per sphere -- the regime the reference's PCCP timings describe.  Inputs are
+resident in HBM before timing.  One step = [list rebuild when triggered] + pair forces +
+ordered sums, walls, gravity, integration.
 
-Prints ONE JSON line.  The CPU baseline is the oracle (the same PCCP algorithm in C with
-OpenMP) on a bounded sample of the same workload.
+Other legs, each a JSON object beside the headline: the workload's own 50,000-step run from
+its random insertion (`full_run`; its first 300 steps are `loose_start`), the 2M bimodal box's
+20,000-step run (`c2`), the list rebuild stage by stage (`rebuild`), the reference-named drop-in
+entry points in the reference driver's order (`dropin`), the batched form (`batched`), the
+adjoint form and calibration (`adjoint`), and the CPU baseline (`cpu_baseline`: the oracle, the
+same PCCP algorithm in C with OpenMP, on the host cores).
+
+Prints ONE JSON line.
 """
 import argparse
 import importlib.util
 import json
 import math
 import os
+import platform
 import sys
+import threading
 import time
 
 import numpy as np
@@ -44,7 +52,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-LOOSE = 300   # the first timed steps, reported on their own (loose start)
+LOOSE = 300   # the first steps of the full run, reported on their own (loose start)
 
 
 def kernel_bytes(P, N):
@@ -63,6 +71,31 @@ def kernel_bytes(P, N):
                    record r/w 2*128, wall history r/w 2*32, two CSR offsets 8, contact
                    count 4."""
     return {"step": 76 * P + 232 * N, "pair": 152 * P + 88 * N, "particle": 84 * P + 332 * N}
+
+
+def reference_step_bytes(P, N):
+    """The reference's own non-rebuild step (pair force, reduction, walls + gravity,
+    integration as separate kernels with their per-pair arrays) by the same counting rule:
+    268 P + 636 N bytes."""
+    return 268 * P + 636 * N
+
+
+def rebuild_bytes(N, P, P_old, C, passes):
+    """Algorithmic bytes of each rebuild stage (same counting rule; kernels and arrays of this
+    build): hash reads pos 24, writes (cell, index) 8; each 8-bit radix pass reads the keys
+    twice (histogram, scatter) and writes them once; cell ranges write both cell bounds and
+    read the sorted keys; reorder moves the 128-byte record and 32-byte wall history, reads
+    the sorted index and inverse map, writes the inverse map, wall flag, compact position
+    (32 + 16) and index; the count reads the compact positions, indices and keys and writes
+    two counts; two scans read and write n words each; the fill reads what the count reads
+    plus both offsets and writes the pair (8), partner (4) and transposed entry (4); the
+    remap reads the pair, both members' indices and old slots, the old offsets, the old pair
+    it finds (8) and its history (32), and writes the history (32); the transpose reads the
+    offsets, the transposed entry and the pair it finds, and writes two words per entry."""
+    return {"hash": 32 * N, "sort": 24 * N * passes, "cell_range": 8 * N + 8 * C,
+            "reorder": 396 * N, "nbr_count": 48 * N + 8 * C, "scans": 16 * N,
+            "nbr_fill": 56 * N + 8 * C + 16 * P, "remap": 12 * N + 80 * P + 8 * P_old,
+            "transpose": 8 * N + 24 * P}
 
 
 TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f) for f in (
@@ -96,32 +129,287 @@ def measured_traffic(kernel: str, n: int, p_avg: float):
         f"average P")
 
 
-def cpu_baseline(dem, w, budget_s):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert, width=64)
+def dominant_kernel(tm, N, label):
+    """Dominant kernel of a run timed with OPT_TIME_STEPS (events around each step's kernels
+    on the context stream): its average duration, algorithmic bytes at the average pair count
+    and the roofline object."""
+    cand = []
+    sp, fu = tm["split"], tm["fused"]
+    if sp["steps"]:
+        pa = sp["pairs_sum"] / sp["steps"]
+        cand.append(("pair", "k_pair_force", sp["pair_walls_ms"], sp["steps"], pa))
+        cand.append(("particle", "k_particle", sp["particle_ms"], sp["steps"], pa))
+    if fu["steps"]:
+        cand.append(("step", "k_step", fu["step_ms"], fu["steps"], fu["pairs_sum"] / fu["steps"]))
+    dom, kname, tot_ms, launches, p_avg = max(cand, key=lambda c: c[2])
+    ms = tot_ms / launches
+    kb = kernel_bytes(p_avg, N)[dom]
+    achieved = kb / (ms * 1e-3) / 1e9
+    traffic, src = measured_traffic(kname, N, p_avg)
+    log(f"{label}: dominant kernel {kname}, {launches} launches, {ms * 1e3:.1f} us average at "
+        f"P {p_avg:.0f}: {achieved:.0f} GB/s algorithmic ({achieved / HBM_PEAK_GBS:.3f} of peak)")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": src,
+            "kernel": kname, "launches": launches, "avg_us": ms * 1e3, "avg_pairs": p_avg,
+            "bytes_per_launch": kb}
+
+
+# ---------------------------------------------------------------------------------------
+# Headline: the poured 500K bed
+# ---------------------------------------------------------------------------------------
+
+def dense_bed(dem, w, settle_steps, load=None, save=None, graph=True):
+    """The 500K workload poured into a bed on the GPU (graph steps), or loaded from a file
+    a previous run saved (so a profiler can run the timed window alone)."""
+    if load:
+        d = np.load(load)
+        ws = dem.Workload(w.name + "-settled", w.instances, d["pos"], d["pdis"], d["angv"], w.rad,
+                          w.rmass, w.inert, meta=dict(w.meta, bed_top=float(d["pos"][:, 2].max()),
+                                                      settle_steps=int(d["settle_steps"])))
+        log(f"bed loaded from {load}")
+        return ws
     t0 = time.perf_counter()
-    assert sim.step(1) == 0          # first step builds the lists
-    t_first = time.perf_counter() - t0
+    ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0, graph=graph)
+    log(f"bed poured in {time.perf_counter() - t0:.1f} s ({settle_steps} steps, top "
+        f"{ws.meta['bed_top']:.1f})")
+    if save:
+        np.savez(save, pos=ws.pos, pdis=ws.pdis, angv=ws.angv, settle_steps=settle_steps)
+        log(f"bed saved to {save}")
+    return ws
+
+
+def new_context(dem, ws, split_opt):
+    ctx = dem.Context(ws.instances)
+    if split_opt is not None:
+        ctx.set_option(dem.OPT_FUSED_STEP, 0 if split_opt else 1)
+    ctx.load(ws)
+    return ctx
+
+
+def bench_headline(dem, ws, steps, warmup, split_opt):
+    """`steps` timed steps of the poured bed after `warmup` (displacement trigger: the first
+    warm-up step builds the lists), wall clock around device work only; then the same steps
+    again from the same state with HIP events around every step's kernels (the dominant
+    kernel's duration; the events cost a few percent, so `value` is from the clean pass;
+    the trajectory is the same bit for bit); then one rebuild timed stage by stage."""
+    N = ws.n
+    ctx = new_context(dem, ws, split_opt)
+    ctx.step(warmup)
+    ctx.sync()
+    s0 = ctx.stats()
     t0 = time.perf_counter()
-    assert sim.step(2) == 0
-    per = (time.perf_counter() - t0) / 2
-    k = max(2, min(500, int(budget_s / max(per, 1e-6))))
-    t0 = time.perf_counter()
-    assert sim.step(k) == 0
+    ctx.step(steps)
+    ctx.sync()
     el = time.perf_counter() - t0
-    return {"value": w.n * k / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{k} non-rebuild steps of the same {w.n}-sphere workload after its first "
-                      f"step (list build {t_first:.1f} s, excluded), oracle C/OpenMP"}
+    s1 = ctx.stats()
+    value = N * steps / el
+    log(f"timed {steps} steps of the bed in {el * 1e3:.2f} ms: {value:.4e} particle-steps/s, "
+        f"P={s1.npairs} (P/N {s1.npairs / N:.2f}), {s1.rebuilds - s0.rebuilds} rebuilds, "
+        f"path {'fused' if s1.step_path else 'split'}")
+    ctx.close()
+    ctx = new_context(dem, ws, split_opt)
+    ctx.step(warmup)
+    ctx.set_option(dem.OPT_TIME_STEPS, 1)
+    ctx.step_timing()
+    ctx.step(steps)
+    ctx.sync()
+    tm = ctx.step_timing()
+    roof = dominant_kernel(tm, N, "bed")
+    contacts = int(ctx.contacts().sum())
+    # one rebuild of the bed, stage by stage
+    rb = ctx.time_rebuild()
+    ctx.close()
+    P = s1.npairs
+    ref_b = reference_step_bytes(P, N)
+    out = {"value": value, "ms_per_step": el / steps * 1e3, "npairs": P, "contacts": contacts,
+           "rebuilds_timed": int(s1.rebuilds - s0.rebuilds),
+           "step_path": "fused" if s1.step_path else "split",
+           "reference_step_bytes": {"bytes": ref_b,
+                                    "GB_s_over_step": ref_b / (el / steps) / 1e9,
+                                    "frac": ref_b / (el / steps) / 1e9 / HBM_PEAK_GBS}}
+    return out, roof, rebuild_object(rb)
 
 
-def bench_batched(dem, steps, interval, tune_steps=48, graph=True):
+def rebuild_object(rb):
+    b = rebuild_bytes(rb["n"], rb["npairs"], rb["npairs_old"], rb["ncells"], rb["passes"])
+    stages = {}
+    tot_b = tot_ms = 0.0
+    for k in rb:
+        if k not in b:
+            continue
+        ms = rb[k]
+        gbs = b[k] / (ms * 1e-3) / 1e9 if ms > 0 else None
+        stages[k] = {"ms": round(ms, 4), "bytes": b[k], "GB_s": round(gbs, 1) if gbs else None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+        tot_b += b[k]
+        tot_ms += ms
+    log("rebuild: " + ", ".join(f"{k} {v['ms'] * 1e3:.0f} us" for k, v in stages.items()) +
+        f"; total {tot_ms:.3f} ms, {tot_b / (tot_ms * 1e-3) / 1e9:.0f} GB/s")
+    return {"n": rb["n"], "npairs": rb["npairs"], "npairs_old": rb["npairs_old"],
+            "ncells": rb["ncells"], "radix_passes": rb["passes"], "total_ms": round(tot_ms, 4),
+            "bytes": tot_b, "GB_s": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+            "frac": round(tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "stages": stages}
+
+
+# ---------------------------------------------------------------------------------------
+# The workload's own runs: 500K from its random insertion, 2M bimodal box
+# ---------------------------------------------------------------------------------------
+
+def bench_run(dem, w, steps, split_opt, label, loose=0):
+    """`steps` steps of workload `w` from its initial state (displacement trigger), then the
+    same steps again with events around every step's kernels."""
+    ctx = new_context(dem, w, split_opt)
+    ctx.step(1)                      # the first list (not timed)
+    ctx.sync()
+    s0 = ctx.stats()
+    t0 = time.perf_counter()
+    done, el_loose, t_last = 0, None, t0
+    while done < steps:
+        k = min(steps - done, loose if (done == 0 and loose) else 1000)
+        ctx.step(k)
+        done += k
+        now = time.perf_counter()
+        if el_loose is None:
+            ctx.sync()
+            el_loose = time.perf_counter() - t0
+        if now - t_last > 10.0:
+            log(f"  {label}: {done} / {steps} steps ({now - t0:.1f} s)")
+            t_last = now
+    ctx.sync()
+    el = time.perf_counter() - t0
+    s1 = ctx.stats()
+    ctx.close()
+    ctx = new_context(dem, w, split_opt)
+    ctx.step(1)
+    ctx.set_option(dem.OPT_TIME_STEPS, 1)
+    ctx.step_timing()
+    for k0 in range(0, steps, 5000):
+        ctx.step(min(5000, steps - k0))
+    ctx.sync()
+    tm = ctx.step_timing()
+    ctx.close()
+    out = {"workload": w.name, "n": w.n, "steps": steps, "value": w.n * steps / el,
+           "ms_per_step": el / steps * 1e3, "npairs_start": s0.npairs, "npairs_end": s1.npairs,
+           "rebuilds": int(s1.rebuilds - s0.rebuilds),
+           "kernel_totals_ms": {"fused_step": tm["fused"]["step_ms"],
+                                "fused_steps": tm["fused"]["steps"],
+                                "split_pair_walls": tm["split"]["pair_walls_ms"],
+                                "split_particle": tm["split"]["particle_ms"],
+                                "split_steps": tm["split"]["steps"], "rebuild": tm["rebuild_ms"],
+                                "rebuilds": tm["rebuilds"]},
+           "roofline": dominant_kernel(tm, w.n, label)}
+    if loose:
+        out["loose_start"] = {"steps": loose, "value": w.n * loose / el_loose,
+                              "ms_per_step": el_loose / loose * 1e3}
+    log(f"{label}: {steps} steps in {el:.2f} s: {out['value']:.4e} particle-steps/s, P "
+        f"{s0.npairs} -> {s1.npairs}, {out['rebuilds']} rebuilds")
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# Drop-in: the reference-named entry points in the reference driver's order
+# ---------------------------------------------------------------------------------------
+
+def bench_dropin(dem, ws, steps, width=64):
+    """The 11 reference-named entry points called in dempacking.cu:535-739 order on
+    caller-owned device buffers (the reference's AoS double3 layout, rows of `width`
+    neighbours), on the poured bed: zero the totals; on a rebuild step calcHash, RadixSort,
+    findCellStart, neighborarray, prefixsum, pairlist; then calculateforcepair, sumallforcei,
+    calculateforceiw, updateposition (which returns the rebuild flag to the host).  The
+    neighbour test is the reference's fixed cutoff for unit spheres (1.55 d)."""
+    import ctypes
+    L = dem.lib()
+    D = dem.DeviceArray
+    inst = ws.instances[0]
+    n = ws.n
+    S = n * width
+    f64, u32 = np.float64, np.uint32
+    pos, pdis, angv = (D.from_host(getattr(ws, k)) for k in ("pos", "pdis", "angv"))
+    rad, rmass, inert = (D.from_host(getattr(ws, k)) for k in ("rad", "rmass", "inert"))
+    C = inst.gsize[0] * inst.gsize[1] * inst.gsize[2]
+    hash0, hash1 = D(2 * n, u32), D(2 * n, u32)
+    cstart = D(C, u32)
+    anei, aref = D(S, u32), D(S, u32)
+    njgi, njli, sjgi, oldsjgi = D(n, u32), D(n, u32), D(n, u32), D(n, u32)
+    lp, ln, oldln, cpair = D(S, u32), D(S, u32), D(S, u32), D(S, u32)
+    dispt, olddispt, fp, tpi, tpj = (D(3 * S, f64) for _ in range(5))
+    fricp, oldfricp = D(S, f64), D(S, f64)
+    disptw, fricpw = D(3 * n, f64), D(n, f64)
+    forcei, torquei, qi = D(3 * n, f64), D(3 * n, f64), D(3 * n, f64)
+    contacti = D(n, u32)
+    G = (ctypes.c_uint * 3)(*inst.gsize)
+    cell = (ctypes.c_double * 3)(*inst.cell)
+    orig = (ctypes.c_double * 3)(*inst.origin)
+    wsz = (ctypes.c_double * 3)(*inst.walls.size)
+    L.setMaterialProperties(ctypes.byref(inst.mat.p), ctypes.byref(inst.mat.w))
+    L.setWallGeometry(ctypes.byref(inst.walls))
+    iall = ctypes.c_uint(1)
+    maxhz = ctypes.c_double(0.0)
+    P = ctypes.c_uint(0)
+    L.dem_last_error()
+    dt = inst.dt
+
+    def one_step():
+        forcei.zero()
+        torquei.zero()
+        contacti.zero()
+        rebuilt = iall.value != 0
+        if rebuilt:
+            qi.zero()
+            L.calcHash(pos.p, hash0.p, G, cell, orig, n)
+            L.RadixSort(hash0.p, hash1.p, n, 32)
+            L.findCellStart(hash0.p, cstart.p, n, C)
+            L.neighborarray(pos.p, hash0.p, cstart.p, G, cell, orig, width, n, anei.p, njgi.p,
+                            njli.p)
+            L.prefixsum(n, njgi.p, sjgi.p, oldsjgi.p, ctypes.byref(P))
+            L.pairlist(sjgi.p, oldsjgi.p, oldln.p, anei.p, njgi.p, njli.p, olddispt.p, oldfricp.p,
+                       width, width, n, aref.p, lp.p, ln.p, dispt.p, fricp.p)
+        L.calculateforcepair(pos.p, pdis.p, angv.p, rad.p, rmass.p, lp.p, ln.p, oldln.p, dispt.p,
+                             fricp.p, olddispt.p, oldfricp.p, dt, fp.p, tpi.p, tpj.p, cpair.p,
+                             P.value)
+        L.sumallforcei(width, n, njgi.p, njli.p, fp.p, tpi.p, tpj.p, cpair.p, aref.p, forcei.p,
+                       torquei.p, contacti.p)
+        L.calculateforceiw(wsz, pos.p, pdis.p, angv.p, rad.p, rmass.p, dt, P.value, n, disptw.p,
+                           fricpw.p, forcei.p, torquei.p)
+        L.updateposition(wsz, n, pos.p, pdis.p, angv.p, rmass.p, inert.p, forcei.p, torquei.p,
+                         qi.p, dt, 0.0, ctypes.byref(iall), ctypes.byref(maxhz))
+        return rebuilt
+
+    one_step()                         # builds the lists
+    for _ in range(4):
+        one_step()
+    err = L.dem_last_error()
+    if err:
+        raise dem.DemError(err, "drop-in entry points")
+    L.dem_device_sync()
+    t0 = time.perf_counter()
+    reb = 0
+    for _ in range(steps):
+        reb += one_step()
+    L.dem_device_sync()
+    el = time.perf_counter() - t0
+    err = L.dem_last_error()
+    if err:
+        raise dem.DemError(err, "drop-in entry points")
+    out = {"workload": ws.name, "n": n, "steps": steps, "value": n * steps / el,
+           "ms_per_step": el / steps * 1e3, "npairs": int(P.value), "rebuilds": reb,
+           "row_width": width}
+    log(f"drop-in: {out['value']:.4e} particle-steps/s ({out['ms_per_step']:.3f} ms per step, "
+        f"P {P.value}, {reb} rebuilds)")
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+# Batched and adjoint forms
+# ---------------------------------------------------------------------------------------
+
+def bench_batched(dem, steps, plain_steps, interval, tune_steps=48, graph=True):
     """Batched form (256 instances, hipGraph-captured, autotuned skin / launch shapes / rebuild
-    interval) against the same instances stepped as per-instance plain contexts (same skin and
-    fixed rebuild interval): host loop, and hipGraph replays.  graph=False steps the batched
-    context with the host loop too and skips the per-instance graph replays (the same kernels,
-    for profilers that cannot trace graph launches)."""
+    interval) over `steps` steps, against the same instances stepped as per-instance plain
+    contexts (same skin and fixed rebuild interval) over `plain_steps` steps: host loop, and
+    hipGraph replays.  graph=False steps the batched context with the host loop too and skips
+    the per-instance graph replays (the same kernels, for profilers that cannot trace graph
+    launches)."""
     t0 = time.perf_counter()
     w = dem.config3(rebuild_interval=interval)
     log(f"batched workload: {len(w.instances)} instances, n={w.n} ({time.perf_counter() - t0:.1f} s)")
@@ -137,6 +425,7 @@ def bench_batched(dem, steps, interval, tune_steps=48, graph=True):
         big.set_option(dem.OPT_REBUILD_INTERVAL, interval)
         log(f"autotune: {tune}")
     steps = max(interval, (steps // interval) * interval)
+    plain_steps = max(interval, (plain_steps // interval) * interval)
     adv = big.step_graph if graph else big.step
     adv(2 * interval)
     big.sync()
@@ -162,74 +451,27 @@ def bench_batched(dem, steps, interval, tune_steps=48, graph=True):
         ctxs.append(c)
     for c in ctxs:
         c.sync()
-    log(f"batched {nb * steps / tb:.4e} p-steps/s; {len(ctxs)} per-instance contexts ready")
+    log(f"batched {nb * steps / tb:.4e} p-steps/s over {steps} steps; {len(ctxs)} per-instance "
+        f"contexts ready")
     t0 = time.perf_counter()
     for c in ctxs:
-        c.step(steps)
+        c.step(plain_steps)
     tp = time.perf_counter() - t0
     tg = None
     if graph:
         t0 = time.perf_counter()
         for c in ctxs:
-            c.step_graph(steps)
+            c.step_graph(plain_steps)
         tg = time.perf_counter() - t0
-    vb, vp = nb * steps / tb, nb * steps / tp
-    vg = nb * steps / tg if tg else None
+    vb, vp = nb * steps / tb, nb * plain_steps / tp
+    vg = nb * plain_steps / tg if tg else None
     log(f"batched {vb:.4e} p-steps/s; per-instance plain {vp:.4e}; ratio {vb / vp:.2f}x"
         + (f" (per-instance graphs {vg:.4e}: {vb / vg:.2f}x)" if vg else ""))
     return {"workload": w.name, "instances": len(w.instances), "n": nb, "steps": steps,
-            "rebuild_interval": interval, "value": vb, "per_instance_plain": vp,
-            "per_instance_graph": vg, "ratio_vs_plain": vb / vp, "ratio_vs_plain_graph": vb / vg if vg else None,
-            "ms_per_step": tb / steps * 1e3, "trigger_in_last_group": int(flag),
-            "autotune": tune}
-
-
-def dense_bed(dem, w, settle_steps, load=None, save=None, graph=True):
-    """The 500K workload poured into a bed on the GPU (graph steps), or loaded from a file
-    a previous run saved (so a profiler can run the dense window alone)."""
-    if load:
-        d = np.load(load)
-        ws = dem.Workload(w.name + "-settled", w.instances, d["pos"], d["pdis"], d["angv"], w.rad,
-                          w.rmass, w.inert, meta=dict(w.meta, bed_top=float(d["pos"][:, 2].max()),
-                                                      settle_steps=int(d["settle_steps"])))
-        log(f"dense window: bed loaded from {load}")
-        return ws
-    t0 = time.perf_counter()
-    ws = dem.settle_bed(w, steps=settle_steps, interval=50, tap=0.0, graph=graph)
-    log(f"dense window: bed poured in {time.perf_counter() - t0:.1f} s (top {ws.meta['bed_top']:.1f})")
-    if save:
-        np.savez(save, pos=ws.pos, pdis=ws.pdis, angv=ws.angv, settle_steps=settle_steps)
-        log(f"dense window: bed saved to {save}")
-    return ws
-
-
-def bench_dense(dem, w, steps=1000, settle_steps=200_000, split=False, load=None, save=None,
-                graph=True):
-    """Dense window: `steps` plain steps, displacement trigger, from the poured bed."""
-    ws = dense_bed(dem, w, settle_steps, load, save, graph)
-    ctx = dem.Context(ws.instances)
-    if split is not None:
-        ctx.set_option(dem.OPT_FUSED_STEP, 0 if split else 1)
-    ctx.load(ws)
-    ctx.step(20)
-    ctx.sync()
-    s0 = ctx.stats()
-    t0 = time.perf_counter()
-    ctx.step(steps)
-    ctx.sync()
-    el = time.perf_counter() - t0
-    s1 = ctx.stats()
-    kt = ctx.time_kernels(20)
-    P, N = s1.npairs, ws.n
-    out = {"value": N * steps / el, "steps": steps, "ms_per_step": el / steps * 1e3, "npairs": P,
-           "step_path": "fused" if ctx.stats().step_path else "split",
-           "pairs_per_particle": round(P / N, 3), "rebuilds": int(s1.rebuilds - s0.rebuilds),
-           "step_kernel_us": kt["particle_ms"] * 1e3, "walls_us": kt["pair_ms"] * 1e3,
-           "rebuild_ms": kt["rebuild_ms"], "settle_steps": ws.meta["settle_steps"]}
-    log(f"dense window: {out['value']:.4e} particle-steps/s, P/N {out['pairs_per_particle']}, "
-        f"{out['rebuilds']} rebuilds in {steps} steps, step kernel {out['step_kernel_us']:.1f} us")
-    ctx.close()
-    return out
+            "per_instance_steps": plain_steps, "rebuild_interval": interval, "value": vb,
+            "per_instance_plain": vp, "per_instance_graph": vg, "ratio_vs_plain": vb / vp,
+            "ratio_vs_plain_graph": vb / vg if vg else None, "ms_per_step": tb / steps * 1e3,
+            "trigger_in_last_group": int(flag), "autotune": tune}
 
 
 def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True):
@@ -353,26 +595,103 @@ def copy_instance(inst):
     return c
 
 
+# ---------------------------------------------------------------------------------------
+# CPU baseline
+# ---------------------------------------------------------------------------------------
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(dem, ws, w0, budget_s):
+    """The oracle (this build's C restatement of the same PCCP algorithm, OpenMP, built with
+    -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off) on the GPU box's host cores: the poured
+    500K bed -- the same state the GPU's timed steps start from -- on all threads and on one
+    thread (a bounded sample of steps each, after the first, list-building step), and the
+    2,000-sphere configuration's full 20,000-step run on all threads and on one thread."""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    gomp = ctypes.CDLL("libgomp.so.1")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": threads}
+
+    def bed_rate(th, budget):
+        gomp.omp_set_num_threads(th)
+        sim = orc.OracleSim(ws.instances[0], ws.pos, ws.pdis, ws.angv, ws.rad, ws.rmass,
+                            ws.inert, width=64)
+        assert sim.step(1) == 0
+        t0 = time.perf_counter()
+        assert sim.step(1) == 0
+        per = time.perf_counter() - t0
+        k = max(1, min(200, int(budget / max(per, 1e-6))))
+        t0 = time.perf_counter()
+        assert sim.step(k) == 0
+        el = time.perf_counter() - t0
+        return ws.n * k / el, k
+
+    def c0_rate(th):
+        gomp.omp_set_num_threads(th)
+        sim = orc.OracleSim(w0.instances[0], w0.pos, w0.pdis, w0.angv, w0.rad, w0.rmass, w0.inert)
+        t0 = time.perf_counter()
+        for it in range(w0.steps):
+            sim.set_iall(1 if it % w0.rebuild_interval == 0 else 0)
+            assert sim.step(1) == 0
+        el = time.perf_counter() - t0
+        return w0.n * w0.steps / el
+
+    v_all, k_all = bed_rate(threads, budget_s)
+    log(f"cpu baseline: poured bed {v_all:.4e} p-steps/s on {threads} threads ({k_all} steps)")
+    v_one, k_one = bed_rate(1, 0.5 * budget_s)
+    log(f"cpu baseline: poured bed {v_one:.4e} p-steps/s on 1 thread ({k_one} steps)")
+    c0_all = c0_rate(threads)
+    c0_one = c0_rate(1)
+    log(f"cpu baseline: 2,000-sphere full run {c0_all:.4e} ({threads} threads), {c0_one:.4e} "
+        f"(1 thread) p-steps/s")
+    gomp.omp_set_num_threads(threads)
+    out.update({"value": v_all, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                "sample": f"{k_all} steps of the poured 500000-sphere bed (the state the GPU's "
+                          f"timed steps start from), after its list-building first step; oracle "
+                          f"C/OpenMP",
+                "one_thread": {"value": v_one, "steps": k_one},
+                "config0_full_run": {"n": w0.n, "steps": w0.steps, "value_all_threads": c0_all,
+                                     "value_one_thread": c0_one},
+                "build": "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"})
+    return out
+
+
+# ---------------------------------------------------------------------------------------
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50_000,
-                    help="timed steps (default: the full 50,000-step run of the workload)")
+    ap.add_argument("--steps", type=int, default=1000,
+                    help="timed steps of the poured 500K bed (the headline)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=500_000)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--settle", type=int, default=200_000, help="pour steps before timing")
+    ap.add_argument("--bed-save", default=None, help="save the poured bed (.npz)")
+    ap.add_argument("--bed-load", default=None, help="load the poured bed (.npz)")
+    ap.add_argument("--bed-only", action="store_true",
+                    help="run only the headline steps on the bed (no other leg, no JSON line)")
+    ap.add_argument("--no-full-run", action="store_true")
+    ap.add_argument("--full-steps", type=int, default=50_000)
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=20_000)
+    ap.add_argument("--no-dropin", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--kernel-steps", type=int, default=30)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--batched-steps", type=int, default=10_000)
+    ap.add_argument("--batched-plain-steps", type=int, default=480)
     ap.add_argument("--batched-interval", type=int, default=20)
-    ap.add_argument("--no-dense", action="store_true")
-    ap.add_argument("--dense-steps", type=int, default=1000)
-    ap.add_argument("--dense-settle", type=int, default=200_000)
-    ap.add_argument("--dense-save", default=None, help="save the poured bed (.npz)")
-    ap.add_argument("--dense-load", default=None, help="load the poured bed (.npz)")
-    ap.add_argument("--dense-only", action="store_true",
-                    help="run only the dense window (no plain leg, no JSON line)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--calib-instances", type=int, default=64)
@@ -404,99 +723,36 @@ def main():
         L.dem_dev_free(a)
         L.dem_dev_free(b)
         log(f"copy probe: 3 device-to-device copies of {nb} bytes")
+    split_opt = True if args.split else (False if args.fused else None)
+    graph = not args.no_graph
     t0 = time.perf_counter()
     w = dem.config1(n=args.n)
     log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
-    split_opt = True if args.split else (False if args.fused else None)
-    if args.dense_only:
-        d = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
-                        split=split_opt, load=args.dense_load, save=args.dense_save,
-                        graph=not args.no_graph)
-        log(json.dumps(d))
+    # the 2M box's random insertion runs on the host while the GPU pours the 500K bed
+    c2_box = {}
+    gen = None
+    if not (args.no_c2 or args.bed_only):
+        gen = threading.Thread(target=lambda: c2_box.update(w=dem.config2()), daemon=True)
+        gen.start()
+    ws = dense_bed(dem, w, args.settle, args.bed_load, args.bed_save, graph)
+    head, roof, rebuild = bench_headline(dem, ws, args.steps, args.warmup, split_opt)
+    if args.bed_only:
+        log(json.dumps({"bed": head, "roofline": roof, "rebuild": rebuild}))
         return
-    ctx = dem.Context(w.instances)
-    if args.split or args.fused:
-        ctx.set_option(dem.OPT_FUSED_STEP, 0 if args.split else 1)
-    ctx.load(w)
-    ctx.step(args.warmup)
-    ctx.sync()
-    s0 = ctx.stats()
-    # the timed region is the run itself, in chunks (device work only between the clock reads;
-    # no statistics or copies inside); the first LOOSE steps are also reported on their own
-    t0 = time.perf_counter()
-    done, el_loose, t_last = 0, None, t0
-    while done < args.steps:
-        k = min(args.steps - done, LOOSE if done == 0 else 1000)
-        ctx.step(k)
-        done += k
-        now = time.perf_counter()
-        if el_loose is None:
-            el_loose = now - t0
-        if now - t_last > 10.0:
-            log(f"  {done} / {args.steps} steps ({now - t0:.1f} s)")
-            t_last = now
-    ctx.sync()
-    el = time.perf_counter() - t0
-    s1 = ctx.stats()
-    value = w.n * args.steps / el
-    loose = {"steps": min(LOOSE, args.steps), "value": w.n * min(LOOSE, args.steps) / el_loose,
-             "ms_per_step": el_loose / min(LOOSE, args.steps) * 1e3, "npairs_at_start": s0.npairs}
-    log(f"timed {args.steps} steps in {el:.3f} s: {value:.4e} particle-steps/s, "
-        f"P={s1.npairs} (P/N {s1.npairs / w.n:.2f}) rebuilds={s1.rebuilds - s0.rebuilds}; first "
-        f"{loose['steps']} steps {loose['value']:.4e}")
-    # kernel durations: the same run again from the same state, with HIP events around the
-    # kernels of every step (the events cost ~9% of the step rate, so `value` is from the
-    # clean pass above; the trajectory is the same bit for bit)
-    ctx.load(w)
-    ctx.step(args.warmup)
-    ctx.set_option(dem.OPT_TIME_STEPS, 1)
-    ctx.step_timing()
-    t1 = time.perf_counter()
-    for k0 in range(0, args.steps, 5000):
-        ctx.step(min(5000, args.steps - k0))
-    ctx.sync()
-    el_events = time.perf_counter() - t1
-    tm = ctx.step_timing()
-    log(f"event-timed pass: {args.steps} steps in {el_events:.2f} s")
-    P, N = s1.npairs, w.n
-    # dominant kernel over the timed region, from the events dem_step recorded around its
-    # kernels: total time per kernel, average pair count over its launches
-    cand = []
-    sp, fu = tm["split"], tm["fused"]
-    if sp["steps"]:
-        pa = sp["pairs_sum"] / sp["steps"]
-        cand.append(("pair", "k_pair_force", sp["pair_walls_ms"], sp["steps"], pa))
-        cand.append(("particle", "k_particle", sp["particle_ms"], sp["steps"], pa))
-    if fu["steps"]:
-        cand.append(("step", "k_step", fu["step_ms"], fu["steps"], fu["pairs_sum"] / fu["steps"]))
-    dom, kname, tot_ms, launches, p_avg = max(cand, key=lambda c: c[2])
-    ms = tot_ms / launches
-    kb = kernel_bytes(p_avg, N)
-    achieved = kb[dom] / (ms * 1e-3) / 1e9
-    traffic, traffic_src = measured_traffic(kname, N, p_avg)
-    log(f"steps by path: split {sp['steps']}, fused {fu['steps']}; rebuilds {tm['rebuilds']} "
-        f"({tm['rebuild_ms'] / max(tm['rebuilds'], 1):.3f} ms each)")
-    log(f"dominant kernel {kname}: {launches} launches, {ms * 1e3:.1f} us average, P {p_avg:.0f}: "
-        f"{achieved:.0f} GB/s algorithmic")
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "traffic_source": traffic_src, "kernel": kname, "launches": launches,
-            "avg_us": ms * 1e3, "avg_pairs": p_avg, "bytes_per_launch": kb[dom]}
-    kernel_ms = {"event_pass_s": el_events,
-                 "split_pair_walls_ms": sp["pair_walls_ms"], "split_particle_ms": sp["particle_ms"],
-                 "split_steps": sp["steps"], "fused_walls_ms": fu["walls_ms"],
-                 "fused_step_ms": fu["step_ms"], "fused_steps": fu["steps"],
-                 "rebuild_ms": tm["rebuild_ms"], "rebuilds": tm["rebuilds"]}
-    del ctx
-    dense = None
-    if not args.no_dense:
-        dense = bench_dense(dem, w, steps=args.dense_steps, settle_steps=args.dense_settle,
-                            split=split_opt, load=args.dense_load, save=args.dense_save,
-                            graph=not args.no_graph)
+    full = None
+    if not args.no_full_run:
+        full = bench_run(dem, w, args.full_steps, split_opt, "full run", loose=LOOSE)
+    c2 = None
+    if gen is not None:
+        gen.join()
+        c2 = bench_run(dem, c2_box.pop("w"), args.c2_steps, split_opt, "2M bimodal")
+    dropin = None
+    if not args.no_dropin:
+        dropin = bench_dropin(dem, ws, args.dropin_steps)
     batched = None
     if not args.no_batched:
-        batched = bench_batched(dem, args.batched_steps, args.batched_interval,
-                                graph=not args.no_graph)
+        batched = bench_batched(dem, args.batched_steps, args.batched_plain_steps,
+                                args.batched_interval, graph=graph)
     adjoint = None
     if not args.no_adjoint:
         log("importing torch for the autograd layer (the first import on a fresh machine can "
@@ -504,35 +760,40 @@ def main():
         t0 = time.perf_counter()
         import torch  # noqa: F401
         log(f"torch imported ({time.perf_counter() - t0:.1f} s)")
-        adjoint = bench_adjoint(dem, graph=not args.no_graph)
+        adjoint = bench_adjoint(dem, graph=graph)
         if not args.no_calibration:
             adjoint["calibration"] = bench_calibration(dem, ninst=args.calib_instances,
-                                                       n=args.calib_n, graph=not args.no_graph)
+                                                       n=args.calib_n, graph=graph)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(dem, w, args.cpu_seconds)
-        log(f"cpu baseline: {cpu['value']:.4e} particle-steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(dem, ws, dem.config0(), args.cpu_seconds)
+    N = ws.n
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": value / PUBLISHED_500K,
+        "vs_baseline": head["value"] / PUBLISHED_500K,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": w.name, "n": N, "npairs": P, "pairs_per_particle": round(P / N, 3),
+        "config": {"workload": "plain-500k-lognormal-cylinder-poured-bed", "n": N,
+                   "settle_steps": ws.meta["settle_steps"], "npairs": head["npairs"],
+                   "pairs_per_particle": round(head["npairs"] / N, 3),
+                   "contacts_per_particle": round(head["contacts"] / N, 3),
                    "rebuild_policy": "displacement trigger |q| > 0.275",
-                   "rebuilds_timed": int(s1.rebuilds - s0.rebuilds),
-                   "kernel_totals": kernel_ms},
+                   "rebuilds_timed": head["rebuilds_timed"], "step_path": head["step_path"]},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "loose_start": loose,
-        "dense_window": dense,
+        "reference_step_bytes": head["reference_step_bytes"],
+        "rebuild": rebuild,
+        "full_run": full,
+        "c2": c2,
+        "dropin": dropin,
         "batched": batched,
         "adjoint": adjoint,
     }

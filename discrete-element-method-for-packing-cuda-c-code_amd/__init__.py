@@ -168,7 +168,7 @@ EXPORTS = [
     "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_contact_filter", "dem_law_pair_vjp",
     "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
     "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
-    "dem_adj_release", "dem_get_step_timing",
+    "dem_adj_release", "dem_get_step_timing", "dem_time_rebuild",
 ]
 
 
@@ -208,6 +208,7 @@ def lib() -> ctypes.CDLL:
         "dem_last_ms": ([P], c_double),
         "dem_time_kernels": ([P, c_int, P, P], c_int),
         "dem_get_step_timing": ([P, P], c_int),
+        "dem_time_rebuild": ([P, P], c_int),
         "dem_get_hash": ([P, P], c_int),
         "dem_get_sorted": ([P, P], c_int),
         "dem_get_cell_start": ([P, P, c_uint], c_int),
@@ -668,6 +669,18 @@ class Context:
                           "pairs_sum": o[3]},
                 "fused": {"walls_ms": o[4], "step_ms": o[5], "steps": int(o[6]), "pairs_sum": o[7]},
                 "rebuild_ms": o[8], "rebuilds": int(o[9])}
+
+    REBUILD_STAGES = ("hash", "sort", "cell_range", "reorder", "nbr_count", "scans", "nbr_fill",
+                      "remap", "transpose")
+
+    def time_rebuild(self) -> dict:
+        """One list rebuild now, timed stage by stage (ms), with the sizes it ran at."""
+        o = np.zeros(14)
+        _check(self._L.dem_time_rebuild(self._h, _ptr(o)), "dem_time_rebuild")
+        out = {k: float(o[i]) for i, k in enumerate(self.REBUILD_STAGES)}
+        out.update(passes=int(o[9]), npairs=int(o[10]), npairs_old=int(o[11]), ncells=int(o[12]),
+                   n=int(o[13]))
+        return out
 
     def sync(self):
         _check(self._L.dem_sync(self._h), "dem_sync")

@@ -1416,18 +1416,42 @@ __global__ __launch_bounds__(kBlock) void c_pair_list(
             dispt[3 * (size_t)slot + 2] = h2;
             fricp[slot] = f;
         } else {
-            unsigned cj = njgi[j] + njli[j];
-            unsigned rr = 0, found = kEmpty;
-            for (unsigned mm = 0; mm < cj && mm < (unsigned)width; mm++) {
-                unsigned jj = anei[(size_t)j * width + mm];
-                if (jj == i) { found = rr; break; }
-                if (jj > j) rr++;
-            }
+            // pair (j, i) sits in j's own segment of the new list, which j's lane fills with
+            // its larger-index candidates in Anei order: its slot is the first in
+            // [sjgi[j] - njgi[j], sjgi[j]) whose second member is i.  The segment is read
+            // from the list the lanes write (Ln), so it is searched once the fill is done:
+            // c_pair_list_partner does that in a second launch; here the entry is marked.
             unsigned pos = njgi[i] + r;
-            if (pos < (unsigned)width)
-                aref[(size_t)i * width + pos] = found == kEmpty ? kEmpty : (sjgi[j] - njgi[j]) + found;
+            if (pos < (unsigned)width) aref[(size_t)i * width + pos] = kEmpty - 1u - j;
             r++;
         }
+    }
+}
+
+// Second pass of pairlistD: each entry (j, i), j < i, of Aref row i marked by c_pair_list
+// (as kEmpty - 1 - j) gets the slot of pair (j, i) -- a search of j's own segment of Ln, a few
+// contiguous entries (its larger-index candidates), instead of j's whole neighbour row.
+__global__ __launch_bounds__(kBlock) void c_pair_list_partner(const unsigned *sjgi,
+                                                              const unsigned *njgi,
+                                                              const unsigned *njli, int width,
+                                                              unsigned n, const unsigned *ln,
+                                                              unsigned *aref)
+{
+    unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const unsigned g = njgi[i];
+    const unsigned e = min(g + njli[i], (unsigned)width);
+    for (unsigned m = g; m < e; m++) {
+        const unsigned v = aref[(size_t)i * width + m];
+        if (v == kEmpty || v < n) continue;
+        const unsigned j = kEmpty - 1u - v;
+        unsigned found = kEmpty;
+        if (j < n) {
+            const unsigned lo = sjgi[j] - njgi[j], hi = sjgi[j];
+            for (unsigned q = lo; q < hi; q++)
+                if (ln[q] == i) { found = q; break; }
+        }
+        aref[(size_t)i * width + m] = found;
     }
 }
 

@@ -350,8 +350,17 @@ unsigned pair_grid(const dem_ctx *c, unsigned np)
     return std::max(1u, std::min(g, gmax));
 }
 
+// Stage events of a timed rebuild (dem_time_rebuild): recorded on the context stream at the
+// stage boundaries when ev is non-null.
+enum { kEvStart, kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount, kEvScan, kEvFillStart,
+       kEvFill, kEvRemap, kEvTranspose, kEvCountAll };
+inline void stage_event(hipEvent_t *ev, int k, hipStream_t st)
+{
+    if (ev) (void)hipEventRecord(ev[k], st);
+}
+
 // Phase A+B of a rebuild: hash, sort, cell ranges, reorder, neighbour counts and scans.
-int rebuild_count(dem_ctx *c, const unsigned *active)
+int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
     unsigned n = c->n;
@@ -363,13 +372,16 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
     else
         hipLaunchKernelGGL(k_hash<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc], n,
                            c->dinst, one, c->kv_hash, c->ctl, active);
+    stage_event(ev, kEvHash, st);
     int rc;
     c->kv_sorted = radix_sort(c->S, c->kv_hash, c->kv_a, c->kv_b, n, c->key_bits, st, active, &rc);
     if (rc) return rc;
+    stage_event(ev, kEvSort, st);
     hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(c->ncells, kBlock), 4096u)), dim3(kBlock), 0, st,
                        c->cstart, c->ncells, kEmpty, active);
     hipLaunchKernelGGL(k_cell_range, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
                        c->cstart, c->cend, active);
+    stage_event(ev, kEvCells, st);
     int nr = 1 - c->rc;
     if (c->batched)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
@@ -384,6 +396,7 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
                            c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, active);
     c->rc = nr;
     c->pc = 1 - c->pc;
+    stage_event(ev, kEvReorder, st);
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
@@ -394,20 +407,23 @@ int rebuild_count(dem_ctx *c, const unsigned *active)
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->ng, c->nl, active);
+    stage_event(ev, kEvCount, st);
     int nl = 1 - c->lc;
     if (scan_excl(c->S, c->ng, n, c->off[nl], c->S.total, st, active)) return DEM_ERR_HIP;
     if (scan_excl(c->S, c->nl, n, c->toff, c->S.total + 1, st, active)) return DEM_ERR_HIP;
     hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, c->ctl, (const unsigned *)c->S.total,
                        (const unsigned *)(c->S.total + 1),
                        std::min(std::min(c->cap[nl], c->out_cap), c->hcap[1 - c->hc]), active);
+    stage_event(ev, kEvScan, st);
     DEM_HIP(hipGetLastError());
     return DEM_OK;
 }
 
 // Phase C: fill the pair list with the remapped history and build the transposed index.
-int rebuild_fill(dem_ctx *c, const unsigned *active)
+int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
 {
     hipStream_t st = c->stream;
+    stage_event(ev, kEvFillStart, st);
     unsigned n = c->n;
     unsigned gb = cdiv(n, kBlock);
     InstDev one = c->hinst[0];
@@ -427,15 +443,18 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->ctl, active);
+    stage_event(ev, kEvFill, st);
     // histories by pair identity, one lane per pair over the device pair count
     const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
     hipLaunchKernelGGL(k_remap, dim3(rg), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const DevCtl *)c->ctl, capn, (const unsigned *)c->cid, c->hist[nh], inv_old,
                        (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
                        (const Hist *)c->hist[oh], c->n_list, capo, c->tape_src, active);
+    stage_event(ev, kEvRemap, st);
     hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
                        c->jf, active);
+    stage_event(ev, kEvTranspose, st);
     DEM_HIP(hipGetLastError());
     c->lc = nl;
     c->hc = nh;
@@ -451,9 +470,11 @@ int rebuild_fill(dem_ctx *c, const unsigned *active)
 typedef int (*pair_hook_fn)(dem_ctx *c, void *arg, unsigned np);
 
 // Host-driven rebuild: count, read the pair total, grow capacities if needed, fill.
-int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullptr)
+int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullptr,
+                 hipEvent_t *ev = nullptr)
 {
-    int rc = rebuild_count(c, nullptr);
+    stage_event(ev, kEvStart, c->stream);
+    int rc = rebuild_count(c, nullptr, ev);
     if (rc) return rc;
     unsigned tot[2];
     DEM_HIP(hipMemcpyAsync(tot, c->S.total, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
@@ -468,7 +489,7 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->npairs, np);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, c->stream, &c->ctl->overflow, 0u);
     if (hook && (rc = hook(c, hook_arg, np))) return rc;
-    if ((rc = rebuild_fill(c, nullptr))) return rc;
+    if ((rc = rebuild_fill(c, nullptr, ev))) return rc;
     if (np > c->hcap[1 - c->hc]) {
         // the buffer the fused step writes next must hold the new list too (its contents
         // were the old list's histories, read by the fill just launched)
@@ -1846,6 +1867,42 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
     return DEM_OK;
 }
 
+// One host-driven rebuild now, timed stage by stage with HIP events on the context stream:
+// out[0..8] = ms of hash, radix sort (all passes), cell ranges, reorder, neighbour count,
+// the two scans (+ capacity check), pair fill, history remap, transpose; out[9] = radix
+// passes; out[10] = pairs of the new list; out[11] = pairs of the old list; out[12] = cells;
+// out[13] = particles.  The trajectory is unchanged (a rebuild at any step gives the same
+// forces: lists hold every contact, sums follow particle order).
+int dem_time_rebuild(dem_ctx *c, double *out)
+{
+    if (!c || !out) return DEM_ERR_INVALID;
+    int rc = ring_flush_all(c);
+    if (rc) return rc;
+    hipEvent_t ev[kEvCountAll];
+    for (int k = 0; k < kEvCountAll; k++) DEM_HIP(hipEventCreate(&ev[k]));
+    const unsigned np_old = c->have_list ? c->npairs : 0u;
+    rc = rebuild_host(c, nullptr, nullptr, ev);
+    if (!rc) rc = hipEventSynchronize(ev[kEvTranspose]) == hipSuccess ? DEM_OK : DEM_ERR_HIP;
+    if (!rc) {
+        static const int a[9] = {kEvStart, kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount,
+                                 kEvFillStart, kEvFill, kEvRemap};
+        static const int b[9] = {kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount, kEvScan,
+                                 kEvFill, kEvRemap, kEvTranspose};
+        for (int k = 0; k < 9; k++) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[a[k]], ev[b[k]]) != hipSuccess) rc = DEM_ERR_HIP;
+            out[k] = ms;
+        }
+        out[9] = (c->key_bits + 7) / 8;
+        out[10] = c->npairs;
+        out[11] = np_old;
+        out[12] = c->ncells;
+        out[13] = c->n;
+    }
+    for (int k = 0; k < kEvCountAll; k++) (void)hipEventDestroy(ev[k]);
+    return rc;
+}
+
 // Accumulated kernel times of dem_step under DEM_OPT_TIME_STEPS (ms), then reset:
 // out[0..3] = split path: first group (pair kernel + walls) total, per-particle kernel total,
 // steps, sum of pair counts; out[4..7] = fused path: walls total, step kernel total, steps,
@@ -2491,6 +2548,9 @@ void pairlist(uint *sjgi, uint *oldsjgi, uint *oldLn, uint *Anei, uint *njgi, ui
                        (const unsigned *)Anei, (const unsigned *)njgi, (const unsigned *)njli,
                        (const double *)olddispt, (const double *)oldfricp, maxParticlesPerCell,
                        numBodies, Aref, Lp, Ln, dispt, fricp);
+    hipLaunchKernelGGL(c_pair_list_partner, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
+                       (const unsigned *)sjgi, (const unsigned *)njgi, (const unsigned *)njli,
+                       maxParticlesPerCell, numBodies, (const unsigned *)Ln, Aref);
     check_launch("pairlist");
 }
 

@@ -814,6 +814,67 @@ def test_gpusphere_with_sidecar_runs_configuration_0(dem, gpu_ok, tmp_path):
         assert np.array_equal(got[k], ref[k]), k
 
 
+def test_gpusphere_ppor_values_equal_the_oracle(dem, orc, gpu_ok, tmp_path):
+    """ppor.dat of the driver's full configuration-0 run (20,000 steps: the lattice falls,
+    its lowest layer reaches the floor near step 13,000 and the contacts grow) against the
+    oracle run of the same workload: at every 1,000-step mark the particle count and
+    contacts / 2 are equal and the
+    porosity, packing fraction, mean coordination number and bed height columns are the same
+    text ("%15.6f", dempacking.cu:758-777), computed from the oracle's contact total and
+    maximum height with the reference's formulas (dempacking.cu:768-770); particle.dat's
+    zone of step 1 and view.dat are the same text as the oracle's state and the deck
+    formatted alike."""
+    import os
+    import shutil
+    import subprocess
+    for f in ("config0.inp", "config0.side"):
+        shutil.copy(os.path.join(dem.DECKS, f), tmp_path / f)
+    exe = os.path.join(dem.HERE, "gpusphere")
+    nsteps = 20000
+    r = subprocess.run([exe, "config0.inp", "--side", "config0.side", "--tstop", str(nsteps),
+                        "--quiet"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = (tmp_path / "ppor.dat").read_text().splitlines()
+    assert len(lines) == nsteps // 1000
+    w = dem.config0()
+    n = w.n
+    sim = orc.OracleSim(w.instances[0], w.pos, w.pdis, w.angv, w.rad, w.rmass, w.inert)
+    pi = math.atan(1.0) * 4.0
+    area = w.instances[0].walls.size[0] * w.instances[0].walls.size[1]
+    expect = []
+    pos1 = None
+    for it in range(1, nsteps + 1):
+        sim.set_iall(1 if (it - 1) % w.rebuild_interval == 0 else 0)
+        assert sim.step(1) == 0
+        if it == 1:
+            pos1 = sim.field("pos").copy()
+        if it % 1000 == 0:
+            sc = sim.scalars()
+            avercn = sc["pcntot"] / (n * 1.0 + 1.0e-30)
+            porosity5 = 1.0 - n * pi / 6 / (area * sc["maxhz"])
+            expect.append("%8d %8d %8d %15.6f %15.6f %15.6f %15.6f" % (
+                it, n, sc["pcntot"] // 2, porosity5, 1.0 - porosity5, avercn, sc["maxhz"]))
+    assert lines == expect
+    # particle.dat: the Tecplot zone of step 1 (dempacking.cu:779-792), positions and radii in
+    # metres, the same text as the oracle's state formatted alike
+    d = dem.read_deck(str(tmp_path / "config0.inp"))
+    zone = ["ZONE T=\"ZONE001\""] + [
+        "%15.6g%15.6g%15.6g%15.6g%8d" % (pos1[i, 0] * d.diam, pos1[i, 1] * d.diam,
+                                       pos1[i, 2] * d.diam, w.rad[i] * d.diam, i)
+        for i in range(n)]
+    assert (tmp_path / "particle.dat").read_text().splitlines() == zone
+    # view.dat: the run description of dempacking.cu:227-234 from the deck values
+    view = ["number of particles=%4d" % n, "diameters =%6.3f" % d.diam,
+            "time step=%f bed width=%6.3f thickniss=%6.3f" % (d.dt, d.dmt1 / d.diam,
+                                                             d.dmt2 / d.diam),
+            "glabal damping=%10.4f bed hbot=%10.4f htop=%10.4f" % (
+                d.cdamp, d.zht1 / d.diam, d.zht1 / d.diam + d.zht2 / d.diam)]
+    assert (tmp_path / "view.dat").read_text().splitlines() == view
+    last = lines[-1].split()
+    assert int(last[2]) > 500, "the spheres must be in contact by the end of the run"
+    say(f"[ppor] {lines[-1]}")
+
+
 @pytest.mark.parametrize("config", ["config1", "config2"])
 def test_gpusphere_sidecar_cylinder_and_bimodal(dem, gpu_ok, tmp_path, config):
     """The driver on the shipped configuration-1 (cylinder, lognormal, random insertion) and
