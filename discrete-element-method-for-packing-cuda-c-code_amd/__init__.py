@@ -168,7 +168,7 @@ EXPORTS = [
     "dem_memzero", "dem_device_sync", "dem_law_pair_eval", "dem_contact_filter", "dem_law_pair_vjp",
     "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
     "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
-    "dem_adj_release", "dem_get_step_timing", "dem_time_rebuild",
+    "dem_adj_release", "dem_get_step_timing", "dem_time_rebuild", "dem_checkpoint_layout",
 ]
 
 
@@ -209,6 +209,7 @@ def lib() -> ctypes.CDLL:
         "dem_time_kernels": ([P, c_int, P, P], c_int),
         "dem_get_step_timing": ([P, P], c_int),
         "dem_time_rebuild": ([P, P], c_int),
+        "dem_checkpoint_layout": ([P], c_int),
         "dem_get_hash": ([P, P], c_int),
         "dem_get_sorted": ([P, P], c_int),
         "dem_get_cell_start": ([P, P, c_uint], c_int),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <mutex>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 #include "adjoint.hpp"
 #include "law_adj.hpp"
 #include "sort.hpp"
+#include "ckpt.hpp"
 
 using namespace dem;
 
@@ -1267,74 +1269,27 @@ int restore_state(dem_ctx *c, const StateSrc &S, hipMemcpyKind kind)
 
 // Checkpoint file: a fixed header, the instance table, then the state arrays in the
 // library's slot layout, then an FNV-1a 64-bit checksum of everything before it.
-constexpr char kCkptMagic[8] = {'D', 'E', 'M', 'C', 'K', 'P', 'T', '1'};
-constexpr unsigned kCkptVersion = 1, kCkptEndian = 0x01020304u;
+using dem_ckpt::Header;
 
-struct CkptHeader {
-    char magic[8];
-    unsigned version, endian;
-    unsigned sz_rec, sz_hist, sz_inst, sz_ctl;
-    unsigned ninst, cap_n, n, npairs;
-    unsigned have_list, need_rebuild, since, last_flag;
-    unsigned n_list, interval, pad0, pad1;
-    unsigned long long steps, rebuilds;
-};
-static_assert(sizeof(CkptHeader) == 96, "checkpoint header layout");
-
-struct Fnv64 {
-    unsigned long long h = 1469598103934665603ull;
-    void add(const void *p, size_t bytes)
-    {
-        const unsigned char *b = static_cast<const unsigned char *>(p);
-        for (size_t k = 0; k < bytes; k++) {
-            h ^= b[k];
-            h *= 1099511628211ull;
-        }
-    }
-};
-
-struct CkptWriter {
-    std::FILE *fp;
-    Fnv64 sum;
-    bool ok = true;
-    void put(const void *p, size_t bytes)
-    {
-        if (!bytes) return;
-        sum.add(p, bytes);
-        ok = ok && std::fwrite(p, 1, bytes, fp) == bytes;
-    }
-};
-
-struct CkptReader {
-    std::FILE *fp;
-    Fnv64 sum;
-    bool ok = true;
-    void get(void *p, size_t bytes)
-    {
-        if (!bytes || !ok) return;
-        ok = std::fread(p, 1, bytes, fp) == bytes;
-        if (ok) sum.add(p, bytes);
-    }
-};
-
-// Host-side consistency of a list read from a file, so a damaged file can never make a
-// kernel index out of bounds: CSR offsets monotone and ending at npairs, slots < n, transposed
-// entries pairs or empty, wall candidates < n.
-bool ckpt_list_ok(unsigned n, unsigned np, const std::vector<unsigned> &off,
-                  const std::vector<unsigned> &toff, const std::vector<uint2> &ab,
-                  const std::vector<unsigned> &jt, const std::vector<unsigned> &wlist,
-                  unsigned nwall)
+// Layout of the structures the checkpoint reader checks (ckpt.hpp works on raw bytes).
+dem_ckpt::Layout ckpt_layout()
 {
-    if (off[0] != 0 || toff[0] != 0 || off[n] != np || toff[n] != np || nwall > n) return false;
-    for (unsigned s = 0; s < n; s++)
-        if (off[s] > off[s + 1] || toff[s] > toff[s + 1]) return false;
-    for (unsigned k = 0; k < np; k++) {
-        if (ab[k].x >= n || ab[k].y >= n) return false;
-        if (jt[k] >= np && jt[k] != kEmpty) return false;
-    }
-    for (unsigned k = 0; k < nwall; k++)
-        if (wlist[k] >= n) return false;
-    return true;
+    dem_ckpt::Layout L;
+    L.sz_rec = sizeof(PRec);
+    L.sz_hist = sizeof(Hist);
+    L.sz_inst = sizeof(InstDev);
+    L.sz_ctl = sizeof(DevCtl);
+    L.rec_inst = offsetof(PRec, inst);
+    L.rec_id = offsetof(PRec, id);
+    L.inst_p_off = offsetof(InstDev, p_off);
+    L.inst_n = offsetof(InstDev, n);
+    L.inst_cell_off = offsetof(InstDev, cell_off);
+    L.inst_gsize = offsetof(InstDev, gsize);
+    L.ctl_npairs = offsetof(DevCtl, npairs);
+    L.ctl_overflow = offsetof(DevCtl, overflow);
+    L.ctl_nwall = offsetof(DevCtl, nwall);
+    L.ctl_zero = offsetof(DevCtl, zero);
+    return L;
 }
 
 }  // namespace
@@ -1369,11 +1324,11 @@ int dem_save_checkpoint(dem_ctx *c, const char *path)
             DEM_HIP(hipMemcpy(jt.data(), c->jt, sizeof(unsigned) * np, hipMemcpyDeviceToHost));
         }
     }
-    CkptHeader h;
+    Header h;
     std::memset(&h, 0, sizeof(h));
-    std::memcpy(h.magic, kCkptMagic, sizeof(h.magic));
-    h.version = kCkptVersion;
-    h.endian = kCkptEndian;
+    std::memcpy(h.magic, dem_ckpt::kMagic, sizeof(h.magic));
+    h.version = dem_ckpt::kVersion;
+    h.endian = dem_ckpt::kEndian;
     h.sz_rec = sizeof(PRec);
     h.sz_hist = sizeof(Hist);
     h.sz_inst = sizeof(InstDev);
@@ -1396,7 +1351,7 @@ int dem_save_checkpoint(dem_ctx *c, const char *path)
     std::snprintf(tmp.data(), tmp.size(), "%s.part", path);
     std::FILE *fp = std::fopen(tmp.data(), "wb");
     if (!fp) return DEM_ERR_IO;
-    CkptWriter w{fp};
+    dem_ckpt::Writer w{fp};
     w.put(&h, sizeof(h));
     w.put(c->hinst.data(), sizeof(InstDev) * c->hinst.size());
     w.put(rec.data(), sizeof(PRec) * N);
@@ -1422,69 +1377,48 @@ int dem_save_checkpoint(dem_ctx *c, const char *path)
     return DEM_OK;
 }
 
+int dem_checkpoint_layout(uint *out)
+{
+    if (!out) return DEM_ERR_INVALID;
+    const dem_ckpt::Layout L = ckpt_layout();
+    const unsigned v[14] = {L.sz_rec, L.sz_hist, L.sz_inst, L.sz_ctl, L.rec_inst, L.rec_id,
+                            L.inst_p_off, L.inst_n, L.inst_cell_off, L.inst_gsize,
+                            L.ctl_npairs, L.ctl_overflow, L.ctl_nwall, L.ctl_zero};
+    std::memcpy(out, v, sizeof(v));
+    return DEM_OK;
+}
+
 int dem_load_checkpoint(dem_ctx *c, const char *path)
 {
     if (!c || !path) return DEM_ERR_INVALID;
-    std::FILE *fp = std::fopen(path, "rb");
-    if (!fp) return DEM_ERR_IO;
-    CkptReader r{fp};
-    CkptHeader h;
-    r.get(&h, sizeof(h));
-    auto bad = [&](int code) {
-        std::fclose(fp);
-        return code;
-    };
-    if (!r.ok || std::memcmp(h.magic, kCkptMagic, sizeof(h.magic)) != 0 ||
-        h.version != kCkptVersion || h.endian != kCkptEndian || h.sz_rec != sizeof(PRec) ||
-        h.sz_hist != sizeof(Hist) || h.sz_inst != sizeof(InstDev) || h.sz_ctl != sizeof(DevCtl))
-        return bad(DEM_ERR_IO);
-    // the context must have been created for the same instances (particle counts)
-    if (h.ninst != (unsigned)c->ninst || h.cap_n != c->cap_n || h.n == 0 || h.n > h.cap_n ||
-        (c->ninst > 1 && h.n != h.cap_n) || h.npairs > 0xfffffff0u || (!h.have_list && h.npairs))
-        return bad(DEM_ERR_INVALID);
-    std::vector<InstDev> inst(h.ninst);
-    r.get(inst.data(), sizeof(InstDev) * h.ninst);
-    if (!r.ok) return bad(DEM_ERR_IO);
-    unsigned long long cells = 0;
-    for (unsigned i = 0; i < h.ninst; i++) {
-        const InstDev &a = inst[i], &b = c->hinst[i];
-        if (a.p_off != b.p_off || a.n != b.n || a.cell_off != cells) return bad(DEM_ERR_INVALID);
-        cells += (unsigned long long)a.gsize[0] * a.gsize[1] * a.gsize[2];
-        if (cells >= 0xffffffffull) return bad(DEM_ERR_INVALID);
+    dem_ckpt::Expect E;
+    E.ninst = (unsigned)c->ninst;
+    E.cap_n = c->cap_n;
+    for (const InstDev &I : c->hinst) {
+        E.p_off.push_back(I.p_off);
+        E.n.push_back(I.n);
     }
-    const size_t N = h.cap_n, n = h.n, np = h.npairs;
+    dem_ckpt::Data d;
+    int rc0 = dem_ckpt::read(path, ckpt_layout(), E, d);
+    if (rc0) return rc0;
+    const Header &h = d.h;
+    const unsigned long long cells = d.cells;
+    const size_t N = h.cap_n, np = h.npairs;
+    std::vector<InstDev> inst(h.ninst);
+    std::memcpy(inst.data(), d.inst.data(), sizeof(InstDev) * h.ninst);
     std::vector<PRec> rec(N);
     std::vector<Hist> wh(N), hist(np);
-    std::vector<unsigned> inv(N), off, toff, wflag, wlist, jt(np);
     std::vector<uint2> ab(np);
     DevCtl ctl;
-    r.get(rec.data(), sizeof(PRec) * N);
-    r.get(wh.data(), sizeof(Hist) * N);
-    r.get(inv.data(), sizeof(unsigned) * N);
-    r.get(&ctl, sizeof(ctl));
-    if (h.have_list) {
-        off.resize(n + 1);
-        toff.resize(n + 1);
-        wflag.resize(n);
-        wlist.resize(n);
-        r.get(off.data(), sizeof(unsigned) * (n + 1));
-        r.get(toff.data(), sizeof(unsigned) * (n + 1));
-        r.get(wflag.data(), sizeof(unsigned) * n);
-        r.get(wlist.data(), sizeof(unsigned) * n);
-        r.get(ab.data(), sizeof(uint2) * np);
-        r.get(hist.data(), sizeof(Hist) * np);
-        r.get(jt.data(), sizeof(unsigned) * np);
+    std::memcpy(rec.data(), d.rec.data(), sizeof(PRec) * N);
+    std::memcpy(wh.data(), d.wh.data(), sizeof(Hist) * N);
+    std::memcpy(&ctl, d.ctl.data(), sizeof(DevCtl));
+    if (h.have_list && np) {
+        std::memcpy(ab.data(), d.ab.data(), sizeof(uint2) * np);
+        std::memcpy(hist.data(), d.hist.data(), sizeof(Hist) * np);
     }
-    unsigned long long want = r.sum.h, sum = 0;
-    bool ok = r.ok && std::fread(&sum, sizeof(sum), 1, fp) == 1 && sum == want;
-    std::fclose(fp);
-    if (!ok) return DEM_ERR_IO;
-    for (size_t s = 0; s < N; s++)
-        if (rec[s].id >= N || inv[s] >= N || rec[s].inst >= h.ninst) return DEM_ERR_IO;
-    // the kernels bound their pair loops by the control block's count and gate on its zero word
-    if (ctl.npairs != h.npairs || ctl.overflow != 0u || ctl.zero != 0u) return DEM_ERR_IO;
-    if (h.have_list && !ckpt_list_ok((unsigned)n, (unsigned)np, off, toff, ab, jt, wlist, ctl.nwall))
-        return DEM_ERR_IO;
+    std::vector<unsigned> &inv = d.inv, &off = d.off, &toff = d.toff, &wflag = d.wflag,
+                          &wlist = d.wlist, &jt = d.jt;
 
     adj_free(c);   // tapes and checkpoints of another trajectory
     c->hinst = inst;

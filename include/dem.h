@@ -271,6 +271,12 @@ int dem_get_wall_history(dem_ctx *ctx, double *disptw, double *fricpw);
  * other sizes. */
 int dem_save_checkpoint(dem_ctx *ctx, const char *path);
 int dem_load_checkpoint(dem_ctx *ctx, const char *path);
+/* Layout the checkpoint reader checks (host only, no device): out[0..3] = bytes of a particle
+ * record, a history, an instance entry, the control block; out[4..5] = offsets of the
+ * record's instance and index words; out[6..9] = offsets of the instance entry's particle
+ * offset, count, cell offset and grid size; out[10..13] = offsets of the control block's
+ * pair count, overflow, wall-candidate count and zero word (out holds 14 uints). */
+int dem_checkpoint_layout(uint *out);
 
 /* Single-instance contexts: make particles [0, n) active; the rest of the state set by
  * dem_set_state waits, as the reference's not-yet-inserted layers do (dempacking.cu:378-462).
