@@ -474,17 +474,19 @@ def bench_batched(dem, steps, plain_steps, interval, tune_steps=48, graph=True):
             "trigger_in_last_group": int(flag), "autotune": tune}
 
 
-def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True):
+def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True, roll_floor=3.0):
     """Differentiable form on the 20K-sphere bed: checkpointed forward + adjoint backward
     against the plain forward (cost ratio), and the adjoint gradient of phi and CN in
-    (mu, log E, e, mu_r) against central differences of the plain forward."""
+    (mu, log E, e, mu_r) against central differences of the plain forward -- with the
+    form's rolling floor (roll_floor = k; 0: the reference's rolling moment, regularised in
+    the reverse pass only)."""
     import copy
     A = dem.adjoint
     t0 = time.perf_counter()
     w = dem.settle_bed(dem.config4(), tap=0.0, graph=graph)
     log(f"adjoint workload: n={w.n}, bed top {w.meta['bed_top']:.2f} ({time.perf_counter() - t0:.1f} s)")
     obs = dem.make_obs(zhi=w.meta["bed_top"] - 3.0, margin=2.0, zlo=2.0, eps=0.5, eps_c=1e-3)
-    prob = A.PackingProblem(w, nsteps, every, obs)
+    prob = A.PackingProblem(w, nsteps, every, obs, roll_floor=roll_floor)
     th0 = [0.3, math.log(1e7), 0.6, 0.002]
     prob.evaluate([th0])          # warm-up
     phi_p, cn_p = prob.evaluate([th0])
@@ -513,7 +515,7 @@ def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True):
     r["fd_cn"][2] = (vals[0][1][0] - vals[1][1][0]) / (2 * fd_rel * gam) * dg
     rel = lambda a, b: abs(a - b) / max(abs(a), abs(b), 1e-300)
     out = {"workload": w.name, "n": w.n, "steps": nsteps, "checkpoint_every": every,
-           "phi": float(phi[0]), "cn": float(cn[0]),
+           "roll_floor": roll_floor, "phi": float(phi[0]), "cn": float(cn[0]),
            "plain_s": t_plain, "forward_s": t_fwd, "backward_s": t_bwd,
            "cost_ratio": (t_fwd + t_bwd) / t_plain, "backward_over_forward": t_bwd / t_fwd,
            "device_ms": {"plain": ms_plain, "forward": info["forward_ms"],
@@ -543,7 +545,7 @@ def norm_err(adj, fd, theta):
 
 
 def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, iters=16,
-                      seed=7, graph=True):
+                      seed=7, graph=True, roll_floor=3.0):
     """Calibration of (mu, log E, e, mu_r) to the observables of a bed run by the plain method
     at seeded hidden parameters: `ninst` seeded starts in one batched context (replicas of one
     settled bed), one forward + adjoint backward per iteration."""
@@ -557,14 +559,14 @@ def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, 
         return np.stack([rng.uniform(0.2, 0.6, k), np.log(rng.uniform(3e6, 3e7, k)),
                          rng.uniform(0.4, 0.8, k), rng.uniform(0.001, 0.005, k)], axis=1)
     hidden = draw(1)[0]
-    tgt = A.PackingProblem(w1, nsteps, every, obs)
+    tgt = A.PackingProblem(w1, nsteps, every, obs, roll_floor=roll_floor)
     tphi, tcn = tgt.evaluate([hidden])
     del tgt
     insts = [copy_instance(w1.instances[0]) for _ in range(ninst)]
     rep = lambda a: np.concatenate([a] * ninst)
     big = dem.Workload(f"calibration-{ninst}x{n}", insts, rep(w1.pos), rep(w1.pdis), rep(w1.angv),
                        rep(w1.rad), rep(w1.rmass), rep(w1.inert), rebuild_interval=0)
-    prob = A.PackingProblem(big, nsteps, every, obs)
+    prob = A.PackingProblem(big, nsteps, every, obs, roll_floor=roll_floor)
     starts = draw(ninst)
     log(f"calibration: {ninst} x {n} spheres, target phi {tphi[0]:.5f} cn {tcn[0]:.4f} "
         f"({time.perf_counter() - t0:.1f} s setup)")
@@ -696,6 +698,12 @@ def main():
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--calib-instances", type=int, default=64)
     ap.add_argument("--calib-n", type=int, default=2000)
+    ap.add_argument("--fd-rel", type=float, default=1e-4,
+                    help="relative step of the adjoint leg's central differences")
+    ap.add_argument("--roll-floor", type=float, default=3.0,
+                    help="differentiable form: rolling floor k (0: the reference's moment)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg (and calibration unless --no-calibration)")
     ap.add_argument("--split", action="store_true",
                     help="force the split path (pair kernel + gather); default: auto")
     ap.add_argument("--fused", action="store_true", help="force the fused step; default: auto")
@@ -725,6 +733,13 @@ def main():
         log(f"copy probe: 3 device-to-device copies of {nb} bytes")
     split_opt = True if args.split else (False if args.fused else None)
     graph = not args.no_graph
+    if args.adjoint_only:
+        a = bench_adjoint(dem, graph=graph, roll_floor=args.roll_floor, fd_rel=args.fd_rel)
+        if not args.no_calibration:
+            a["calibration"] = bench_calibration(dem, ninst=args.calib_instances, n=args.calib_n,
+                                                 graph=graph, roll_floor=args.roll_floor)
+        log(json.dumps(a))
+        return
     t0 = time.perf_counter()
     w = dem.config1(n=args.n)
     log(f"workload {w.name}: n={w.n} height={w.meta['height']:.1f} ({time.perf_counter() - t0:.1f} s)")
@@ -760,10 +775,11 @@ def main():
         t0 = time.perf_counter()
         import torch  # noqa: F401
         log(f"torch imported ({time.perf_counter() - t0:.1f} s)")
-        adjoint = bench_adjoint(dem, graph=graph)
+        adjoint = bench_adjoint(dem, graph=graph, roll_floor=args.roll_floor, fd_rel=args.fd_rel)
         if not args.no_calibration:
             adjoint["calibration"] = bench_calibration(dem, ninst=args.calib_instances,
-                                                       n=args.calib_n, graph=graph)
+                                                       n=args.calib_n, graph=graph,
+                                                       roll_floor=args.roll_floor)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dem, ws, dem.config0(), args.cpu_seconds)

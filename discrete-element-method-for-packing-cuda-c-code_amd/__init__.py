@@ -51,6 +51,7 @@ OPT_PIPELINE = 11
 OPT_FUSED_FAST = 12
 OPT_XCD_CHUNK = 13
 OPT_PIPELINE_BATCH = 14
+OPT_ROLL_FLOOR = 15
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0
@@ -169,6 +170,8 @@ EXPORTS = [
     "dem_law_walls_eval", "dem_law_walls_vjp", "dem_set_material", "dem_observe",
     "dem_adj_forward", "dem_adj_backward", "dem_adj_state_grad", "dem_adj_info",
     "dem_adj_release", "dem_get_step_timing", "dem_time_rebuild", "dem_checkpoint_layout",
+    "dem_law_pair_eval_floor", "dem_law_pair_vjp_floor", "dem_law_walls_eval_floor",
+    "dem_law_walls_vjp_floor",
 ]
 
 
@@ -227,6 +230,10 @@ def lib() -> ctypes.CDLL:
         "dem_law_pair_vjp": ([P, P, P, P, P], c_int),
         "dem_law_walls_eval": ([P, P, P, P], c_int),
         "dem_law_walls_vjp": ([P, P, P, P, P, P], c_int),
+        "dem_law_pair_eval_floor": ([P, P, c_double, P], c_int),
+        "dem_law_pair_vjp_floor": ([P, P, c_double, P, P, P], c_int),
+        "dem_law_walls_eval_floor": ([P, P, P, c_double, P], c_int),
+        "dem_law_walls_vjp_floor": ([P, P, P, c_double, P, P, P], c_int),
         "dem_set_material": ([P, c_int, P], c_int),
         "dem_observe": ([P, P, P, P], c_int),
         "dem_adj_forward": ([P, c_int, c_int, P, P, P], c_int),

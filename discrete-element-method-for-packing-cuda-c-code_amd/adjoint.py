@@ -21,8 +21,8 @@ import time
 
 import numpy as np
 
-from . import (OPT_ADJ_ROLL_REG, Context, Material, Obs, Workload, damping_for_restitution,
-               reduced_material)
+from . import (OPT_ADJ_ROLL_REG, OPT_ROLL_FLOOR, Context, Material, Obs, Workload,
+               damping_for_restitution, reduced_material)
 
 PARAMS = ("mu", "logE", "e", "mu_r")
 
@@ -55,12 +55,19 @@ class MaterialMap:
 
 class PackingProblem:
     """One context over a workload: fixed initial state, `nsteps` steps, checkpoints every
-    `every` steps, observables on `obs`.  theta has one row per instance.  `roll_reg` is the
-    rolling-moment regularisation of the reverse pass (0: exact reverse mode; 1: |w_rel|
-    floored at the one-step chatter scale, stable over long windows of resting contact)."""
+    `every` steps, observables on `obs`.  theta has one row per instance.
+
+    `roll_floor` (DEM_OPT_ROLL_FLOOR) floors the rolling moment's |w_rel| in the forward at k
+    times the one-step chatter scale: the moment is viscous there instead of flipping sign
+    every step about rolling rest, the forward is a Lipschitz function of its inputs and the
+    adjoint is its exact reverse mode.  With roll_floor = 0 the forward is the reference's
+    moment and `roll_reg` regularises the reverse pass only (0: exact reverse mode, unstable
+    over long windows of resting contact; 1: |w_rel| floored at the chatter scale in the
+    derivative)."""
 
     def __init__(self, w: Workload, nsteps: int, every: int, obs: Obs,
-                 mmap: MaterialMap | None = None, roll_reg: float = 1.0):
+                 mmap: MaterialMap | None = None, roll_reg: float = 1.0,
+                 roll_floor: float = 0.0):
         self.w = w
         self.nsteps = int(nsteps)
         self.every = int(every)
@@ -68,6 +75,8 @@ class PackingProblem:
         self.mmap = mmap or MaterialMap()
         self.ctx = Context(w.instances)
         self.ctx.set_option(OPT_ADJ_ROLL_REG, roll_reg)
+        self.ctx.set_option(OPT_ROLL_FLOOR, roll_floor)
+        self.roll_floor = float(roll_floor)
         self.ninst = len(w.instances)
         self.theta = None
         self.timing = {}

@@ -70,9 +70,13 @@ __global__ __launch_bounds__(kBlock) void k_adj_particle(const PRec *tr, const H
         hb = d3{0.0, 0.0, 0.0};
         LawParamsBar Pb = {0.0, 0.0, 0.0, 0.0, 0.0};
         const Hist H = twh[a];
+        // a forward with the rolling floor (I.roll_k > 0) is reversed exactly; otherwise
+        // roll_k floors the derivative only
+        const bool fwd = I.roll_k > 0.0;
+        const double rc = fwd ? roll_floor(I.roll_k, dt, R.inert, 0.0) : roll_k * dt * dt / R.inert;
         law_walls_adj(I.walls, d3{R.px, R.py, R.pz}, d3{R.vx, R.vy, R.vz}, d3{R.wx, R.wy, R.wz},
                       R.rad, R.mass, d3{H.hx, H.hy, H.hz}, dt, I.pw, Fb, Tb, houtb, xb, vb, wb,
-                      hb, Pb, roll_k * dt * dt / R.inert);
+                      hb, Pb, rc, fwd);
         double *T = thacc + (size_t)kThetaPer * R.id;
         T[5] += Pb.estar; T[6] += Pb.mu; T[7] += Pb.gn; T[8] += Pb.gt; T[9] += Pb.ur;
     }
@@ -90,7 +94,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_adj_pair(const uint2 *ab, unsigned np, const PRec *tr,
                                                      const Hist *th, const double *ftb, double *hb,
                                                      PAdj *pa, const InstDev *inst, LawParams pp,
-                                                     double dt, double roll_k)
+                                                     double dt, double fwd_k, double roll_k)
 {
     const unsigned per = (np + gridDim.x - 1) / gridDim.x;   // contiguous, XCD-aware ranges
     const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
@@ -100,12 +104,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(const uint2 *ab, unsigned n
         const PRec &A = tr[e.x];
         const PRec &B = tr[e.y];
         LawParams P = pp;
-        double tdt = dt;
+        double tdt = dt, fk = fwd_k;
         if (BATCHED) {
             const InstDev &I = inst[A.inst];
             P = I.pp;
             tdt = I.dt;
+            fk = I.roll_k;
         }
+        const bool fwd = fk > 0.0;   // the forward's rolling floor, reversed exactly
+        const double rc = fwd ? roll_floor(fk, tdt, A.inert, B.inert)
+                              : roll_k * tdt * tdt * (1.0 / A.inert + 1.0 / B.inert);
         const double *Fa = ftb + 6 * (size_t)e.x;
         const double *Fb = ftb + 6 * (size_t)e.y;
         d3 fab = d3{Fa[0] - Fb[0], Fa[1] - Fb[1], Fa[2] - Fb[2]};
@@ -120,8 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_pair(const uint2 *ab, unsigned n
         law_pair_adj(d3{A.px, A.py, A.pz}, d3{B.px, B.py, B.pz}, d3{A.vx, A.vy, A.vz},
                      d3{B.vx, B.vy, B.vz}, d3{A.wx, A.wy, A.wz}, d3{B.wx, B.wy, B.wz}, A.rad,
                      B.rad, A.mass, B.mass, d3{H.hx, H.hy, H.hz}, tdt, P, fab, tab, tbb, houtb,
-                     xab, xbb, vab, vbb, wab, wbb, hinb, Pb,
-                     roll_k * tdt * tdt * (1.0 / A.inert + 1.0 / B.inert));
+                     xab, xbb, vab, vbb, wab, wbb, hinb, Pb, rc, fwd);
         PAdj O;
         stv3(O.a, xab); stv3(O.a + 3, vab); stv3(O.a + 6, wab);
         O.th[0] = Pb.estar; O.th[1] = Pb.mu; O.th[2] = Pb.gn; O.th[3] = Pb.gt; O.th[4] = Pb.ur;

@@ -48,7 +48,8 @@ static_assert(sizeof(POut) == 80, "pair output is five 16-byte chunks");
 struct InstDev {
     LawParams pp;          // particle-particle
     LawParams pw;          // particle-wall
-    double dt, skin, ann2, pad0;
+    double dt, skin, ann2;
+    double roll_k;         // rolling-moment floor of the differentiable form, k (0: reference)
     double origin[3], cell[3];
     WallGeom walls;
     unsigned gsize[3], cell_off;
@@ -83,6 +84,15 @@ __device__ __forceinline__ void st2(double *p, double a, double b)
 }
 
 __device__ __forceinline__ bool is_active(const unsigned *active) { return !active || *active != 0u; }
+
+// roll_c of the law for bodies of inertias ia, ib (ib = 0: a wall) under floor factor k:
+// k dt^2 (1/I_a + 1/I_b), the change one step of a rolling moment of unit size makes to the
+// relative spin per unit moment; 0 when k = 0 (the reference's moment).
+__device__ __forceinline__ double roll_floor(double k, double dt, double ia, double ib)
+{
+    if (!(k > 0.0)) return 0.0;
+    return k * dt * dt * (ib > 0.0 ? 1.0 / ia + 1.0 / ib : 1.0 / ia);
+}
 
 
 
@@ -489,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
                                                        const PRec *rec, const Hist *hist,
                                                        Hist *hist_out, POut *out,
                                                        const InstDev *inst, LawParams pp,
-                                                       double dt, Hist *tape_hist,
+                                                       double dt, double roll_k, Hist *tape_hist,
                                                        const unsigned *gate, unsigned xchunk)
 {
     if (gated_off(gate)) return;
@@ -516,18 +526,19 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
             st2(T + 2, h1.x, h1.y);
         }
         LawParams P = pp;
-        double tdt = dt;
+        double tdt = dt, rk = roll_k;
         if (BATCHED) {
             const InstDev &I = inst[rec[e.x].inst];
             P = I.pp;
             tdt = I.dt;
+            rk = I.roll_k;
         }
         d3 fa, ta, tb, ho;
         double fo;
         bool c = law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
                           d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x},
                           a1.y, b1.y, a3.y, b3.y, d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho,
-                          fo);
+                          fo, roll_floor(rk, tdt, a5.y, b5.y));
         double *O = reinterpret_cast<double *>(out + p);
         st2(O, fa.x, fa.y);
         st2(O + 2, fa.z, c ? 1.0 : 0.0);
@@ -563,7 +574,7 @@ __device__ __forceinline__ int wall_slot(const double *R, const InstDev &I, Hist
     double fow;
     int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
                          d3{r4.x, r4.y, r5.x}, r1.y, r3.y, d3{w0.x, w0.y, w1.x}, I.dt, I.pw,
-                         fw, tw, hw, fow);
+                         fw, tw, hw, fow, roll_floor(I.roll_k, I.dt, r5.y, 0.0));
     if (hits) {
         st2(WH, hw.x, hw.y);
         st2(WH + 2, hw.z, fow);
@@ -735,23 +746,24 @@ __global__ __launch_bounds__(kBlock) void k_particle(
 template <bool BATCHED>
 __device__ __forceinline__ bool pair_from_records(const double *A, const double *B, d3 h,
                                                   const InstDev *inst, const LawParams &pp,
-                                                  double dt, unsigned inst_a, d3 &fa, d3 &ta,
-                                                  d3 &tb, d3 &ho, double &fo)
+                                                  double dt, double roll_k, unsigned inst_a,
+                                                  d3 &fa, d3 &ta, d3 &tb, d3 &ho, double &fo)
 {
     double2 a0 = ld2(A), a1 = ld2(A + 2), a2 = ld2(A + 4), a3 = ld2(A + 6), a4 = ld2(A + 8),
             a5 = ld2(A + 10);
     double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6), b4 = ld2(B + 8),
             b5 = ld2(B + 10);
     LawParams P = pp;
-    double tdt = dt;
+    double tdt = dt, rk = roll_k;
     if (BATCHED) {
         const InstDev &I = inst[inst_a];
         P = I.pp;
         tdt = I.dt;
+        rk = I.roll_k;
     }
     return law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
                     d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x}, a1.y, b1.y,
-                    a3.y, b3.y, h, tdt, P, fa, ta, tb, ho, fo);
+                    a3.y, b3.y, h, tdt, P, fa, ta, tb, ho, fo, roll_floor(rk, tdt, a5.y, b5.y));
 }
 
 // Fused step (one lane per particle): the lane evaluates the law of each of its own pairs
@@ -1075,11 +1087,12 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                          wp = d3{b4.x, b4.y, b5.x};
                 const double rs = s1.y, ms = s3.y, rp = b1.y, mp = b3.y;
                 LawParams P = one.pp;
-                double tdt = one.dt;
+                double tdt = one.dt, rk = one.roll_k;
                 if (BATCHED) {
                     const InstDev &IO = inst[reinterpret_cast<const uint2 *>(S + 15)->x];
                     P = IO.pp;
                     tdt = IO.dt;
+                    rk = IO.roll_k;
                 }
                 d3 fa, ta, tb, ho;
                 double fo;
@@ -1087,7 +1100,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 rc = law_pair(sel3(own, xs, xp), sel3(own, xp, xs), sel3(own, vs, vp),
                               sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
                               own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
-                              d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo);
+                              d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo,
+                              roll_floor(rk, tdt, own ? s5.y : b5.y, own ? b5.y : s5.y));
                 if (!valid) {
                     rc = false;
                     fa = ta = tb = d3{0.0, 0.0, 0.0};
@@ -1154,7 +1168,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 d3 fa, ta, tb, ho;
                 double fo;
                 bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
-                                                      one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                                                      one.pp, one.dt, one.roll_k, inst_a, fa, ta, tb, ho,
+                                                      fo);
                 double *HW = reinterpret_cast<double *>(hout + p);
                 st2(HW, ho.x, ho.y);
                 st2(HW + 2, ho.z, fo);
@@ -1188,7 +1203,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 d3 fa, ta, tb, ho;
                 double fo;
                 bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
-                                                      one.pp, one.dt, inst_a, fa, ta, tb, ho, fo);
+                                                      one.pp, one.dt, one.roll_k, inst_a, fa, ta, tb, ho,
+                                                      fo);
                 fx -= fa.x; fy -= fa.y; fz -= fa.z;
                 tx += tb.x; ty += tb.y; tz += tb.z;
                 c += con ? 1u : 0u;

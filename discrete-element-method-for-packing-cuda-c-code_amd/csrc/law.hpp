@@ -50,9 +50,14 @@ constexpr double kXSmall = 1e-30;   // neiglist.cu:29
 // rs, ms effective radius and mass; vrel: displacement increment of the other body's
 // contact point relative to i's; wrel = w_i - w_other; hin: incoming tangential history.
 // Outputs: force on i, tangential force on i, rolling moment on i, new history, |Ft|.
+// roll_c > 0 (the differentiable form's option, off by default) floors the |w_rel| of the
+// rolling moment at roll_c mu_r fcn: below it the moment is viscous, -w_rel / roll_c, instead
+// of flipping its sign every step about rolling rest (law_core_adj gives the reasoning).
+// roll_c = 0 is the reference's moment -mu_r fcn w_rel / |w_rel| bit for bit.
 DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
                                          d3 wrel, d3 hin, double dt, const LawParams &P,
-                                         d3 &fi, d3 &ft, d3 &mr, d3 &hout, double &fout)
+                                         d3 &fi, d3 &ft, d3 &mr, d3 &hout, double &fout,
+                                         double roll_c = 0.0)
 {
     double vns = vrel.x * n.x + vrel.y * n.y + vrel.z * n.z;
     d3 vn = d3{vns * n.x, vns * n.y, vns * n.z};
@@ -107,7 +112,8 @@ DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
 
     double wrm = sqrt(dot3(wrel, wrel));
     if (wrm > kXSmall) {
-        double m = P.ur * fcn / wrm;
+        double wfl = roll_c * P.ur * fcn;
+        double m = P.ur * fcn / (wrm > wfl ? wrm : wfl);
         mr = d3{-m * wrel.x, -m * wrel.y, -m * wrel.z};
     } else {
         mr = d3{0.0, 0.0, 0.0};
@@ -155,7 +161,7 @@ DEM_HD bool maybe_touch(float4 a, float4 b)
 DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
                                          double rb, double ma, double mb, d3 hin, double dt,
                                          const LawParams &P, d3 &fa, d3 &ta, d3 &tb,
-                                         d3 &hout, double &fout)
+                                         d3 &hout, double &fout, double roll_c = 0.0)
 {
     d3 dx = d3{xb.x - xa.x, xb.y - xa.y, xb.z - xa.z};
     double dist = sqrt(dot3(dx, dx));
@@ -174,7 +180,7 @@ DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
     d3 vrel = d3{(vb.x - va.x) - c.x, (vb.y - va.y) - c.y, (vb.z - va.z) - c.z};
     d3 wrel = d3{wa.x - wb.x, wa.y - wb.y, wa.z - wb.z};
     d3 ft, mr;
-    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout);
+    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout, roll_c);
     d3 ca = cross3(scale3(n, ra), ft);
     ta = d3{ca.x + mr.x, ca.y + mr.y, ca.z + mr.z};
     d3 cb = cross3(scale3(n, rb), ft);
@@ -215,7 +221,8 @@ DEM_HD double wall_overlap(const WallGeom &g, int k, d3 x, double r, d3 &n)
 // Returns the number of walls hit.
 DEM_HD int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
                                          double m, d3 hin, double dt, const LawParams &P,
-                                         d3 &fw, d3 &tw, d3 &hout, double &fout)
+                                         d3 &fw, d3 &tw, d3 &hout, double &fout,
+                                         double roll_c = 0.0)
 {
     int nw = wall_count(g);
     int prim = -1;
@@ -242,7 +249,7 @@ DEM_HD int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
         d3 f, ft, mr, ht;
         double fo;
         law_core(n, dn, r, m, vrel, w, k == prim ? hin : d3{0.0, 0.0, 0.0}, dt, P, f, ft, mr,
-                 ht, fo);
+                 ht, fo, roll_c);
         d3 cr = cross3(scale3(n, r), ft);
         fw = d3{fw.x + f.x, fw.y + f.y, fw.z + f.z};
         tw = d3{tw.x + (cr.x + mr.x), tw.y + (cr.y + mr.y), tw.z + (cr.z + mr.z)};

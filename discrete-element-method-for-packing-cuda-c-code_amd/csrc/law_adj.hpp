@@ -35,10 +35,15 @@ DEM_HD void acc3s(d3 &a, double s, d3 b) { a.x += s * b.x; a.y += s * b.y; a.z +
 // inertia for a wall), and the derivative uses max(|w|, roll_c mu_r fcn) = max(|w|, k w_c):
 // exact away from chatter, at most critically damped (k = 1) within it.  roll_c = 0 is the
 // exact reverse mode.
+//
+// roll_fwd: the forward itself floored |w| there (law_core's roll_c): the moment is then
+// -w_rel / roll_c below the floor -- linear in w_rel, independent of mu_r and fcn -- and
+// this is the exact reverse of that forward (the projector and the material terms only
+// above the floor).
 DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel, d3 hin,
                          double dt, const LawParams &P, d3 fib, d3 ftb_ext, d3 mrb, d3 houtb,
                          d3 &nb, double &dnb, d3 &vrelb, d3 &wrelb, d3 &hinb, LawParamsBar &Pb,
-                         double roll_c = 0.0)
+                         double roll_c = 0.0, bool roll_fwd = false)
 {
     // ---- forward recompute ----
     double vns = dot3(vrel, n);
@@ -107,14 +112,19 @@ DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel
     double fcnb = 0.0;
     d3 vtb = d3{0.0, 0.0, 0.0}, vnb = d3{0.0, 0.0, 0.0};
     if (roll) {   // mr = -ur fcn w_hat, w_hat = wrel / |wrel|
-        d3 wh = scale3(wrel, 1.0 / wrm);
-        double mrw = dot3(mrb, wh);
-        Pb.ur += -mrw * fcn;
-        fcnb += -mrw * P.ur;
         double wfl = roll_c * P.ur * fcn;
-        double mc = P.ur * fcn / (wrm > wfl ? wrm : wfl);   // = mcoef unless regularised
-        acc3s(wrelb, -mc, mrb);   // d w_hat: (I - w_hat w_hat^T) / |wrel|
-        acc3s(wrelb, mc * mrw, wh);
+        if (roll_fwd && !(wrm > wfl)) {
+            // floored forward: mr = -(ur fcn / wfl) wrel
+            acc3s(wrelb, -(P.ur * fcn / wfl), mrb);
+        } else {
+            d3 wh = scale3(wrel, 1.0 / wrm);
+            double mrw = dot3(mrb, wh);
+            Pb.ur += -mrw * fcn;
+            fcnb += -mrw * P.ur;
+            double mc = P.ur * fcn / (wrm > wfl ? wrm : wfl);   // = mcoef unless regularised
+            acc3s(wrelb, -mc, mrb);   // d w_hat: (I - w_hat w_hat^T) / |wrel|
+            acc3s(wrelb, mc * mrw, wh);
+        }
     }
     if (tang) {
         double dtmaxb = 0.0, omb = 0.0, htm2b = 0.0;
@@ -213,7 +223,8 @@ DEM_HD void law_core_adj(d3 n, double dn, double rs, double ms, d3 vrel, d3 wrel
 DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, double rb,
                          double ma, double mb, d3 hin, double dt, const LawParams &P, d3 fab,
                          d3 tab, d3 tbb, d3 houtb, d3 &xab, d3 &xbb, d3 &vab, d3 &vbb, d3 &wab,
-                         d3 &wbb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0)
+                         d3 &wbb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0,
+                         bool roll_fwd = false)
 {
     d3 dx = sub3(xb, xa);
     double dist = sqrt(dot3(dx, dx));
@@ -229,7 +240,8 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
     d3 wrel = sub3(wa, wb);
     d3 fa, ft, mr, hout;
     double fout;
-    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout);
+    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout,
+             roll_fwd ? roll_c : 0.0);
     // torques: ta = (ra n) x ft + mr, tb = (rb n) x ft - mr
     d3 ua = scale3(n, ra), ub = scale3(n, rb);
     d3 nb = add3(scale3(cross3(ft, tab), ra), scale3(cross3(ft, tbb), rb));
@@ -238,7 +250,7 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
     double dnb = 0.0;
     d3 vrelb = d3{0.0, 0.0, 0.0}, wrelb = d3{0.0, 0.0, 0.0};
     law_core_adj(n, dn, rs, ms, vrel, wrel, hin, dt, P, fab, ftb, mrb, houtb, nb, dnb, vrelb,
-                 wrelb, hinb, Pb, roll_c);
+                 wrelb, hinb, Pb, roll_c, roll_fwd);
     acc3(wab, wrelb);   // wrel = wa - wb
     wbb.x -= wrelb.x; wbb.y -= wrelb.y; wbb.z -= wrelb.z;
     acc3(vbb, vrelb);   // vrel = (vb - va) - c
@@ -259,7 +271,8 @@ DEM_HD void law_pair_adj(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra, do
 // torque), houtb (the history slot's new value).  Accumulates into xb, vb, wb, hinb, Pb.
 DEM_HD void law_walls_adj(const WallGeom &g, d3 x, d3 v, d3 w, double r, double m, d3 hin,
                           double dt, const LawParams &P, d3 fwb, d3 twb, d3 houtb, d3 &xb,
-                          d3 &vb, d3 &wb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0)
+                          d3 &vb, d3 &wb, d3 &hinb, LawParamsBar &Pb, double roll_c = 0.0,
+                          bool roll_fwd = false)
 {
     int nw = wall_count(g);
     int prim = -1;
@@ -282,7 +295,7 @@ DEM_HD void law_walls_adj(const WallGeom &g, d3 x, d3 v, d3 w, double r, double 
         d3 vrel = d3{(0.0 - v.x) - c.x, (0.0 - v.y) - c.y, (0.0 - v.z) - c.z};
         d3 f, ft, mr, ht;
         double fo;
-        law_core(n, dn, r, m, vrel, w, hk, dt, P, f, ft, mr, ht, fo);
+        law_core(n, dn, r, m, vrel, w, hk, dt, P, f, ft, mr, ht, fo, roll_fwd ? roll_c : 0.0);
         // torque (r n) x ft + mr
         d3 un = scale3(n, r);
         d3 nb = scale3(cross3(ft, twb), r);
@@ -291,7 +304,7 @@ DEM_HD void law_walls_adj(const WallGeom &g, d3 x, d3 v, d3 w, double r, double 
         d3 vrelb = d3{0.0, 0.0, 0.0}, wrelb = d3{0.0, 0.0, 0.0}, hkb = d3{0.0, 0.0, 0.0};
         law_core_adj(n, dn, r, m, vrel, w, hk, dt, P, fwb, ftb, twb,
                      k == prim ? houtb : d3{0.0, 0.0, 0.0}, nb, dnb, vrelb, wrelb, hkb, Pb,
-                     roll_c);
+                     roll_c, roll_fwd);
         if (k == prim) acc3(hinb, hkb);
         acc3(wb, wrelb);   // wrel = w
         vb.x -= vrelb.x; vb.y -= vrelb.y; vb.z -= vrelb.z;   // vrel = -v - c

@@ -528,11 +528,11 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         if (c->batched)
             hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist, gate, c->xcd_chunk);
+                               dinst, one.pp, one.dt, one.roll_k, c->tape_hist, gate, c->xcd_chunk);
         else
             hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
-                               dinst, one.pp, one.dt, c->tape_hist, gate, c->xcd_chunk);
+                               dinst, one.pp, one.dt, one.roll_k, c->tape_hist, gate, c->xcd_chunk);
     }
     if (!c->fused) {   // the fused step evaluates the wall contacts itself
         if (c->batched)
@@ -1049,6 +1049,14 @@ int dem_set_option(dem_ctx *c, int option, double value)
         if (!(value >= 0.0)) return DEM_ERR_INVALID;
         c->adj_roll_k = value;
         return DEM_OK;
+    case DEM_OPT_ROLL_FLOOR: {
+        if (!(value >= 0.0 && value <= 1e6)) return DEM_ERR_INVALID;
+        DEM_HIP(hipStreamSynchronize(c->stream));
+        for (InstDev &I : c->hinst) I.roll_k = value;
+        DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
+        drop_graph(c);
+        return DEM_OK;
+    }
     case DEM_OPT_PAIR_GRID: {
         if (!(value >= 1.0 && value <= 256.0)) return DEM_ERR_INVALID;
         c->pair_grid_per_cu = (unsigned)value;
@@ -2124,13 +2132,13 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
+                                   one.roll_k, (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
             else
                 hipLaunchKernelGGL(k_pair_force<false>, dim3(g), dim3(c->pair_block), 0, c->stream,
                                    (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
                                    (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
                                    (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
+                                   one.roll_k, (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
             DEM_HIP(hipGetLastError());
             DEM_HIP(hipStreamSynchronize(c->stream));
         }
@@ -2194,16 +2202,22 @@ WallGeom wall_geom(const dem_walls *w)
 }
 }  // namespace
 
-int dem_law_pair_eval(const double *in, const double *mat, double *out)
+int dem_law_pair_eval_floor(const double *in, const double *mat, double roll_c, double *out)
 {
-    if (!in || !mat || !out) return -1;
+    if (!in || !mat || !out || !(roll_c >= 0.0)) return -1;
     d3 fa, ta, tb, ho;
     double fo;
     bool c = law_pair(v3(in), v3(in + 3), v3(in + 6), v3(in + 9), v3(in + 12), v3(in + 15), in[18],
-                      in[19], in[20], in[21], v3(in + 22), in[25], law6(mat), fa, ta, tb, ho, fo);
+                      in[19], in[20], in[21], v3(in + 22), in[25], law6(mat), fa, ta, tb, ho, fo,
+                      roll_c);
     put3(out, fa); put3(out + 3, ta); put3(out + 6, tb); put3(out + 9, ho);
     out[12] = fo;
     return c ? 1 : 0;
+}
+
+int dem_law_pair_eval(const double *in, const double *mat, double *out)
+{
+    return dem_law_pair_eval_floor(in, mat, 0.0, out);
 }
 
 int dem_contact_filter(const double *in, int count, int *keep)
@@ -2216,47 +2230,65 @@ int dem_contact_filter(const double *in, int count, int *keep)
     return DEM_OK;
 }
 
-int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
-                     double *matbar)
+int dem_law_pair_vjp_floor(const double *in, const double *mat, double roll_c,
+                           const double *outbar, double *inbar, double *matbar)
 {
-    if (!in || !mat || !outbar || !inbar || !matbar) return -1;
+    if (!in || !mat || !outbar || !inbar || !matbar || !(roll_c >= 0.0)) return -1;
     d3 g[7];
     for (int k = 0; k < 7; k++) g[k] = d3{0.0, 0.0, 0.0};
     LawParamsBar pb = {0.0, 0.0, 0.0, 0.0, 0.0};
     law_pair_adj(v3(in), v3(in + 3), v3(in + 6), v3(in + 9), v3(in + 12), v3(in + 15), in[18],
                  in[19], in[20], in[21], v3(in + 22), in[25], law6(mat), v3(outbar),
                  v3(outbar + 3), v3(outbar + 6), v3(outbar + 9), g[0], g[1], g[2], g[3], g[4],
-                 g[5], g[6], pb);
+                 g[5], g[6], pb, roll_c, roll_c > 0.0);
     for (int k = 0; k < 7; k++) put3(inbar + 3 * k, g[k]);
     matbar[0] = pb.estar; matbar[1] = pb.mu; matbar[2] = pb.gn; matbar[3] = pb.gt; matbar[4] = pb.ur;
     return DEM_OK;
 }
 
-int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out)
+int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, double *inbar,
+                     double *matbar)
 {
-    if (!walls || !in || !mat || !out) return -1;
+    return dem_law_pair_vjp_floor(in, mat, 0.0, outbar, inbar, matbar);
+}
+
+int dem_law_walls_eval_floor(const dem_walls *walls, const double *in, const double *mat,
+                             double roll_c, double *out)
+{
+    if (!walls || !in || !mat || !out || !(roll_c >= 0.0)) return -1;
     d3 fw, tw, hw;
     double fo;
     int hits = law_walls(wall_geom(walls), v3(in), v3(in + 3), v3(in + 6), in[9], in[10],
-                         v3(in + 11), in[14], law6(mat), fw, tw, hw, fo);
+                         v3(in + 11), in[14], law6(mat), fw, tw, hw, fo, roll_c);
     put3(out, fw); put3(out + 3, tw); put3(out + 6, hw);
     out[9] = fo;
     return hits;
 }
 
-int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
-                      const double *outbar, double *inbar, double *matbar)
+int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out)
 {
-    if (!walls || !in || !mat || !outbar || !inbar || !matbar) return -1;
+    return dem_law_walls_eval_floor(walls, in, mat, 0.0, out);
+}
+
+int dem_law_walls_vjp_floor(const dem_walls *walls, const double *in, const double *mat,
+                            double roll_c, const double *outbar, double *inbar, double *matbar)
+{
+    if (!walls || !in || !mat || !outbar || !inbar || !matbar || !(roll_c >= 0.0)) return -1;
     d3 g[4];
     for (int k = 0; k < 4; k++) g[k] = d3{0.0, 0.0, 0.0};
     LawParamsBar pb = {0.0, 0.0, 0.0, 0.0, 0.0};
     law_walls_adj(wall_geom(walls), v3(in), v3(in + 3), v3(in + 6), in[9], in[10], v3(in + 11),
                   in[14], law6(mat), v3(outbar), v3(outbar + 3), v3(outbar + 6), g[0], g[1], g[2],
-                  g[3], pb);
+                  g[3], pb, roll_c, roll_c > 0.0);
     for (int k = 0; k < 4; k++) put3(inbar + 3 * k, g[k]);
     matbar[0] = pb.estar; matbar[1] = pb.mu; matbar[2] = pb.gn; matbar[3] = pb.gt; matbar[4] = pb.ur;
     return DEM_OK;
+}
+
+int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
+                      const double *outbar, double *inbar, double *matbar)
+{
+    return dem_law_walls_vjp_floor(walls, in, mat, 0.0, outbar, inbar, matbar);
 }
 
 // =========================================================================================
@@ -2973,11 +3005,11 @@ int reverse_segment(dem_ctx *c, AdjData *A, int set, int ks, const std::vector<L
         if (c->batched)
             hipLaunchKernelGGL(k_adj_pair<true>, dim3(pg), dim3(kBlock), 0, st, V.ab, V.np, tr, th,
                                (const double *)A->ftb, A->hb[hc], A->pa, (const InstDev *)c->dinst,
-                               one.pp, one.dt, c->adj_roll_k);
+                               one.pp, one.dt, one.roll_k, c->adj_roll_k);
         else
             hipLaunchKernelGGL(k_adj_pair<false>, dim3(pg), dim3(kBlock), 0, st, V.ab, V.np, tr, th,
                                (const double *)A->ftb, A->hb[hc], A->pa, (const InstDev *)c->dinst,
-                               one.pp, one.dt, c->adj_roll_k);
+                               one.pp, one.dt, one.roll_k, c->adj_roll_k);
         hipLaunchKernelGGL(k_adj_gather, dim3(gn), dim3(kBlock), 0, st, A->lam[A->cur],
                            (const PAdj *)A->pa, V.off, V.toff, V.jt, V.np, tr, A->thacc, n);
         if (step_reb[j]) {

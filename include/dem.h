@@ -224,10 +224,16 @@ enum {
     DEM_OPT_XCD_CHUNK = 13,        /* step kernels: the 8 XCDs take chunks of this many
                                       blocks in turn (default 8; 0: one contiguous eighth of
                                       the particles or pairs each).  Same results bit for bit */
-    DEM_OPT_PIPELINE_BATCH = 14    /* queued steps issued together behind one HIP event
+    DEM_OPT_PIPELINE_BATCH = 14,   /* queued steps issued together behind one HIP event
                                       (1..16, at most DEM_OPT_PIPELINE; default 8): an event
                                       between two kernels adds ~4 us to the gap between them.
                                       Same results bit for bit */
+    DEM_OPT_ROLL_FLOOR = 15        /* differentiable form: every instance's rolling moment
+                                      floors |w_rel| at k mu_r fcn dt^2 (1/I_a + 1/I_b) (one
+                                      inertia for a wall), i.e. it is viscous, -w_rel / (k dt^2
+                                      (1/I_a + 1/I_b)), where the reference's moment would flip
+                                      sign every step; the adjoint then reverses that forward
+                                      exactly.  k >= 0; 0 (default) = the reference's moment */
 };
 
 /* Result of dem_autotune. */
@@ -369,6 +375,16 @@ int dem_law_pair_vjp(const double *in, const double *mat, const double *outbar, 
 int dem_law_walls_eval(const dem_walls *walls, const double *in, const double *mat, double *out);
 int dem_law_walls_vjp(const dem_walls *walls, const double *in, const double *mat,
                       const double *outbar, double *inbar, double *matbar);
+/* The same four with the rolling floor of DEM_OPT_ROLL_FLOOR: roll_c = k dt^2 (1/I_a + 1/I_b)
+ * (pair) or k dt^2 / I (walls); the vjp is the exact reverse of that forward.  roll_c = 0
+ * gives the functions above. */
+int dem_law_pair_eval_floor(const double *in, const double *mat, double roll_c, double *out);
+int dem_law_pair_vjp_floor(const double *in, const double *mat, double roll_c,
+                           const double *outbar, double *inbar, double *matbar);
+int dem_law_walls_eval_floor(const dem_walls *walls, const double *in, const double *mat,
+                             double roll_c, double *out);
+int dem_law_walls_vjp_floor(const dem_walls *walls, const double *in, const double *mat,
+                            double roll_c, const double *outbar, double *inbar, double *matbar);
 
 /* ----------------------------------------------------------------------------------- */
 /* 3. Adjoint (differentiable) form -- no counterpart in the reference                  */

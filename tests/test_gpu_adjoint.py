@@ -286,3 +286,50 @@ def test_calibration_reduces_misfit(dem, gpu_ok):
     losses = [h["loss"] for h in res["history"]]
     print("calibration losses", losses, flush=True)
     assert losses[-1] < 0.6 * losses[0], losses
+
+
+def test_config4_gradients_at_full_size(dem, gpu_ok):
+    """Configuration 4 at its own size: 20K lognormal spheres settled in a 30d-wide box, 5,000
+    steps, checkpoints every 100, with the differentiable form's rolling floor (k = 3: the
+    forward's rolling moment is viscous where the reference's flips sign every step, and the
+    adjoint is that forward's exact reverse mode).  Adjoint gradients of the inner-region
+    packing fraction and coordination number in (mu, log E, e, mu_r) against central
+    differences of the same forward, relative step 1e-4 (e through the dashpot coefficient,
+    gamma(1 +- 1e-4), times d gamma / d e).  The bars are what this window supports: the
+    response is piecewise smooth (contact onsets, Coulomb and spring caps), and central
+    differences of phi at steps 1e-3 ... 1e-5 scatter by up to 4e-3 of the value for mu and e;
+    log E and mu_r agree to about 1e-3."""
+    A = dem.adjoint
+    w = dem.settle_bed(dem.config4(), tap=0.0)
+    obs = dem.make_obs(zhi=w.meta["bed_top"] - 3.0, margin=2.0, zlo=2.0, eps=0.5, eps_c=1e-3)
+    prob = A.PackingProblem(w, 5000, 100, obs, roll_floor=3.0)
+    th0 = [0.3, math.log(1e7), 0.6, 0.002]
+    rel = 1e-4
+    r = A.fd_check(prob, [th0], rel=rel, which=(0, 1, 3))
+    base = prob.mmap.material(th0)
+    gam = base.p.dampnp
+    vals = []
+    for sgn in (1, -1):
+        m = copy.deepcopy(base)
+        m.p.dampnp = m.p.damptp = m.w.dampnw = m.w.damptw = gam * (1 + sgn * rel)
+        prob.ctx.set_material(0, m)
+        prob.ctx.load(w)
+        prob.ctx.step(5000)
+        vals.append(prob.ctx.observe(obs))
+    dg = prob.mmap.dgamma_de(0.6)
+    r["fd_phi"][2] = (vals[0][0][0] - vals[1][0][0]) / (2 * rel * gam) * dg
+    r["fd_cn"][2] = (vals[0][1][0] - vals[1][1][0]) / (2 * rel * gam) * dg
+    ga, gf = np.asarray(r["adj_phi"]), np.asarray(r["fd_phi"])
+    ca, cf = np.asarray(r["adj_cn"]), np.asarray(r["fd_cn"])
+    relerr = np.abs(ga - gf) / np.maximum(np.abs(ga), np.abs(gf))
+    nat = np.array([th0[0], 1.0, th0[2], th0[3]])
+    norm_phi = np.abs(ga - gf) * nat / np.max(np.abs(gf) * nat)
+    norm_cn = np.abs(ca - cf) * nat / np.max(np.abs(cf) * nat)
+    print(f"[config4] phi {r['phi']:.5f} adj {ga} fd {gf} rel {relerr} norm phi {norm_phi} "
+          f"cn {norm_cn}", flush=True)
+    assert np.all(np.isfinite(ga)) and np.all(np.isfinite(ca))
+    assert relerr[1] <= 1.5e-3 and relerr[3] <= 1.5e-3, relerr   # log E, mu_r
+    assert relerr[0] <= 6e-3, relerr                              # mu
+    assert relerr[2] <= 5e-2, relerr                              # e
+    assert np.all(norm_phi <= 2e-3), norm_phi
+    assert np.all(norm_cn <= 5e-3), norm_cn

@@ -150,3 +150,90 @@ def test_wall_vjp_matches_finite_differences(dem, geom):
         g = fd_vjp(f, x, idx, ybar)
         scale = np.abs(g).max()
         assert np.max(np.abs(inbar - g)) <= 2e-5 * scale, (geom, trial, inbar - g)
+
+
+@pytest.mark.parametrize("regime", ["floored", "above"])
+def test_pair_vjp_with_rolling_floor(dem, regime):
+    """The rolling floor of the differentiable form (DEM_OPT_ROLL_FLOOR): below the floor the
+    moment is -w_rel / roll_c (viscous, independent of mu_r and fcn), above it the reference's
+    moment; the vjp is the exact reverse of that forward (central differences), and roll_c = 0
+    is the plain law bit for bit."""
+    L = dem.lib()
+    rng = np.random.default_rng({"floored": 11, "above": 12}[regime])
+    for trial in range(6):
+        x, mat = pair_case(rng, "plain")
+        dn = x[18] + x[19] - np.linalg.norm(x[3:6] - x[0:3])
+        rs = x[18] * x[19] / (x[18] + x[19])
+        fcn = mat[0] * np.sqrt(rs) * dn ** 1.5
+        wrel = x[12:15] - x[15:18]
+        wfl_target = np.linalg.norm(wrel) * (3.0 if regime == "floored" else 0.3)
+        roll_c = wfl_target / (mat[5] * fcn)
+        out0 = np.zeros(13)
+        out1 = np.zeros(13)
+        assert L.dem_law_pair_eval(ptr(x), ptr(mat), ptr(out0)) == 1
+        assert L.dem_law_pair_eval_floor(ptr(x), ptr(mat), 0.0, ptr(out1)) == 1
+        assert np.array_equal(out0, out1)
+        outf = np.zeros(13)
+        assert L.dem_law_pair_eval_floor(ptr(x), ptr(mat), roll_c, ptr(outf)) == 1
+        if regime == "floored":   # the moment is -w_rel / roll_c
+            ta_minus_tb = outf[3:6] - outf[6:9]
+            # ta - tb = (ra - rb) n x ft + 2 mr
+            n = (x[3:6] - x[0:3]) / (x[18] + x[19] - dn)
+            ft = outf[0:3] - np.dot(outf[0:3], n) * n
+            mr2 = 0.5 * (ta_minus_tb - np.cross((x[18] - x[19]) * n, ft))
+            assert np.allclose(mr2, -wrel / roll_c, rtol=1e-9, atol=1e-14)
+        else:
+            assert np.array_equal(outf, out0)
+        ybar = rng.normal(size=12) * np.r_[np.full(9, 1.0 / (np.abs(outf[:9]).max() + 1e-30)),
+                                           np.full(3, 1e3)]
+        f = lambda z: (lambda o: (L.dem_law_pair_eval_floor(ptr(z), ptr(mat), roll_c, ptr(o)), o)[1])(
+            np.zeros(13))[:12]
+        inbar = np.zeros(21)
+        matbar = np.zeros(5)
+        assert L.dem_law_pair_vjp_floor(ptr(x), ptr(mat), roll_c, ptr(ybar), ptr(inbar),
+                                        ptr(matbar)) == 0
+        idx = list(range(18)) + [22, 23, 24]
+        g = fd_vjp(f, x, idx, ybar)
+        scale = np.abs(g).max()
+        assert np.max(np.abs(inbar - g)) <= 2e-5 * scale, (regime, trial, inbar - g)
+        fm = lambda m: (lambda o: (L.dem_law_pair_eval_floor(ptr(x), ptr(m), roll_c, ptr(o)), o)[1])(
+            np.zeros(13))[:12]
+        for k_out, k_mat in ((0, 0), (1, 1), (2, 3), (3, 4), (4, 5)):
+            gm = mat_fd(fm, mat, k_mat, ybar)
+            assert matbar[k_out] == pytest.approx(gm, rel=2e-5, abs=2e-5 * (abs(gm) + scale * 1e-6))
+
+
+def test_wall_vjp_with_rolling_floor(dem):
+    """The wall law under the rolling floor (one inertia): vjp against central differences in
+    the floored regime."""
+    L = dem.lib()
+    rng = np.random.default_rng(13)
+    walls = dem.make_walls(dem.WALLS_BOX, (6.0, 6.0, 20.0))
+    for trial in range(6):
+        r = rng.uniform(0.4, 0.6)
+        xw = np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), r - rng.uniform(2e-4, 2e-3)])
+        w = rng.normal(scale=4e-8, size=3)
+        x = np.concatenate([xw, rng.normal(scale=2e-5, size=3), w, [r, 8 * r ** 3],
+                            rng.normal(scale=2e-5, size=3), [2e-4]])
+        mat = np.array([5.6e4, 0.4, 0.29, 0.35, 0.3, 0.01])
+        dn = r - xw[2]
+        fcn = mat[0] * np.sqrt(r) * dn ** 1.5
+        roll_c = 3.0 * np.linalg.norm(w) / (mat[5] * fcn)
+        out = np.zeros(10)
+        assert L.dem_law_walls_eval_floor(ctypes.byref(walls), ptr(x), ptr(mat), roll_c,
+                                          ptr(out)) == 1
+        ybar = rng.normal(size=9) * np.r_[np.full(6, 1.0 / (np.abs(out[:6]).max() + 1e-30)),
+                                          np.full(3, 1e3)]
+
+        def f(z):
+            o = np.zeros(10)
+            L.dem_law_walls_eval_floor(ctypes.byref(walls), ptr(z), ptr(mat), roll_c, ptr(o))
+            return o[:9]
+        inbar = np.zeros(12)
+        matbar = np.zeros(5)
+        assert L.dem_law_walls_vjp_floor(ctypes.byref(walls), ptr(x), ptr(mat), roll_c, ptr(ybar),
+                                         ptr(inbar), ptr(matbar)) == 0
+        idx = list(range(9)) + [11, 12, 13]
+        g = fd_vjp(f, x, idx, ybar)
+        scale = np.abs(g).max()
+        assert np.max(np.abs(inbar - g)) <= 2e-5 * scale, (trial, inbar - g)
