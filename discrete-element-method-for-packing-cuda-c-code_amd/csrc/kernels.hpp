@@ -349,14 +349,16 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
 // Pair list of slot a: its partners with a larger particle index, in ascending slot order,
 // at [off[a], off[a+1]) -- each pair is stored once, first member = smaller particle index,
 // exactly the reference's Lp < Ln orientation (dempacking.cu:640-644), while segments follow
-// the cell-sorted slot order.  Partners with a smaller index go to the transposed list, to be
-// resolved to pair indices by k_transpose.  The tangential histories are filled by k_remap.
+// the cell-sorted slot order.  Partners with a smaller index go to the transposed list
+// [toff[a], toff[a+1]): jf = the partner (the pair's first member), jt = a for now; k_transpose
+// turns jt into the pair's index.  The tangential histories are filled by k_remap.
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const double4 *cpos, const unsigned *cid, const uint2 *sorted, unsigned n,
     const InstDev *inst, InstDev one,
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
-    unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, DevCtl *ctl, const unsigned *active)
+    unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, unsigned *jf, DevCtl *ctl,
+    const unsigned *active)
 {
     if (!is_active(active)) return;
     unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -373,7 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
             bo[pg] = b;
             pg++;
         } else {
-            if (pl < cap) jt[pl] = b;
+            if (pl < cap) {
+                jf[pl] = b;
+                jt[pl] = a;
+            }
             pl++;
         }
     });
@@ -420,25 +425,39 @@ __global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl 
     }
 }
 
-// Replace each transposed entry (partner slot b, smaller particle index) by the index of
-// pair (b, a) in b's segment, and keep b in jf[k] (the fused step reads the first member's
-// slot from there instead of from the pair list at a scattered index).
-__global__ __launch_bounds__(kBlock) void k_transpose(const uint2 *ab, const unsigned *off,
+// Second member b of pair index p, for the search below: bo[p] (the list's second slots).
+__device__ __forceinline__ unsigned find_slot(const unsigned *bo, unsigned lo, unsigned hi,
+                                              unsigned target)
+{
+    while (lo < hi) {
+        unsigned mid = (lo + hi) >> 1;
+        unsigned v = bo[mid];
+        if (v == target) return mid;
+        if (v < target) lo = mid + 1;
+        else hi = mid;
+    }
+    return kEmpty;
+}
+
+// Resolve each transposed entry k (first member jf[k], second member jt[k], as the fill left
+// them) to the index of that pair in the first member's segment, whose second slots ascend --
+// one lane per entry over contiguous, XCD-aware ranges of the n particles' toff[n] entries
+// (clamped to the capacity), so a fixed grid serves a captured rebuild.
+__global__ __launch_bounds__(kBlock) void k_transpose(const unsigned *bo, const unsigned *off,
                                                       const unsigned *toff, unsigned n,
-                                                      unsigned cap, unsigned *jt, unsigned *jf,
-                                                      const unsigned *active)
+                                                      unsigned cap, unsigned *jt,
+                                                      const unsigned *jf, const unsigned *active)
 {
     if (!is_active(active)) return;
-    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (a >= n) return;
-    unsigned e = toff[a + 1];
-    if (e > cap) e = cap;
-    for (unsigned k = toff[a]; k < e; k++) {
-        unsigned b = jt[k];
+    const unsigned nt = min(toff[n], cap);
+    const unsigned per = (nt + gridDim.x - 1) / gridDim.x;
+    const unsigned k_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned k_end = min(k_begin + per, nt);
+    for (unsigned k = k_begin + threadIdx.x; k < k_end; k += blockDim.x) {
+        const unsigned b = jf[k], a = jt[k];
         unsigned lo = off[b], hi = off[b + 1];
         if (hi > cap) hi = cap;
-        jt[k] = lo < hi ? find_second(ab, lo, hi, a) : kEmpty;
-        jf[k] = b;
+        jt[k] = lo < hi ? find_slot(bo, lo, hi, a) : kEmpty;
     }
 }
 

@@ -439,12 +439,14 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->ctl, active);
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
+                           active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->ctl, active);
+                           c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
+                           active);
     stage_event(ev, kEvFill, st);
     // histories by pair identity, one lane per pair over the device pair count
     const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
@@ -453,9 +455,9 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
                        (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
                        (const Hist *)c->hist[oh], c->n_list, capo, c->tape_src, active);
     stage_event(ev, kEvRemap, st);
-    hipLaunchKernelGGL(k_transpose, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
+    hipLaunchKernelGGL(k_transpose, dim3(rg), dim3(kBlock), 0, st, (const unsigned *)c->bo,
                        (const unsigned *)c->off[nl], (const unsigned *)c->toff, n, capn, c->jt,
-                       c->jf, active);
+                       (const unsigned *)c->jf, active);
     stage_event(ev, kEvTranspose, st);
     DEM_HIP(hipGetLastError());
     c->lc = nl;
