@@ -346,12 +346,18 @@ __device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, un
     return kEmpty;
 }
 
+// Entries of one block's own-pair and transposed ranges the fill stages in LDS and writes
+// out together (consecutive lanes on consecutive entries) instead of as per-lane appends.
+constexpr unsigned kFillStage = 2048;
+
 // Pair list of slot a: its partners with a larger particle index, in ascending slot order,
 // at [off[a], off[a+1]) -- each pair is stored once, first member = smaller particle index,
 // exactly the reference's Lp < Ln orientation (dempacking.cu:640-644), while segments follow
 // the cell-sorted slot order.  Partners with a smaller index go to the transposed list
 // [toff[a], toff[a+1]): jf = the partner (the pair's first member), jt = a for now; k_transpose
-// turns jt into the pair's index.  The tangential histories are filled by k_remap.
+// turns jt into the pair's index.  The tangential histories are filled by k_remap.  Both
+// ranges of a block are contiguous; entries are staged in LDS and written out by consecutive
+// lanes (a range longer than kFillStage is written lane by lane).
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const double4 *cpos, const unsigned *cid, const uint2 *sorted, unsigned n,
@@ -361,27 +367,56 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const unsigned *active)
 {
     if (!is_active(active)) return;
-    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (a >= n) return;
-    const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
-    unsigned pg = off[a];
-    unsigned pl = toff[a];
-    unsigned ida = cid[a];
+    __shared__ uint2 s_ab[kFillStage];
+    __shared__ unsigned s_bo[kFillStage], s_jf[kFillStage], s_jt[kFillStage];
+    const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+    const unsigned a = s0 + threadIdx.x;
+    const unsigned se = min(s0 + kBlock, n);
+    // the block's contiguous own-pair and transposed ranges
+    const unsigned G0 = off[s0], G1 = off[se], T0 = toff[s0], T1 = toff[se];
+    const bool stage_g = G1 - G0 <= kFillStage, stage_t = T1 - T0 <= kFillStage;
     bool over = false;
-    for_candidates(cpos, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
-        if (cid[b] > ida) {
-            if (pg >= cap) { over = true; return; }
-            ab[pg] = make_uint2(a, b);
-            bo[pg] = b;
-            pg++;
-        } else {
-            if (pl < cap) {
-                jf[pl] = b;
-                jt[pl] = a;
+    if (a < n) {
+        const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
+        unsigned pg = off[a];
+        unsigned pl = toff[a];
+        unsigned ida = cid[a];
+        for_candidates(cpos, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
+            if (cid[b] > ida) {
+                if (pg >= cap) { over = true; return; }
+                if (stage_g) {
+                    s_ab[pg - G0] = make_uint2(a, b);
+                    s_bo[pg - G0] = b;
+                } else {
+                    ab[pg] = make_uint2(a, b);
+                    bo[pg] = b;
+                }
+                pg++;
+            } else {
+                if (pl < cap) {
+                    if (stage_t) {
+                        s_jf[pl - T0] = b;
+                        s_jt[pl - T0] = a;
+                    } else {
+                        jf[pl] = b;
+                        jt[pl] = a;
+                    }
+                }
+                pl++;
             }
-            pl++;
+        });
+    }
+    __syncthreads();
+    if (stage_g)
+        for (unsigned k = threadIdx.x; G0 + k < min(G1, cap); k += kBlock) {
+            ab[G0 + k] = s_ab[k];
+            bo[G0 + k] = s_bo[k];
         }
-    });
+    if (stage_t)
+        for (unsigned k = threadIdx.x; T0 + k < min(T1, cap); k += kBlock) {
+            jf[T0 + k] = s_jf[k];
+            jt[T0 + k] = s_jt[k];
+        }
     if (over && ctl) atomicOr(&ctl->overflow, 1u);
 }
 
