@@ -99,7 +99,7 @@ def rebuild_bytes(N, P, P_old, C, passes):
 
 
 TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f) for f in (
-    "hbm_traffic_step_500k_lognormal_cylinder.md", "dense_window_500k_lognormal_cylinder.md")]
+    "hbm_traffic_step_500k_lognormal_cylinder.md", "bed_step_500k_lognormal_cylinder.md")]
 
 
 def measured_traffic(kernel: str, n: int, p_avg: float):
@@ -611,12 +611,14 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(dem, ws, w0, budget_s):
+def cpu_baseline(dem, ws, w0, budget_s, w2=None):
     """The oracle (this build's C restatement of the same PCCP algorithm, OpenMP, built with
     -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off) on the GPU box's host cores: the poured
     500K bed -- the same state the GPU's timed steps start from -- on all threads and on one
-    thread (a bounded sample of steps each, after the first, list-building step), and the
-    2,000-sphere configuration's full 20,000-step run on all threads and on one thread."""
+    thread (a bounded sample of steps each, after the first, list-building step), the
+    2,000-sphere configuration's full 20,000-step run on all threads and on one thread, and
+    (w2) a bounded sample of the 2M bimodal box from its initial state, as its GPU leg starts,
+    on all threads."""
     import ctypes
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
@@ -624,10 +626,10 @@ def cpu_baseline(dem, ws, w0, budget_s):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     out = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": threads}
 
-    def bed_rate(th, budget):
+    def bed_rate(th, budget, wk=ws, width=64):
         gomp.omp_set_num_threads(th)
-        sim = orc.OracleSim(ws.instances[0], ws.pos, ws.pdis, ws.angv, ws.rad, ws.rmass,
-                            ws.inert, width=64)
+        sim = orc.OracleSim(wk.instances[0], wk.pos, wk.pdis, wk.angv, wk.rad, wk.rmass,
+                            wk.inert, width=width)
         assert sim.step(1) == 0
         t0 = time.perf_counter()
         assert sim.step(1) == 0
@@ -636,7 +638,10 @@ def cpu_baseline(dem, ws, w0, budget_s):
         t0 = time.perf_counter()
         assert sim.step(k) == 0
         el = time.perf_counter() - t0
-        return ws.n * k / el, k
+        P = sim.scalars()["npairs"]
+        assert sim.scalars()["overflow"] == 0
+        del sim
+        return wk.n * k / el, k, P
 
     def c0_rate(th):
         gomp.omp_set_num_threads(th)
@@ -648,10 +653,15 @@ def cpu_baseline(dem, ws, w0, budget_s):
         el = time.perf_counter() - t0
         return w0.n * w0.steps / el
 
-    v_all, k_all = bed_rate(threads, budget_s)
+    v_all, k_all, _ = bed_rate(threads, budget_s)
     log(f"cpu baseline: poured bed {v_all:.4e} p-steps/s on {threads} threads ({k_all} steps)")
-    v_one, k_one = bed_rate(1, 0.5 * budget_s)
+    v_one, k_one, _ = bed_rate(1, 0.5 * budget_s)
     log(f"cpu baseline: poured bed {v_one:.4e} p-steps/s on 1 thread ({k_one} steps)")
+    c2 = None
+    if w2 is not None:
+        v2, k2, P2 = bed_rate(threads, 0.5 * budget_s, w2, 128)
+        c2 = {"n": w2.n, "steps": k2, "value_all_threads": v2, "npairs": P2}
+        log(f"cpu baseline: 2M bimodal box {v2:.4e} p-steps/s on {threads} threads ({k2} steps)")
     c0_all = c0_rate(threads)
     c0_one = c0_rate(1)
     log(f"cpu baseline: 2,000-sphere full run {c0_all:.4e} ({threads} threads), {c0_one:.4e} "
@@ -664,6 +674,7 @@ def cpu_baseline(dem, ws, w0, budget_s):
                 "one_thread": {"value": v_one, "steps": k_one},
                 "config0_full_run": {"n": w0.n, "steps": w0.steps, "value_all_threads": c0_all,
                                      "value_one_thread": c0_one},
+                "config2_sample": c2,
                 "build": "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"})
     return out
 
@@ -757,10 +768,11 @@ def main():
     full = None
     if not args.no_full_run:
         full = bench_run(dem, w, args.full_steps, split_opt, "full run", loose=LOOSE)
-    c2 = None
+    c2 = w2 = None
     if gen is not None:
         gen.join()
-        c2 = bench_run(dem, c2_box.pop("w"), args.c2_steps, split_opt, "2M bimodal")
+        w2 = c2_box.pop("w")
+        c2 = bench_run(dem, w2, args.c2_steps, split_opt, "2M bimodal")
     dropin = None
     if not args.no_dropin:
         dropin = bench_dropin(dem, ws, args.dropin_steps)
@@ -782,7 +794,7 @@ def main():
                                                        roll_floor=args.roll_floor)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(dem, ws, dem.config0(), args.cpu_seconds)
+        cpu = cpu_baseline(dem, ws, dem.config0(), args.cpu_seconds, w2)
     N = ws.n
     out = {
         "metric": METRIC,

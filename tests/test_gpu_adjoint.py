@@ -296,9 +296,11 @@ def test_config4_gradients_at_full_size(dem, gpu_ok):
     packing fraction and coordination number in (mu, log E, e, mu_r) against central
     differences of the same forward, relative step 1e-4 (e through the dashpot coefficient,
     gamma(1 +- 1e-4), times d gamma / d e).  The bars are what this window supports: the
-    response is piecewise smooth (contact onsets, Coulomb and spring caps), and central
-    differences of phi at steps 1e-3 ... 1e-5 scatter by up to 4e-3 of the value for mu and e;
-    log E and mu_r agree to about 1e-3."""
+    response is piecewise smooth (contact onsets, Coulomb and spring caps), so central
+    differences of phi do not settle as the step shrinks: against the adjoint, the quotients
+    at steps 1e-3, 1e-4, 3e-5 and 1e-5 differ by up to 3.9e-3 for mu and by 2.5e-3 ... 3.4e-2
+    for e (the two larger steps straddle contact onsets); log E and mu_r agree to about 1e-3
+    at every step."""
     A = dem.adjoint
     w = dem.settle_bed(dem.config4(), tap=0.0)
     obs = dem.make_obs(zhi=w.meta["bed_top"] - 3.0, margin=2.0, zlo=2.0, eps=0.5, eps_c=1e-3)
