@@ -310,26 +310,170 @@ __device__ __forceinline__ void for_candidates(const double4 *cpos, unsigned a, 
     }
 }
 
+// Block-wide minimum of lo and maximum of hi over the lanes (every lane of the block calls
+// it; red holds two words per wave).
+__device__ __forceinline__ void block_minmax(unsigned lo, unsigned hi, unsigned *red,
+                                             unsigned &blo, unsigned &bhi)
+{
+#pragma unroll
+    for (int d = 32; d; d >>= 1) {
+        lo = min(lo, (unsigned)__shfl_xor((int)lo, d, 64));
+        hi = max(hi, (unsigned)__shfl_xor((int)hi, d, 64));
+    }
+    const unsigned w = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    __syncthreads();   // the previous call's readers are done
+    if ((threadIdx.x & 63u) == 0) {
+        red[2 * w] = lo;
+        red[2 * w + 1] = hi;
+    }
+    __syncthreads();
+    blo = red[0];
+    bhi = red[1];
+    for (unsigned k = 1; k < nw; k++) {
+        blo = min(blo, red[2 * k]);
+        bhi = max(bhi, red[2 * k + 1]);
+    }
+}
+
+// Positions staged per chunk of the cooperative search, and the longest union of a stencil
+// row's ranges it stages (a block spanning a cell layer's end reaches a whole layer: its
+// lanes read their ranges from cpos directly instead).
+constexpr unsigned kTile = 512;
+constexpr unsigned kTileMax = 4 * kTile;
+
+// for_candidates for the consecutive slots of a block, every lane of which calls it (live:
+// the lane has a slot a < n).  For each stencil row (z-, y-offset ascending) the lanes' slot
+// ranges -- contiguous, and overlapping for nearby cells -- are united, the union's positions
+// are staged in LDS (tile: kTile entries) by coalesced loads, and each lane tests its own
+// range from there in ascending slot order: the same candidates in the same order as
+// for_candidates, with each position read from HBM once per block instead of once per lane.
+// f(b, s0, s1) gets each candidate b with the lane's range [s0, s1) of its row; row_done(r,
+// s0, s1) follows each row r = 0..8 on every lane (s0 = kEmpty, s1 = 0: no range).
+template <bool BATCHED, class F, class R>
+__device__ __forceinline__ void for_candidates_block(const double4 *cpos, unsigned a, bool live,
+                                                     unsigned key, const InstDev &I,
+                                                     const unsigned *cstart, const unsigned *cend,
+                                                     double4 *tile, unsigned *red, F &&f,
+                                                     R &&row_done)
+{
+    double4 ra = make_double4(0.0, 0.0, 0.0, 0.0);
+    int cz = 0, cy = 0, cx = 0, x0 = 0, x1 = 0;
+    const unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2], gxy = gx * gy;
+    if (live) {
+        ra = cpos[a];
+        const unsigned local = key - I.cell_off;
+        cz = (int)(local / gxy);
+        cy = (int)((local - (unsigned)cz * gxy) / gx);
+        cx = (int)(local - (unsigned)cz * gxy - (unsigned)cy * gx);
+        x0 = cx > 0 ? cx - 1 : 0;
+        x1 = cx + 1 < (int)gx ? cx + 1 : (int)gx - 1;
+    }
+    const d3 xa = d3{ra.x, ra.y, ra.z};
+    int r = -1;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            r++;
+            unsigned s0 = kEmpty, s1 = 0;
+            const int z = cz + dz, y = cy + dy;
+            if (live && z >= 0 && z < (int)gz && y >= 0 && y < (int)gy) {
+                const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
+                if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) {
+                    s0 = kEmpty;
+                    s1 = 0;
+                }
+            }
+            unsigned lo, hi;
+            block_minmax(s0, s1, red, lo, hi);
+            if (lo < hi && hi - lo > kTileMax) {   // uniform: each lane reads its own range
+                for (unsigned b = s0; b < s1; b++) {
+                    if (b == a) continue;
+                    const double4 rb = cpos[b];
+                    if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin)) f(b, s0, s1);
+                }
+            } else {
+                for (unsigned c0 = lo; c0 < hi; c0 += kTile) {   // (none when lo >= hi)
+                    const unsigned c1 = min(c0 + kTile, hi);
+                    __syncthreads();   // the previous chunk's readers are done
+                    for (unsigned k = threadIdx.x; c0 + k < c1; k += blockDim.x)
+                        tile[k] = cpos[c0 + k];
+                    __syncthreads();
+                    const unsigned b1 = min(s1, c1);
+                    for (unsigned b = max(s0, c0); b < b1; b++) {
+                        if (b == a) continue;
+                        const double4 rb = tile[b - c0];
+                        if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
+                            f(b, s0, s1);
+                    }
+                }
+            }
+            row_done(r, s0, s1);
+        }
+}
+
+// Candidates of one stencil row of a lane as a bit mask: bit k for slot s0 + k.  Rows longer
+// than kRowBits slots are not recorded (the fill tests them again).
+constexpr unsigned kRowBits = 64;
+constexpr unsigned kRows = 9;
+
+// The lane's stencil rows r = 0..8 (z-, y-offset ascending) with their slot ranges, as
+// for_candidates_block forms them: fn(r, s0, s1) for each non-empty row.
+template <class Fn>
+__device__ __forceinline__ void for_rows(unsigned key, const InstDev &I, const unsigned *cstart,
+                                         const unsigned *cend, Fn &&fn)
+{
+    const unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2], gxy = gx * gy;
+    const unsigned local = key - I.cell_off;
+    const int cz = (int)(local / gxy);
+    const int cy = (int)((local - (unsigned)cz * gxy) / gx);
+    const int cx = (int)(local - (unsigned)cz * gxy - (unsigned)cy * gx);
+    const int x0 = cx > 0 ? cx - 1 : 0;
+    const int x1 = cx + 1 < (int)gx ? cx + 1 : (int)gx - 1;
+    int r = -1;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            r++;
+            const int z = cz + dz, y = cy + dy;
+            if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) continue;
+            const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
+            unsigned s0, s1;
+            if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) continue;
+            fn(r, s0, s1);
+        }
+}
+
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const double4 *cpos,
                                                       const unsigned *cid, const uint2 *sorted,
                                                       unsigned n, const InstDev *inst,
                                                       InstDev one, const unsigned *cstart,
                                                       const unsigned *cend, unsigned *ng,
-                                                      unsigned *nl, const unsigned *active)
+                                                      unsigned *nl, unsigned long long *rmask,
+                                                      const unsigned *active)
 {
-    if (!is_active(active)) return;
-    unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (a >= n) return;
-    const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
+    if (!is_active(active)) return;   // uniform over the grid
+    __shared__ double4 tile[kTile];
+    __shared__ unsigned red[2 * (kBlock / 64)];
+    const unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const bool live = a < n;
+    const InstDev &I = BATCHED ? inst[rec[live ? a : 0u].inst] : one;
     unsigned cg = 0, cl = 0;
-    unsigned ida = cid[a];
-    for_candidates(cpos, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
-        if (cid[b] > ida) cg++;
-        else cl++;
-    });
-    ng[a] = cg;
-    nl[a] = cl;
+    unsigned long long m = 0ull;
+    const unsigned ida = live ? cid[a] : 0u;
+    for_candidates_block<BATCHED>(
+        cpos, a, live, live ? sorted[a].x : 0u, I, cstart, cend, tile, red,
+        [&](unsigned b, unsigned r0, unsigned r1) {
+            if (cid[b] > ida) cg++;
+            else cl++;
+            if (r1 - r0 <= kRowBits) m |= 1ull << (b - r0);
+        },
+        [&](int r, unsigned, unsigned) {
+            if (live) rmask[(size_t)r * n + a] = m;
+            m = 0ull;
+        });
+    if (live) {
+        ng[a] = cg;
+        nl[a] = cl;
+    }
 }
 
 // Binary search of `target` in the ascending second members of ab[lo, hi); kEmpty if absent.
@@ -364,9 +508,9 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const InstDev *inst, InstDev one,
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
     unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, unsigned *jf, DevCtl *ctl,
-    const unsigned *active)
+    const unsigned long long *rmask, const unsigned *active)
 {
-    if (!is_active(active)) return;
+    if (!is_active(active)) return;   // uniform over the grid
     __shared__ uint2 s_ab[kFillStage];
     __shared__ unsigned s_bo[kFillStage], s_jf[kFillStage], s_jt[kFillStage];
     const unsigned s0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
@@ -380,8 +524,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
         const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
         unsigned pg = off[a];
         unsigned pl = toff[a];
-        unsigned ida = cid[a];
-        for_candidates(cpos, a, sorted[a].x, I, cstart, cend, [&](unsigned b) {
+        const unsigned ida = cid[a];
+        auto emit = [&](unsigned b) {
             if (cid[b] > ida) {
                 if (pg >= cap) { over = true; return; }
                 if (stage_g) {
@@ -403,6 +547,26 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
                     }
                 }
                 pl++;
+            }
+        };
+        // the count's candidate masks, in the same rows and slot order; rows longer than
+        // kRowBits are tested again
+        const double4 ra = cpos[a];
+        const d3 xa = d3{ra.x, ra.y, ra.z};
+        for_rows(sorted[a].x, I, cstart, cend, [&](int r, unsigned r0, unsigned r1) {
+            if (r1 - r0 <= kRowBits) {
+                unsigned long long m = rmask[(size_t)r * n + a];
+                while (m) {
+                    const unsigned k = (unsigned)__builtin_ctzll(m);
+                    m &= m - 1ull;
+                    emit(r0 + k);
+                }
+            } else {
+                for (unsigned b = r0; b < r1; b++) {
+                    if (b == a) continue;
+                    const double4 rb = cpos[b];
+                    if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin)) emit(b);
+                }
             }
         });
     }
