@@ -1294,18 +1294,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 const double *S = lrec + (wbase + L) * kRow;   // the owner's record
                 const double *H = reinterpret_cast<const double *>(hin + pp);
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
-                // the partner's record: from the block's LDS rows when the partner is one of
-                // the block's particles (the same values as rin), else from rin
-                double2 b0, b1, b2, b3, b4, b5;
-                if (pb - s0 < nb) {
-                    const double *B = lrec + (pb - s0) * kRow;
-                    b0 = ld2(B); b1 = ld2(B + 2); b2 = ld2(B + 4);
-                    b3 = ld2(B + 6); b4 = ld2(B + 8); b5 = ld2(B + 10);
-                } else {
-                    const double *B = reinterpret_cast<const double *>(rin + pb);
-                    b0 = ld2(B); b1 = ld2(B + 2); b2 = ld2(B + 4);
-                    b3 = ld2(B + 6); b4 = ld2(B + 8); b5 = ld2(B + 10);
-                }
+                const double *B = reinterpret_cast<const double *>(rin + pb);
+                double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6),
+                        b4 = ld2(B + 8), b5 = ld2(B + 10);
                 double2 s0v = ld2(S), s1 = ld2(S + 2), s2 = ld2(S + 4), s3 = ld2(S + 6),
                         s4 = ld2(S + 8), s5 = ld2(S + 10);
                 const d3 xs = d3{s0v.x, s0v.y, s1.x}, vs = d3{s2.x, s2.y, s3.x},
@@ -1438,9 +1429,6 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             }
         }
     }
-    // the pair work of every wave has read its partners' LDS rows before any lane updates its
-    // own row
-    __syncthreads();
     if (live)
         flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, hits != 0, fw, tw,
                                         inst, one, contacti, fdbg, tape_rec, cout);
