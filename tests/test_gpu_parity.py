@@ -380,6 +380,55 @@ def test_full_size_batched_256_equals_per_instance(dem, gpu_ok):
     say(f"[c3] all {len(w.instances)} instances bit-identical")
 
 
+def sparse_workload(dem, n):
+    """n spheres more than a cutoff apart, the lowest pressed 0.001 d into the floor, the rest
+    just above it, with spins and sideways velocities: wall contacts from the first step and
+    never a candidate pair (P = 0, the reference's zero-pair case)."""
+    walls = dem.make_walls(dem.WALLS_BOX, (24.0, 24.0, 30.0))
+    k = np.arange(n, dtype=np.float64)
+    pos = np.stack([-10.0 + 4.0 * k, 9.0 - 4.5 * k, 0.499 + 0.3 * k], axis=1)
+    rng = np.random.default_rng(n)
+    pdis = rng.normal(scale=2e-5, size=(n, 3))
+    angv = rng.normal(scale=4e-5, size=(n, 3))
+    rad = np.full(n, 0.5)
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    mat = dem.reduced_material(E=1e6, nu=0.29, rho=2500.0, diam=1e-3, mu=0.4, mu_w=0.4, e=0.5,
+                               mu_r=0.01)
+    inst = dem.make_instance(n, walls, 30.0, 1.0 + dem.SKIN, mat, 2e-4)
+    return dem.Workload("sparse", [inst], pos, pdis, angv, rad, rmass, inert)
+
+
+@pytest.mark.parametrize("n", [1, 5])
+@pytest.mark.parametrize("mode", ["fused", "split", "graph"])
+def test_workloads_without_pairs(dem, orc, gpu_ok, n, mode):
+    """One sphere, and spheres farther apart than the cutoff: empty pair lists through every
+    rebuild kernel and both step paths (and a captured graph), wall contacts and free fall
+    equal to the oracle."""
+    say(f"[sparse] n={n} mode={mode}")
+    w = sparse_workload(dem, n)
+    K = 10
+    ctx, sim = make_pair(dem, orc, w, interval=K if mode == "graph" else 0)
+    ctx.set_option(dem.OPT_FUSED_STEP, 0 if mode == "split" else 1)
+    steps = 200
+    if mode == "graph":
+        ctx.step_graph(steps)
+        oracle_steps(sim, steps, K, 0)
+    else:
+        ctx.step(steps)
+        for _ in range(steps):
+            assert sim.step(1) == 0
+    st = ctx.state()
+    assert ctx.stats().npairs == 0 and sim.scalars()["npairs"] == 0
+    for key in ("pdis", "angv"):
+        assert rel_err(st[key], sim.field(key)) <= TOL, key
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+    f, t, c = ctx.forces()
+    assert np.all(c == 0)
+    assert rel_err(f, sim.field("forcei")) <= TOL and rel_err(t, sim.field("torquei")) <= TOL
+    assert abs(st["pos"][0, 2] - 0.5) < 0.01, "the lowest sphere rests on the floor"
+
+
 def test_pair_capacity_grows(dem, orc, gpu_ok):
     w = dense_workload(dem, n=1500, seed=17)
     ctx = dem.Context(w.instances, pair_capacity=64)
