@@ -544,7 +544,7 @@ def norm_err(adj, fd, theta):
     return (np.abs(np.asarray(adj) - fd) * nat / top).tolist()
 
 
-def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, iters=16,
+def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, iters=24,
                       seed=7, graph=True, roll_floor=3.0):
     """Calibration of (mu, log E, e, mu_r) to the observables of a bed run by the plain method
     at seeded hidden parameters: `ninst` seeded starts in one batched context (replicas of one
@@ -572,7 +572,7 @@ def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, 
         f"({time.perf_counter() - t0:.1f} s setup)")
     t0 = time.perf_counter()
     res = A.calibrate(prob, float(tphi[0]), float(tcn[0]), starts, iters=iters,
-                      lr=(0.03, 0.1, 0.03, 0.0005), scale=(1e-3, 0.05),
+                      lr=(0.05, 0.25, 0.05, 0.001), scale=(1e-3, 0.05), decay=0.9,
                       progress=lambda h: log(f"  calibration iter {h['iter']}: loss {h['loss']:.4f} "
                                              f"(fwd {h['forward_s']:.2f} s, bwd {h['backward_s']:.2f} s)"))
     el = time.perf_counter() - t0
@@ -583,6 +583,8 @@ def bench_calibration(dem, ninst=64, n=2000, side=12.0, nsteps=5000, every=100, 
            "loss_first": h0["loss"], "loss_last": h1["loss"],
            "median_instance_loss_first": float(np.median(h0["loss_per_instance"])),
            "median_instance_loss_best": float(np.median(best)),
+           "median_instance_loss_last": float(np.median(h1["loss_per_instance"])),
+           "instances_loss_below_1": int(np.sum(np.asarray(h1["loss_per_instance"]) < 1.0)),
            "seconds": el, "seconds_per_iter": el / iters,
            "particle_steps_per_s_fwd_bwd": big.n * nsteps * iters / el}
     log(f"calibration: loss {h0['loss']:.3f} -> {h1['loss']:.3f} in {iters} iterations "

@@ -187,11 +187,12 @@ BOUNDS = np.array([[0.05, 1.0], [math.log(1e6), math.log(1e9)], [0.1, 0.95], [0.
 
 def calibrate(problem: PackingProblem, target_phi: float, target_cn: float, theta0,
               iters: int = 10, lr=(0.05, 0.2, 0.05, 0.002), scale=(0.01, 0.5),
-              progress=None) -> dict:
+              progress=None, decay: float = 1.0) -> dict:
     """Gradient-based calibration of theta (one row per instance, each its own start) to the
     target observables: Adam on the scaled squared misfit with per-parameter step sizes `lr`,
-    projected onto BOUNDS after each step.  Every iteration is one batched forward + backward
-    through the autograd function; `progress(record)` is called after each."""
+    multiplied by `decay` after every iteration, projected onto BOUNDS after each step.  Every
+    iteration is one batched forward + backward through the autograd function;
+    `progress(record)` is called after each."""
     torch = _torch()
     step = torch.tensor(lr, dtype=torch.float64)
     # Adam's steps are about lr in the optimised variable whatever the gradient's scale, so
@@ -216,6 +217,8 @@ def calibrate(problem: PackingProblem, target_phi: float, target_cn: float, thet
         if progress:
             progress(hist[-1])
         opt.step()
+        for g in opt.param_groups:
+            g["lr"] *= decay
         with torch.no_grad():
             z.copy_(torch.max(torch.min(z * step, hi), lo) / step)
     return {"history": hist, "theta": (z * step).detach().numpy()}
