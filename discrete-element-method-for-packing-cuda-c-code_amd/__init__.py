@@ -52,6 +52,7 @@ OPT_FUSED_FAST = 12
 OPT_XCD_CHUNK = 13
 OPT_PIPELINE_BATCH = 14
 OPT_ROLL_FLOOR = 15
+OPT_FLOAT_LENGTH = 16
 
 # neiglist.cu:27-28: cua (unit diameter) and cub (candidate cutoff); the list skin is cub - cua.
 CUA = 1.0

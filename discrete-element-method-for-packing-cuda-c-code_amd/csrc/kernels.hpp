@@ -53,7 +53,10 @@ struct InstDev {
     double origin[3], cell[3];
     WallGeom walls;
     unsigned gsize[3], cell_off;
-    unsigned p_off, n, pad1, pad2;
+    unsigned p_off, n;
+    unsigned flen;         // 1: the reference's float length() in vprojec and the trigger
+                           // (context-wide: kernels take it from the launch's copy `one`)
+    unsigned pad2;
 };
 
 // Device control block, copied to a host-mapped mirror by k_publish after each step.  Its
@@ -712,7 +715,7 @@ __global__ void k_set_u32(unsigned *p, unsigned v)
 
 // One lane per candidate pair (the reference's calculateforcepairD, neiglist.cu:156-233).
 // Grid-stride over npairs read from the device, so a fixed grid serves any list size.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const unsigned *npairs,
                                                        const PRec *rec, const Hist *hist,
                                                        Hist *hist_out, POut *out,
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
         bool c = law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
                           d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x},
                           a1.y, b1.y, a3.y, b3.y, d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho,
-                          fo, roll_floor(rk, tdt, a5.y, b5.y));
+                          fo, roll_floor(rk, tdt, a5.y, b5.y), FLEN);
         double *O = reinterpret_cast<double *>(out + p);
         st2(O, fa.x, fa.y);
         st2(O + 2, fa.z, c ? 1.0 : 0.0);
@@ -775,7 +778,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_force(const uint2 *ab, const un
 // R (pos/rad, pdis/mass, angv/inert): reads the slot's wall history, writes it back (zeroed
 // when no wall is touched) and returns the number of walls hit with the wall force and
 // torque.  tape_whist receives the incoming history, for the adjoint.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __device__ __forceinline__ int wall_slot(const double *R, const InstDev &I, Hist *whist,
                                          unsigned a, Hist *tape_whist, d3 &fw, d3 &tw)
 {
@@ -792,7 +795,7 @@ __device__ __forceinline__ int wall_slot(const double *R, const InstDev &I, Hist
     double fow;
     int hits = law_walls(I.walls, d3{r0.x, r0.y, r1.x}, d3{r2.x, r2.y, r3.x},
                          d3{r4.x, r4.y, r5.x}, r1.y, r3.y, d3{w0.x, w0.y, w1.x}, I.dt, I.pw,
-                         fw, tw, hw, fow, roll_floor(I.roll_k, I.dt, r5.y, 0.0));
+                         fw, tw, hw, fow, roll_floor(I.roll_k, I.dt, r5.y, 0.0), FLEN);
     if (hits) {
         st2(WH, hw.x, hw.y);
         st2(WH + 2, hw.z, fow);
@@ -805,7 +808,7 @@ __device__ __forceinline__ int wall_slot(const double *R, const InstDev &I, Hist
 
 // Wall contacts of the rebuild's wall candidates for the split path: wforce[slot] =
 // (F xyz, hits, T xyz, 0).  Grid-stride over the device-resident candidate count.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
                                                   const unsigned *wlist, const DevCtl *ctl,
                                                   const InstDev *inst, InstDev one,
@@ -818,8 +821,8 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
         unsigned a = wlist[i];
         const InstDev &I = BATCHED ? inst[rec[a].inst] : one;
         d3 fw, tw;
-        int hits = wall_slot<BATCHED>(reinterpret_cast<const double *>(rec + a), I, whist, a,
-                                      tape_whist, fw, tw);
+        int hits = wall_slot<BATCHED, FLEN>(reinterpret_cast<const double *>(rec + a), I, whist,
+                                            a, tape_whist, fw, tw);
         double *W = wforce + 8 * (size_t)a;
         st2(W, fw.x, fw.y);
         st2(W + 2, fw.z, (double)hits);
@@ -833,7 +836,7 @@ __global__ __launch_bounds__(kBlock) void k_walls(const PRec *rec, Hist *whist,
 // updatepositionD, neiglist.cu:308-409).  R is the record before the step, W receives the
 // record after it (the same record on the split path, the other buffer on the fused one).
 // Returns 1 if the particle moved more than half the skin since the last rebuild.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, unsigned a,
                                                     double fx, double fy, double fz, double tx,
                                                     double ty, double tz, unsigned c, bool wall,
@@ -895,6 +898,9 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
     cp32_out[a] = pos32(x.x, x.y, x.z, r1.y);
     contacti[a] = c;
     double q2 = qx * qx + qy * qy + qzz * qzz;
+    // the reference's rmov = length(qqi) > ann (neiglist.cu:335,359,370) under the
+    // float-length option; |q|^2 > ann^2 in FP64 otherwise
+    if (FLEN) return (double)sqrtf((float)q2) > 0.5 * I.skin ? 1u : 0u;
     return q2 > I.ann2 ? 1u : 0u;
 }
 
@@ -902,7 +908,7 @@ __device__ __forceinline__ unsigned finish_particle(const double *R, double *W, 
 // list order, then pairs where it is second, via the transposed index) from k_pair_force's
 // output, then finish_particle.  Max z and contact totals are not reduced here (they are
 // diagnostics): k_inst_reduce computes them on demand.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __global__ __launch_bounds__(kBlock) void k_particle(
     PRec *rec, const unsigned *wflag, const double *wforce, const unsigned *off,
     const unsigned *toff, const unsigned *jt, const POut *out, float4 *cp32, unsigned n,
@@ -953,7 +959,7 @@ __global__ __launch_bounds__(kBlock) void k_particle(
             }
         }
         double *R = reinterpret_cast<double *>(rec + a);
-        flag = finish_particle<BATCHED>(R, R, a, fx, fy, fz, tx, ty, tz, c, wall, fw, tw, inst,
+        flag = finish_particle<BATCHED, FLEN>(R, R, a, fx, fy, fz, tx, ty, tz, c, wall, fw, tw, inst,
                                         one, contacti, fdbg, tape_rec, cp32);
     }
     // rebuild trigger: one atomic per wave that saw a particle past the threshold (rare)
@@ -961,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_particle(
 }
 
 // Law of pair (A first, B second) from the records' first 96 bytes.
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __device__ __forceinline__ bool pair_from_records(const double *A, const double *B, d3 h,
                                                   const InstDev *inst, const LawParams &pp,
                                                   double dt, double roll_k, unsigned inst_a,
@@ -981,7 +987,8 @@ __device__ __forceinline__ bool pair_from_records(const double *A, const double 
     }
     return law_pair(d3{a0.x, a0.y, a1.x}, d3{b0.x, b0.y, b1.x}, d3{a2.x, a2.y, a3.x},
                     d3{b2.x, b2.y, b3.x}, d3{a4.x, a4.y, a5.x}, d3{b4.x, b4.y, b5.x}, a1.y, b1.y,
-                    a3.y, b3.y, h, tdt, P, fa, ta, tb, ho, fo, roll_floor(rk, tdt, a5.y, b5.y));
+                    a3.y, b3.y, h, tdt, P, fa, ta, tb, ho, fo, roll_floor(rk, tdt, a5.y, b5.y),
+                    FLEN);
 }
 
 // Fused step (one lane per particle): the lane evaluates the law of each of its own pairs
@@ -1123,7 +1130,7 @@ __device__ __forceinline__ d3 sel3(bool c, d3 u, d3 v)
     return d3{c ? u.x : v.x, c ? u.y : v.y, c ? u.z : v.z};
 }
 
-template <bool BATCHED>
+template <bool BATCHED, bool FLEN = false>
 __global__ __launch_bounds__(kBlock, 4) void k_step(
     const PRec *rin, PRec *rout, const unsigned *bo, const unsigned *off, const unsigned *toff,
     const unsigned *jt, const unsigned *jf, const Hist *hin, Hist *hout, const unsigned *wflag,
@@ -1196,7 +1203,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
         const unsigned inst_a = BATCHED ? reinterpret_cast<const uint2 *>(RA + 15)->x : 0u;
         const InstDev &IA = BATCHED ? inst[inst_a] : one;
         // wall contact first (its law needs many registers; few are live here)
-        if (wf) hits = wall_slot<BATCHED>(RA, IA, whist, a, tape_whist, fw, tw);
+        if (wf) hits = wall_slot<BATCHED, FLEN>(RA, IA, whist, a, tape_whist, fw, tw);
         if (p1 > np) p1 = np;
         if (q1 > np) q1 = np;
         const unsigned no = p1 > p0 ? p1 - p0 : 0u, nt = q1 > q0 ? q1 - q0 : 0u;
@@ -1319,7 +1326,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                               sel3(own, vp, vs), sel3(own, ws, wp), sel3(own, wp, ws),
                               own ? rs : rp, own ? rp : rs, own ? ms : mp, own ? mp : ms,
                               d3{h0.x, h0.y, h1.x}, tdt, P, fa, ta, tb, ho, fo,
-                              roll_floor(rk, tdt, own ? s5.y : b5.y, own ? b5.y : s5.y));
+                              roll_floor(rk, tdt, own ? s5.y : b5.y, own ? b5.y : s5.y),
+                              FLEN);
                 if (!valid) {
                     rc = false;
                     fa = ta = tb = d3{0.0, 0.0, 0.0};
@@ -1385,7 +1393,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
                 d3 fa, ta, tb, ho;
                 double fo;
-                bool con = pair_from_records<BATCHED>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
+                bool con = pair_from_records<BATCHED, FLEN>(RA, B, d3{h0.x, h0.y, h1.x}, inst,
                                                       one.pp, one.dt, one.roll_k, inst_a, fa, ta, tb, ho,
                                                       fo);
                 double *HW = reinterpret_cast<double *>(hout + p);
@@ -1420,7 +1428,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
                 d3 fa, ta, tb, ho;
                 double fo;
-                bool con = pair_from_records<BATCHED>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
+                bool con = pair_from_records<BATCHED, FLEN>(F, RA, d3{h0.x, h0.y, h1.x}, inst,
                                                       one.pp, one.dt, one.roll_k, inst_a, fa, ta, tb, ho,
                                                       fo);
                 fx -= fa.x; fy -= fa.y; fz -= fa.z;
@@ -1430,7 +1438,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
         }
     }
     if (live)
-        flag = finish_particle<BATCHED>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, hits != 0, fw, tw,
+        flag = finish_particle<BATCHED, FLEN>(RA, RA, a, fx, fy, fz, tx, ty, tz, c, hits != 0, fw, tw,
                                         inst, one, contacti, fdbg, tape_rec, cout);
     __syncthreads();
     {

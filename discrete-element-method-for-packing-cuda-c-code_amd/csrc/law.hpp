@@ -44,6 +44,14 @@ struct LawParams {
     double estar, mu, nu, gn, gt, ur;
 };
 
+// |v| as vprojec takes it: FP64, or with flen the reference's float length() -- sqrtf of
+// the FP64 dot product rounded to float, returned as double (cutil_math.h:1086-1089).
+DEM_HD double vlen(d3 v, bool flen)
+{
+    const double d = dot3(v, v);
+    return flen ? (double)sqrtf((float)d) : sqrt(d);
+}
+
 constexpr double kXSmall = 1e-30;   // neiglist.cu:29
 
 // Common core.  n: unit normal from body i toward the other body; dn > 0 overlap;
@@ -53,11 +61,12 @@ constexpr double kXSmall = 1e-30;   // neiglist.cu:29
 // roll_c > 0 (the differentiable form's option, off by default) floors the |w_rel| of the
 // rolling moment at roll_c mu_r fcn: below it the moment is viscous, -w_rel / roll_c, instead
 // of flipping its sign every step about rolling rest (law_core_adj gives the reasoning).
-// roll_c = 0 is the reference's moment -mu_r fcn w_rel / |w_rel| bit for bit.
+// roll_c = 0 is the reference's moment -mu_r fcn w_rel / |w_rel| bit for bit.  flen: vprojec's
+// magnitudes (neiglist.cu:140-142) by the reference's float length() instead of FP64.
 DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
                                          d3 wrel, d3 hin, double dt, const LawParams &P,
                                          d3 &fi, d3 &ft, d3 &mr, d3 &hout, double &fout,
-                                         double roll_c = 0.0)
+                                         double roll_c = 0.0, bool flen = false)
 {
     double vns = vrel.x * n.x + vrel.y * n.y + vrel.z * n.z;
     d3 vn = d3{vns * n.x, vns * n.y, vns * n.z};
@@ -71,9 +80,9 @@ DEM_HD void law_core(d3 n, double dn, double rs, double ms, d3 vrel,
     // vprojec(hin, n, &ht, 1)
     d3 q = cross3(n, hin);
     d3 ht = cross3(d3{-n.x, -n.y, -n.z}, q);
-    double pmag = sqrt(dot3(hin, hin));
+    double pmag = vlen(hin, flen);
     if (!(pmag < 0.001)) {
-        double rmag = sqrt(dot3(ht, ht));
+        double rmag = vlen(ht, flen);
         if (!(pmag > rmag * 100.0)) {
             double pr = sqrt(pmag / rmag);
             ht = scale3(ht, pr);
@@ -161,7 +170,8 @@ DEM_HD bool maybe_touch(float4 a, float4 b)
 DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
                                          double rb, double ma, double mb, d3 hin, double dt,
                                          const LawParams &P, d3 &fa, d3 &ta, d3 &tb,
-                                         d3 &hout, double &fout, double roll_c = 0.0)
+                                         d3 &hout, double &fout, double roll_c = 0.0,
+                                         bool flen = false)
 {
     d3 dx = d3{xb.x - xa.x, xb.y - xa.y, xb.z - xa.z};
     double dist = sqrt(dot3(dx, dx));
@@ -180,7 +190,7 @@ DEM_HD bool law_pair(d3 xa, d3 xb, d3 va, d3 vb, d3 wa, d3 wb, double ra,
     d3 vrel = d3{(vb.x - va.x) - c.x, (vb.y - va.y) - c.y, (vb.z - va.z) - c.z};
     d3 wrel = d3{wa.x - wb.x, wa.y - wb.y, wa.z - wb.z};
     d3 ft, mr;
-    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout, roll_c);
+    law_core(n, dn, rs, ms, vrel, wrel, hin, dt, P, fa, ft, mr, hout, fout, roll_c, flen);
     d3 ca = cross3(scale3(n, ra), ft);
     ta = d3{ca.x + mr.x, ca.y + mr.y, ca.z + mr.z};
     d3 cb = cross3(scale3(n, rb), ft);
@@ -222,7 +232,8 @@ DEM_HD double wall_overlap(const WallGeom &g, int k, d3 x, double r, d3 &n)
 DEM_HD int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
                                          double m, d3 hin, double dt, const LawParams &P,
                                          d3 &fw, d3 &tw, d3 &hout, double &fout,
-                                         double roll_c = 0.0)
+                                         double roll_c = 0.0,
+                                         bool flen = false)
 {
     int nw = wall_count(g);
     int prim = -1;
@@ -249,7 +260,7 @@ DEM_HD int law_walls(const WallGeom &g, d3 x, d3 v, d3 w, double r,
         d3 f, ft, mr, ht;
         double fo;
         law_core(n, dn, r, m, vrel, w, k == prim ? hin : d3{0.0, 0.0, 0.0}, dt, P, f, ft, mr,
-                 ht, fo, roll_c);
+                 ht, fo, roll_c, flen);
         d3 cr = cross3(scale3(n, r), ft);
         fw = d3{fw.x + f.x, fw.y + f.y, fw.z + f.z};
         tw = d3{tw.x + (cr.x + mr.x), tw.y + (cr.y + mr.y), tw.z + (cr.z + mr.z)};

@@ -514,9 +514,9 @@ int rebuild_host(dem_ctx *c, pair_hook_fn hook = nullptr, void *hook_arg = nullp
 // (one lane per particle, wall contacts included, both sides of each pair evaluated, records
 // and histories from the current buffers into the other ones; the buffer indices flip).  `mid` is
 // recorded before the per-particle kernel, `end` after it.
-void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
-                       hipEvent_t end = nullptr, const unsigned *gate = nullptr,
-                       unsigned *slot = nullptr)
+template <bool FLEN>
+void launch_force_step_t(dem_ctx *c, unsigned np_grid, hipEvent_t mid, hipEvent_t end,
+                         const unsigned *gate, unsigned *slot)
 {
     hipStream_t st = c->stream;
     unsigned pb = c->particle_block;
@@ -529,21 +529,21 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
     if (!c->fused) {
         Hist *h = c->hist[c->hc];
         if (c->batched)
-            hipLaunchKernelGGL(k_pair_force<true>, dim3(np_grid), dim3(c->pair_block), 0, st,
+            hipLaunchKernelGGL((k_pair_force<true, FLEN>), dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
                                dinst, one.pp, one.dt, one.roll_k, c->tape_hist, gate, c->xcd_chunk);
         else
-            hipLaunchKernelGGL(k_pair_force<false>, dim3(np_grid), dim3(c->pair_block), 0, st,
+            hipLaunchKernelGGL((k_pair_force<false, FLEN>), dim3(np_grid), dim3(c->pair_block), 0, st,
                                (const uint2 *)c->ab[c->lc], np, rin, (const Hist *)h, h, c->out,
                                dinst, one.pp, one.dt, one.roll_k, c->tape_hist, gate, c->xcd_chunk);
     }
     if (!c->fused) {   // the fused step evaluates the wall contacts itself
         if (c->batched)
-            hipLaunchKernelGGL(k_walls<true>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+            hipLaunchKernelGGL((k_walls<true, FLEN>), dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
                                c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
                                dinst, one, c->wforce, c->tape_whist, gate);
         else
-            hipLaunchKernelGGL(k_walls<false>, dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
+            hipLaunchKernelGGL((k_walls<false, FLEN>), dim3(wall_grid(c)), dim3(kBlock), 0, st, rin,
                                c->whist[c->pc], (const unsigned *)c->wlist, (const DevCtl *)c->ctl,
                                dinst, one, c->wforce, c->tape_whist, gate);
     }
@@ -565,7 +565,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by the step's wall contact, so they move with the records
         if (c->batched)
-            hipLaunchKernelGGL(k_step<true>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
+            hipLaunchKernelGGL((k_step<true, FLEN>), dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
@@ -574,7 +574,7 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
                                c->xcd_chunk, ktrig, queued ? slot : nullptr);
         else
-            hipLaunchKernelGGL(k_step<false>, dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
+            hipLaunchKernelGGL((k_step<false, FLEN>), dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
                                (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
@@ -587,13 +587,13 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
         c->out_valid = true;
         c->last_path = 2;
     } else if (c->batched) {
-        hipLaunchKernelGGL(k_particle<true>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
+        hipLaunchKernelGGL((k_particle<true, FLEN>), dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->cp32[c->rc], c->n, dinst,
                            one, c->contacti, fdbg, c->ctl, c->tape_rec, gate, c->xcd_chunk);
     } else {
-        hipLaunchKernelGGL(k_particle<false>, dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
+        hipLaunchKernelGGL((k_particle<false, FLEN>), dim3(gb), dim3(pb), 0, st, c->rec[c->rc],
                            (const unsigned *)c->wflag, (const double *)c->wforce,
                            (const unsigned *)c->off[c->lc], (const unsigned *)c->toff,
                            (const unsigned *)c->jt, (const POut *)c->out, c->cp32[c->rc], c->n, dinst,
@@ -606,6 +606,18 @@ void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
     if (end) (void)hipEventRecord(end, st);
     if (!(c->fused && slot))   // a queued fused step needs no publish kernel
         hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, c->ctl, c->mirror_d, gate, slot);
+}
+
+// The float-length option selects the kernels' FLEN instantiation (the default path carries
+// no test of it).
+void launch_force_step(dem_ctx *c, unsigned np_grid, hipEvent_t mid = nullptr,
+                       hipEvent_t end = nullptr, const unsigned *gate = nullptr,
+                       unsigned *slot = nullptr)
+{
+    if (c->hinst[0].flen)
+        launch_force_step_t<true>(c, np_grid, mid, end, gate, slot);
+    else
+        launch_force_step_t<false>(c, np_grid, mid, end, gate, slot);
 }
 
 InstDev make_inst(const dem_instance &d, unsigned cell_off, unsigned p_off)
@@ -1057,6 +1069,14 @@ int dem_set_option(dem_ctx *c, int option, double value)
         if (!(value >= 0.0 && value <= 1e6)) return DEM_ERR_INVALID;
         DEM_HIP(hipStreamSynchronize(c->stream));
         for (InstDev &I : c->hinst) I.roll_k = value;
+        DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
+        drop_graph(c);
+        return DEM_OK;
+    }
+    case DEM_OPT_FLOAT_LENGTH: {
+        if (!(value == 0.0 || value == 1.0)) return DEM_ERR_INVALID;
+        DEM_HIP(hipStreamSynchronize(c->stream));
+        for (InstDev &I : c->hinst) I.flen = (unsigned)value;
         DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
         drop_graph(c);
         return DEM_OK;
@@ -2131,18 +2151,17 @@ int dem_get_pairs(dem_ctx *c, uint *ids, double *hist, double *out, uint cap)
             }
             InstDev one = c->hinst[0];
             unsigned g = pair_grid(c, (unsigned)np);
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(g), dim3(c->pair_block), 0, c->stream,
+                                   (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
+                                   (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
+                                   (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
+                                   one.roll_k, (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
+            };
             if (c->batched)
-                hipLaunchKernelGGL(k_pair_force<true>, dim3(g), dim3(c->pair_block), 0, c->stream,
-                                   (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
-                                   (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
-                                   (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   one.roll_k, (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
+                one.flen ? go(k_pair_force<true, true>) : go(k_pair_force<true, false>);
             else
-                hipLaunchKernelGGL(k_pair_force<false>, dim3(g), dim3(c->pair_block), 0, c->stream,
-                                   (const uint2 *)c->ab[c->lc], (const unsigned *)&c->ctl->npairs,
-                                   (const PRec *)c->rec[1 - c->rc], (const Hist *)c->hist[1 - c->hc],
-                                   (Hist *)nullptr, c->out, (const InstDev *)c->dinst, one.pp, one.dt,
-                                   one.roll_k, (Hist *)nullptr, (const unsigned *)nullptr, c->xcd_chunk);
+                one.flen ? go(k_pair_force<false, true>) : go(k_pair_force<false, false>);
             DEM_HIP(hipGetLastError());
             DEM_HIP(hipStreamSynchronize(c->stream));
         }
@@ -3058,6 +3077,9 @@ int dem_observe(dem_ctx *c, const dem_obs *obs, double *phi, double *cn)
 int dem_adj_forward(dem_ctx *c, int nsteps, int every, const dem_obs *obs, double *phi, double *cn)
 {
     if (!c || nsteps < 1 || every < 1 || !obs_valid(obs) || c->n != c->cap_n) return DEM_ERR_INVALID;
+    // the reverse sweep differentiates the FP64 law; the float-length option is not supported
+    for (const InstDev &I : c->hinst)
+        if (I.flen) return DEM_ERR_INVALID;
     int rc;
     DEM_HIP(hipStreamSynchronize(c->stream));
     drop_graph(c);

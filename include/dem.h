@@ -228,12 +228,18 @@ enum {
                                       (1..16, at most DEM_OPT_PIPELINE; default 8): an event
                                       between two kernels adds ~4 us to the gap between them.
                                       Same results bit for bit */
-    DEM_OPT_ROLL_FLOOR = 15        /* differentiable form: every instance's rolling moment
+    DEM_OPT_ROLL_FLOOR = 15,       /* differentiable form: every instance's rolling moment
                                       floors |w_rel| at k mu_r fcn dt^2 (1/I_a + 1/I_b) (one
                                       inertia for a wall), i.e. it is viscous, -w_rel / (k dt^2
                                       (1/I_a + 1/I_b)), where the reference's moment would flip
                                       sign every step; the adjoint then reverses that forward
                                       exactly.  k >= 0; 0 (default) = the reference's moment */
+    DEM_OPT_FLOAT_LENGTH = 16      /* 1: the reference's float length() -- sqrtf of the FP64 dot
+                                      product rounded to float (cutil_math.h:1086-1089) -- for
+                                      vprojec's magnitudes (neiglist.cu:140-142) and the rebuild
+                                      trigger rmov = length(q) > 0.5 skin (neiglist.cu:359,370),
+                                      in every instance's pair and wall laws; 0 (default): FP64.
+                                      Not supported by dem_adj_forward (DEM_ERR_INVALID) */
 };
 
 /* Result of dem_autotune. */

@@ -305,6 +305,18 @@ static inline double orc_len(const double *v)
     return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
 }
 
+/* Float-length compatibility (off by default): the reference's length() is sqrtf of the
+ * FP64 dot product rounded to float, returned as double (cutil_math.h:1086-1089).  On, it
+ * replaces the FP64 length in vprojec (neiglist.cu:140-142) and the rebuild trigger
+ * (neiglist.cu:359,370); the library's DEM_OPT_FLOAT_LENGTH does the same. */
+static int orc_float_len = 0;
+void orc_set_float_length(int on) { orc_float_len = on != 0; }
+static inline double orc_len_ref(const double *v)
+{
+    double d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    return orc_float_len ? (double)sqrtf((float)d) : sqrt(d);
+}
+
 /* neiglist.cu:133-146 */
 static inline void orc_vprojec(const double *p, const double *u, double *r, int ityp)
 {
@@ -312,9 +324,9 @@ static inline void orc_vprojec(const double *p, const double *u, double *r, int 
     orc_cross(u, p, q);
     orc_cross(mu, q, r);
     if (ityp == 0) return;
-    double pmag = orc_len(p);
+    double pmag = orc_len_ref(p);
     if (pmag < 0.001) return;
-    double rmag = orc_len(r);
+    double rmag = orc_len_ref(r);
     if (pmag > rmag * 100) return;
     double pr = sqrt(pmag / rmag);
     r[0] = r[0] * pr; r[1] = r[1] * pr; r[2] = r[2] * pr;
@@ -582,13 +594,15 @@ void orc_wall_gravity(int n, const orc_walls *w, const double *pos, const double
 
 /* ------------------------------------------------------------------------------------ */
 /* A12: integrator (neiglist.cu:308-409; launcher + host max particleSystem.cu:315-384).  */
-/* Rebuild flag: |q|^2 > ann2 with ann = 0.5*(cub - cua) (neiglist.cu:335,359,370).       */
+/* Rebuild flag: |q|^2 > ann2 with ann = 0.5*(cub - cua) (neiglist.cu:335,359,370); with  */
+/* the float-length compatibility on, (double)sqrtf((float)|q|^2) > ann as the reference.  */
 /* maxhz = max(0, max z) (particleSystem.cu:373-379).                                    */
 /* ------------------------------------------------------------------------------------ */
 void orc_update(int n, double *pos, double *pdis, double *angv, const double *rmass,
                 const double *inert, const double *forcei, const double *torquei, double *qi,
-                double dt, double ann2, uint *iall, double *maxhz)
+                double dt, double ann2, double ann, uint *iall, double *maxhz)
 {
+    const int flen = orc_float_len;
     uint flag = 0;
     double mz = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : flag) reduction(max : mz)
@@ -600,7 +614,7 @@ void orc_update(int n, double *pos, double *pdis, double *angv, const double *rm
             angv[3 * i + k] = angv[3 * i + k] + torquei[3 * i + k] * dt * dt / inert[i];
         }
         double q2 = qi[3 * i] * qi[3 * i] + qi[3 * i + 1] * qi[3 * i + 1] + qi[3 * i + 2] * qi[3 * i + 2];
-        if (q2 > ann2) flag = 1;
+        if (flen ? (double)sqrtf((float)q2) > ann : q2 > ann2) flag = 1;
         if (pos[3 * i + 2] > mz) mz = pos[3 * i + 2];
     }
     *iall = flag;
@@ -705,7 +719,7 @@ int orc_step(const orc_setup *s, orc_state *st)
     orc_wall_gravity(n, &s->walls, st->pos, st->pdis, st->angv, st->rad, st->rmass, s->dt,
                      st->npairs, &s->mat, st->disptw, st->fricpw, st->forcei, st->torquei);
     orc_update(n, st->pos, st->pdis, st->angv, st->rmass, st->inert, st->forcei, st->torquei,
-               st->qi, s->dt, s->ann2, &st->iall, &st->maxhz);
+               st->qi, s->dt, s->ann2, 0.5 * s->skin, &st->iall, &st->maxhz);
     st->pcntot = orc_contact_total(n, st->contacti);
     return 0;
 }

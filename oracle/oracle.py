@@ -69,7 +69,9 @@ def lib():
                             None),
         "orc_sum_forces": ([c_int, c_int, P, P, P, P, P, P, P, P, P, P], None),
         "orc_wall_gravity": ([c_int, P, P, P, P, P, P, c_double, c_uint, P, P, P, P, P], None),
-        "orc_update": ([c_int, P, P, P, P, P, P, P, P, c_double, c_double, P, P], None),
+        "orc_update": ([c_int, P, P, P, P, P, P, P, P, c_double, c_double, c_double, P, P],
+                       None),
+        "orc_set_float_length": ([c_int], None),
         "orc_contact_total": ([c_int, P], c_uint),
         "orc_rebuild": ([P, P], c_int),
         "orc_step": ([P, P], c_int),
@@ -134,9 +136,12 @@ class OracleSim:
     """Reference-layout CPU simulation of one instance (dem_amd.Instance + state arrays)."""
 
     def __init__(self, inst, pos, pdis, angv, rad, rmass, inert, width: int = 64,
-                 pair_cap: int | None = None):
+                 pair_cap: int | None = None, float_length: bool = False):
         L = lib()
         self.L = L
+        # the reference's float length() in vprojec and the rebuild trigger (a process-wide
+        # switch of the C oracle, set before each rebuild and step of this simulation)
+        self.float_length = bool(float_length)
         n = int(rad.shape[0])
         self.n = n
         self.width = width
@@ -191,10 +196,14 @@ class OracleSim:
         self.L.orc_state_set_n(self.st, int(n))
 
     def rebuild(self) -> int:
+        self.L.orc_set_float_length(1 if self.float_length else 0)
         return self.L.orc_rebuild(ctypes.byref(self.setup), self.st)
 
     def step(self, k: int = 1) -> int:
-        return self.L.orc_run(ctypes.byref(self.setup), self.st, int(k))
+        self.L.orc_set_float_length(1 if self.float_length else 0)
+        rc = self.L.orc_run(ctypes.byref(self.setup), self.st, int(k))
+        self.L.orc_set_float_length(0)
+        return rc
 
     def pairs(self):
         """Current pair list (i, j), i < j, list order."""
@@ -215,9 +224,11 @@ def brute_pairs(pos, rad, skin):
     return out[:c].copy()
 
 
-def law_pair(xi, xj, vi, vj, wi, wj, ri, rj, mi, mj, hin, dt, mat):
-    """Oracle pair law for one pair; returns (contact, F_i, T_i, T_j, h_out, |Ft|)."""
+def law_pair(xi, xj, vi, vj, wi, wj, ri, rj, mi, mj, hin, dt, mat, float_length=False):
+    """Oracle pair law for one pair; returns (contact, F_i, T_i, T_j, h_out, |Ft|).
+    float_length: the reference's float length() in vprojec."""
     L = lib()
+    L.orc_set_float_length(1 if float_length else 0)
     a = [np.ascontiguousarray(v, dtype=np.float64) for v in (xi, xj, vi, vj, wi, wj, hin)]
     hout = np.zeros(3)
     fout = c_double()
@@ -228,6 +239,7 @@ def law_pair(xi, xj, vi, vj, wi, wj, ri, rj, mi, mj, hin, dt, mat):
     c = L.orc_law_pair(ptr(a[0]), ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(a[5]), ri, rj, mi,
                        mj, ptr(a[6]), 0.0, dt, ctypes.byref(m), ptr(hout), ctypes.byref(fout),
                        ptr(fi), ptr(ti), ptr(tj))
+    L.orc_set_float_length(0)
     return c, fi, ti, tj, hout, fout.value
 
 

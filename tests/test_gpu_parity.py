@@ -429,6 +429,53 @@ def test_workloads_without_pairs(dem, orc, gpu_ok, n, mode):
     assert abs(st["pos"][0, 2] - 0.5) < 0.01, "the lowest sphere rests on the floor"
 
 
+@pytest.mark.parametrize("mode", ["fused", "split", "graph"])
+def test_float_length_option(dem, orc, gpu_ok, mode):
+    """DEM_OPT_FLOAT_LENGTH: the reference's float length() in vprojec and in the rebuild
+    trigger.  Fast spheres (histories past vprojec's 0.001 threshold, several triggers): the
+    trajectory and the rebuild steps equal the oracle run with the same option, and differ
+    from the FP64 default; the adjoint form refuses the option."""
+    say(f"[float-length] mode={mode}")
+    w = dense_workload(dem, n=2000, seed=8)
+    w.pdis *= 200.0
+    K = 10
+    graph = mode == "graph"
+    ctx, sim = make_pair(dem, orc, w, interval=K if graph else 0)
+    sim.float_length = True
+    ctx.set_option(dem.OPT_FLOAT_LENGTH, 1)
+    ctx.set_option(dem.OPT_FUSED_STEP, 0 if mode == "split" else 1)
+    steps = 120
+    if graph:
+        ctx.step_graph(steps)
+        oracle_steps(sim, steps, K, 0)
+    else:
+        rebuilds = 0
+        for _ in range(steps):
+            rebuilds += 1 if sim.scalars()["iall"] else 0
+            assert sim.step(1) == 0
+        ctx.step(steps)
+        assert ctx.stats().rebuilds == rebuilds and rebuilds >= 2
+    st = ctx.state()
+    for key in ("pdis", "angv"):
+        assert rel_err(st[key], sim.field(key)) <= TOL, key
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+    ref = dem.Context(w.instances)
+    ref.load(w)
+    ref.set_option(dem.OPT_FUSED_STEP, 0 if mode == "split" else 1)
+    if graph:
+        ref.set_option(dem.OPT_REBUILD_INTERVAL, K)
+        ref.step_graph(steps)
+    else:
+        ref.step(steps)
+    assert not np.array_equal(ref.state()["pdis"], st["pdis"]), "the option must change the bits"
+    with pytest.raises(dem.DemError) as ei:
+        ctx.adj_forward(10, 5, dem.make_obs(zhi=10.0))
+    assert ei.value.code == dem.DEM_ERR_INVALID
+    for bad in (0.5, 2.0, -1.0):
+        with pytest.raises(dem.DemError):
+            ctx.set_option(dem.OPT_FLOAT_LENGTH, bad)
+
+
 def test_pair_capacity_grows(dem, orc, gpu_ok):
     w = dense_workload(dem, n=1500, seed=17)
     ctx = dem.Context(w.instances, pair_capacity=64)

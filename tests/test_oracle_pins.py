@@ -337,3 +337,63 @@ def test_box_corner_two_walls_push(dem, orc):
     assert hits == 2
     k = mat.w.estarw * math.sqrt(r)
     assert np.allclose(fw, [k * d_side ** 1.5, 0.0, k * d_floor ** 1.5], rtol=1e-10, atol=0)
+
+
+# ---------------------------------------------------------------------------------------
+# Float-length compatibility: the reference's length() is sqrtf of the FP64 dot product
+# rounded to float (cutil_math.h:1086-1089), in vprojec (neiglist.cu:140-142) and in the
+# rebuild trigger (neiglist.cu:359,370)
+# ---------------------------------------------------------------------------------------
+
+def _float_len(v):
+    d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2]
+    return float(np.sqrt(np.float32(d)))
+
+
+def test_vprojec_rescale_with_float_length(dem, orc):
+    """Two spheres on the z axis (normal exactly (0, 0, 1)), no relative motion, an incoming
+    history with a normal component (|h| > 0.001, below the spring cap of the 0.5 overlap):
+    the rotated history is (hx, hy, 0) * sqrt(pmag / rmag), with the magnitudes in float
+    under the option and in FP64 without it."""
+    mat = _mat(dem, mu=0.5, mu_w=0.5)
+    z3 = np.zeros(3)
+    xi, xj = np.zeros(3), np.array([0.0, 0.0, 0.5])
+    hin = np.array([1.234567e-3, -7.654321e-4, 3.3e-4])
+    r = np.array([hin[0], hin[1], 0.0])
+    out = {}
+    for flen in (False, True):
+        c, _, _, _, hout, _ = orc.law_pair(xi, xj, z3, z3, z3, z3, 0.5, 0.5, 1.0, 1.0, hin,
+                                           1e-4, mat, float_length=flen)
+        assert c == 1
+        if flen:
+            pmag, rmag = _float_len(hin), _float_len(r)
+        else:
+            pmag, rmag = math.sqrt(float(hin @ hin)), math.sqrt(float(r @ r))
+        pr = math.sqrt(pmag / rmag)
+        want = np.array([r[0] * pr, r[1] * pr, 0.0])
+        assert np.array_equal(hout, want), (flen, hout, want)
+        out[flen] = hout
+    assert not np.array_equal(out[False], out[True]), "float magnitudes must change the bits"
+
+
+def test_rebuild_trigger_with_float_length(orc):
+    """One particle displaced by |q| = 0.2749999999 since the last rebuild: under the FP64
+    test |q| > 0.275 (half the skin) is false, under the reference's float length() the
+    magnitude rounds to the float 0.27500000596 and the rebuild triggers."""
+    import ctypes
+    L = orc.lib()
+    ann = 0.5 * 0.55
+    for flen, want in ((0, 0), (1, 1)):
+        L.orc_set_float_length(flen)
+        pos, pdis, angv = np.zeros(3), np.zeros(3), np.zeros(3)
+        qi = np.array([0.2749999999, 0.0, 0.0])
+        f0 = np.zeros(3)
+        one = np.ones(1)
+        iall = ctypes.c_uint(7)
+        mz = ctypes.c_double(0.0)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        L.orc_update(1, P(pos), P(pdis), P(angv), P(one), P(one), P(f0), P(f0), P(qi), 1e-4,
+                     ann * ann, ann, ctypes.byref(iall), ctypes.byref(mz))
+        assert iall.value == want, (flen, iall.value)
+    L.orc_set_float_length(0)
+    assert _float_len(np.array([0.2749999999, 0.0, 0.0])) > ann > 0.2749999999
