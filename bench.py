@@ -435,6 +435,13 @@ def bench_batched(dem, steps, plain_steps, interval, tune_steps=48, graph=True):
     tb = time.perf_counter() - t0
     nb = w.n
     flag = big.stats().last_rebuild_flag
+    # the dominant kernel's roofline: the same schedule through the host loop with HIP events
+    # around every step's kernels (2 rebuild intervals)
+    big.set_option(dem.OPT_TIME_STEPS, 1)
+    big.step_timing()
+    big.step(2 * interval)
+    big.sync()
+    roof = dominant_kernel(big.step_timing(), nb, "batched")
     del big
     # per-instance plain launches: one context per instance, run back to back
     offs = [0]
@@ -471,7 +478,7 @@ def bench_batched(dem, steps, plain_steps, interval, tune_steps=48, graph=True):
             "per_instance_steps": plain_steps, "rebuild_interval": interval, "value": vb,
             "per_instance_plain": vp, "per_instance_graph": vg, "ratio_vs_plain": vb / vp,
             "ratio_vs_plain_graph": vb / vg if vg else None, "ms_per_step": tb / steps * 1e3,
-            "trigger_in_last_group": int(flag), "autotune": tune}
+            "trigger_in_last_group": int(flag), "autotune": tune, "roofline": roof}
 
 
 def bench_adjoint(dem, nsteps=5000, every=100, fd_rel=1e-4, graph=True, roll_floor=3.0):
