@@ -350,13 +350,15 @@ constexpr unsigned kTileMax = 4 * kTile;
 // are staged in LDS (tile: kTile entries) by coalesced loads, and each lane tests its own
 // range from there in ascending slot order: the same candidates in the same order as
 // for_candidates, with each position read from HBM once per block instead of once per lane.
-// f(b, s0, s1) gets each candidate b with the lane's range [s0, s1) of its row; row_done(r,
-// s0, s1) follows each row r = 0..8 on every lane (s0 = kEmpty, s1 = 0: no range).
+// f(b, s0, s1, id) gets each candidate b with the lane's range [s0, s1) of its row and b's
+// particle index (cid, staged with the positions); row_done(r, s0, s1) follows each row
+// r = 0..8 on every lane (s0 = kEmpty, s1 = 0: no range).
 template <bool BATCHED, class F, class R>
-__device__ __forceinline__ void for_candidates_block(const double4 *cpos, unsigned a, bool live,
-                                                     unsigned key, const InstDev &I,
-                                                     const unsigned *cstart, const unsigned *cend,
-                                                     double4 *tile, unsigned *red, F &&f,
+__device__ __forceinline__ void for_candidates_block(const double4 *cpos, const unsigned *cid,
+                                                     unsigned a, bool live, unsigned key,
+                                                     const InstDev &I, const unsigned *cstart,
+                                                     const unsigned *cend, double4 *tile,
+                                                     unsigned *tcid, unsigned *red, F &&f,
                                                      R &&row_done)
 {
     double4 ra = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -391,21 +393,24 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, unsign
                 for (unsigned b = s0; b < s1; b++) {
                     if (b == a) continue;
                     const double4 rb = cpos[b];
-                    if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin)) f(b, s0, s1);
+                    if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
+                        f(b, s0, s1, cid[b]);
                 }
             } else {
                 for (unsigned c0 = lo; c0 < hi; c0 += kTile) {   // (none when lo >= hi)
                     const unsigned c1 = min(c0 + kTile, hi);
                     __syncthreads();   // the previous chunk's readers are done
-                    for (unsigned k = threadIdx.x; c0 + k < c1; k += blockDim.x)
+                    for (unsigned k = threadIdx.x; c0 + k < c1; k += blockDim.x) {
                         tile[k] = cpos[c0 + k];
+                        tcid[k] = cid[c0 + k];
+                    }
                     __syncthreads();
                     const unsigned b1 = min(s1, c1);
                     for (unsigned b = max(s0, c0); b < b1; b++) {
                         if (b == a) continue;
                         const double4 rb = tile[b - c0];
                         if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
-                            f(b, s0, s1);
+                            f(b, s0, s1, tcid[b - c0]);
                     }
                 }
             }
@@ -413,16 +418,20 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, unsign
         }
 }
 
-// Candidates of one stencil row of a lane as a bit mask: bit k for slot s0 + k.  Rows longer
-// than kRowBits slots are not recorded (the fill tests them again).
+// The count's record of each stencil row of a lane for the fill: the row's first slot s0 and
+// two bit masks over its slots (bit k for slot s0 + k), the candidates with a larger particle
+// index (the lane's own pairs) and those with a smaller one (its transposed entries).  Rows
+// longer than kRowBits slots are marked kLongRow (the fill tests them again); rows without
+// slots kEmpty.
 constexpr unsigned kRowBits = 64;
 constexpr unsigned kRows = 9;
+constexpr unsigned kLongRow = kEmpty - 1u;
 
-// The lane's stencil rows r = 0..8 (z-, y-offset ascending) with their slot ranges, as
-// for_candidates_block forms them: fn(r, s0, s1) for each non-empty row.
-template <class Fn>
-__device__ __forceinline__ void for_rows(unsigned key, const InstDev &I, const unsigned *cstart,
-                                         const unsigned *cend, Fn &&fn)
+// Slot range [s0, s1) of stencil row r (z offset r / 3 - 1, y offset r % 3 - 1) of the cell
+// with key `key`, as for_candidates_block forms it; false if the row is empty or off the grid.
+__device__ __forceinline__ bool stencil_row(unsigned key, const InstDev &I, const unsigned *cstart,
+                                            const unsigned *cend, int r, unsigned &s0,
+                                            unsigned &s1)
 {
     const unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2], gxy = gx * gy;
     const unsigned local = key - I.cell_off;
@@ -431,17 +440,10 @@ __device__ __forceinline__ void for_rows(unsigned key, const InstDev &I, const u
     const int cx = (int)(local - (unsigned)cz * gxy - (unsigned)cy * gx);
     const int x0 = cx > 0 ? cx - 1 : 0;
     const int x1 = cx + 1 < (int)gx ? cx + 1 : (int)gx - 1;
-    int r = -1;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++) {
-            r++;
-            const int z = cz + dz, y = cy + dy;
-            if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) continue;
-            const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
-            unsigned s0, s1;
-            if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) continue;
-            fn(r, s0, s1);
-        }
+    const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+    if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
+    const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
+    return row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1);
 }
 
 template <bool BATCHED>
@@ -451,27 +453,38 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
                                                       InstDev one, const unsigned *cstart,
                                                       const unsigned *cend, unsigned *ng,
                                                       unsigned *nl, unsigned long long *rmask,
-                                                      const unsigned *active)
+                                                      unsigned *rs0, const unsigned *active)
 {
     if (!is_active(active)) return;   // uniform over the grid
     __shared__ double4 tile[kTile];
+    __shared__ unsigned tcid[kTile];
     __shared__ unsigned red[2 * (kBlock / 64)];
     const unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     const bool live = a < n;
     const InstDev &I = BATCHED ? inst[rec[live ? a : 0u].inst] : one;
     unsigned cg = 0, cl = 0;
-    unsigned long long m = 0ull;
+    unsigned long long mg = 0ull, ml = 0ull;
     const unsigned ida = live ? cid[a] : 0u;
     for_candidates_block<BATCHED>(
-        cpos, a, live, live ? sorted[a].x : 0u, I, cstart, cend, tile, red,
-        [&](unsigned b, unsigned r0, unsigned r1) {
-            if (cid[b] > ida) cg++;
-            else cl++;
-            if (r1 - r0 <= kRowBits) m |= 1ull << (b - r0);
+        cpos, cid, a, live, live ? sorted[a].x : 0u, I, cstart, cend, tile, tcid, red,
+        [&](unsigned b, unsigned r0, unsigned r1, unsigned idb) {
+            const unsigned long long bit = r1 - r0 <= kRowBits ? 1ull << (b - r0) : 0ull;
+            if (idb > ida) {
+                cg++;
+                mg |= bit;
+            } else {
+                cl++;
+                ml |= bit;
+            }
         },
-        [&](int r, unsigned, unsigned) {
-            if (live) rmask[(size_t)r * n + a] = m;
-            m = 0ull;
+        [&](int r, unsigned r0, unsigned r1) {
+            if (live) {
+                const size_t k = (size_t)r * n + a;
+                rmask[2 * k] = mg;
+                rmask[2 * k + 1] = ml;
+                rs0[k] = r0 == kEmpty ? kEmpty : (r1 - r0 <= kRowBits ? r0 : kLongRow);
+            }
+            mg = ml = 0ull;
         });
     if (live) {
         ng[a] = cg;
@@ -511,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const InstDev *inst, InstDev one,
     const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
     unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, unsigned *jf, DevCtl *ctl,
-    const unsigned long long *rmask, const unsigned *active)
+    const unsigned long long *rmask, const unsigned *rs0, const unsigned *active)
 {
     if (!is_active(active)) return;   // uniform over the grid
     __shared__ uint2 s_ab[kFillStage];
@@ -528,50 +541,63 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
         unsigned pg = off[a];
         unsigned pl = toff[a];
         const unsigned ida = cid[a];
-        auto emit = [&](unsigned b) {
-            if (cid[b] > ida) {
-                if (pg >= cap) { over = true; return; }
-                if (stage_g) {
-                    s_ab[pg - G0] = make_uint2(a, b);
-                    s_bo[pg - G0] = b;
-                } else {
-                    ab[pg] = make_uint2(a, b);
-                    bo[pg] = b;
-                }
-                pg++;
-            } else {
-                if (pl < cap) {
-                    if (stage_t) {
-                        s_jf[pl - T0] = b;
-                        s_jt[pl - T0] = a;
-                    } else {
-                        jf[pl] = b;
-                        jt[pl] = a;
-                    }
-                }
-                pl++;
+        auto own = [&](unsigned b) {
+            if (pg >= cap) {
+                over = true;
+                return;
             }
+            if (stage_g) {
+                s_ab[pg - G0] = make_uint2(a, b);
+                s_bo[pg - G0] = b;
+            } else {
+                ab[pg] = make_uint2(a, b);
+                bo[pg] = b;
+            }
+            pg++;
         };
-        // the count's candidate masks, in the same rows and slot order; rows longer than
-        // kRowBits are tested again
-        const double4 ra = cpos[a];
-        const d3 xa = d3{ra.x, ra.y, ra.z};
-        for_rows(sorted[a].x, I, cstart, cend, [&](int r, unsigned r0, unsigned r1) {
-            if (r1 - r0 <= kRowBits) {
-                unsigned long long m = rmask[(size_t)r * n + a];
-                while (m) {
-                    const unsigned k = (unsigned)__builtin_ctzll(m);
-                    m &= m - 1ull;
-                    emit(r0 + k);
+        auto transposed = [&](unsigned b) {
+            if (pl < cap) {
+                if (stage_t) {
+                    s_jf[pl - T0] = b;
+                    s_jt[pl - T0] = a;
+                } else {
+                    jf[pl] = b;
+                    jt[pl] = a;
+                }
+            }
+            pl++;
+        };
+        // the count's record of each row, in the same row and slot order: the larger-index
+        // candidates go to the own list, the others to the transposed list (each list keeps
+        // candidate order); rows longer than kRowBits are tested again
+        for (int r = 0; r < (int)kRows; r++) {
+            const size_t k = (size_t)r * n + a;
+            const unsigned r0 = rs0[k];
+            if (r0 == kEmpty) continue;
+            if (r0 != kLongRow) {
+                unsigned long long g = rmask[2 * k], l = rmask[2 * k + 1];
+                while (g) {
+                    own(r0 + (unsigned)__builtin_ctzll(g));
+                    g &= g - 1ull;
+                }
+                while (l) {
+                    transposed(r0 + (unsigned)__builtin_ctzll(l));
+                    l &= l - 1ull;
                 }
             } else {
-                for (unsigned b = r0; b < r1; b++) {
+                unsigned b0, b1;
+                stencil_row(sorted[a].x, I, cstart, cend, r, b0, b1);
+                const double4 ra = cpos[a];
+                const d3 xa = d3{ra.x, ra.y, ra.z};
+                for (unsigned b = b0; b < b1; b++) {
                     if (b == a) continue;
                     const double4 rb = cpos[b];
-                    if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin)) emit(b);
+                    if (!is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin)) continue;
+                    if (cid[b] > ida) own(b);
+                    else transposed(b);
                 }
             }
-        });
+        }
     }
     __syncthreads();
     if (stage_g)

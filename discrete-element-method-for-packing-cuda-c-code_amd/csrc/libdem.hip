@@ -200,7 +200,8 @@ struct dem_ctx {
     uint2 *kv_hash = nullptr, *kv_a = nullptr, *kv_b = nullptr, *kv_sorted = nullptr;
     unsigned *cstart = nullptr, *cend = nullptr;
     unsigned *ng = nullptr, *nl = nullptr, *toff = nullptr;
-    unsigned long long *rmask = nullptr;   // the count's candidate masks, kRows words per slot
+    unsigned long long *rmask = nullptr;   // the count's row masks, 2 kRows words per slot
+    unsigned *rs0 = nullptr;               // the count's row starts, kRows words per slot
     unsigned *off[2] = {nullptr, nullptr};
     uint2 *ab[2] = {nullptr, nullptr};
     unsigned cap[2] = {0, 0};
@@ -404,12 +405,12 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->ng, c->nl, c->rmask, active);
+                           c->ng, c->nl, c->rmask, c->rs0, active);
     else
         hipLaunchKernelGGL(k_nbr_count<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
-                           c->ng, c->nl, c->rmask, active);
+                           c->ng, c->nl, c->rmask, c->rs0, active);
     stage_event(ev, kEvCount, st);
     int nl = 1 - c->lc;
     if (scan_excl(c->S, c->ng, n, c->off[nl], c->S.total, st, active)) return DEM_ERR_HIP;
@@ -441,13 +442,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
-                           (const unsigned long long *)c->rmask, active);
+                           (const unsigned long long *)c->rmask, (const unsigned *)c->rs0, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
                            (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
                            c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
-                           (const unsigned long long *)c->rmask, active);
+                           (const unsigned long long *)c->rmask, (const unsigned *)c->rs0, active);
     stage_event(ev, kEvFill, st);
     // histories by pair identity, one lane per pair over the device pair count
     const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
@@ -701,7 +702,7 @@ void free_ctx(dem_ctx *c)
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
-                    c->nl, c->rmask, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
+                    c->nl, c->rmask, c->rs0, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->jf, c->bo, c->zs, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -965,7 +966,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->cend, sizeof(unsigned) * (size_t)c->ncells));
     DEM_HIP(hipMalloc(&c->ng, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
-    DEM_HIP(hipMalloc(&c->rmask, sizeof(unsigned long long) * kRows * (size_t)n));
+    DEM_HIP(hipMalloc(&c->rmask, sizeof(unsigned long long) * 2 * kRows * (size_t)n));
+    DEM_HIP(hipMalloc(&c->rs0, sizeof(unsigned) * kRows * (size_t)n));
     DEM_HIP(hipMalloc(&c->toff, sizeof(unsigned) * ((size_t)n + 1)));
     c->ctl_bytes = sizeof(DevCtl) + (size_t)ninst * (sizeof(unsigned long long) + sizeof(unsigned));
     DEM_HIP(hipMalloc(&c->ctl, c->ctl_bytes));
