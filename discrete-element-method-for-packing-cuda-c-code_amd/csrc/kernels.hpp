@@ -212,6 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
                                                     unsigned *wflag, unsigned *wlist,
                                                     DevCtl *ctl, unsigned *perm,
                                                     double4 *cpos, float4 *cp32, unsigned *cid,
+                                                    unsigned *oslot, unsigned n_old,
                                                     const unsigned *active)
 {
     if (!is_active(active)) return;
@@ -246,6 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     }
     if (s >= n) return;
     if (perm) perm[s] = o;   // recorded for the adjoint of the reorder
+    // the particle's slot in the old list (kEmpty for particles the old list did not hold),
+    // for the history remap of this rebuild
+    oslot[s] = id < n_old ? o : kEmpty;
     double *dst = reinterpret_cast<double *>(rec_new + s);
 #pragma unroll
     for (int k = 0; k < 6; k++) st2(dst + 2 * k, v[k].x, v[k].y);
@@ -614,7 +618,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
 }
 
 // Tangential history of each pair of a new list, one lane per pair: the history of the same
-// particle pair in the old list, searched under the members' old slots, else zero -- the
+// particle pair in the old list, searched under the members' old slots (oslot, which the
+// reorder of this rebuild records), else zero -- the
 // reference's pairlistD remap (particleSystem.cu:143-191; new particles have empty old
 // segments, dempacking.cu:204,521-522).  Consecutive lanes take consecutive pairs, mostly
 // of one first member and so of one old segment, and write whole history lines.  Each
@@ -622,8 +627,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
 // capacity), so a fixed grid serves a captured rebuild.  src (optional) records each pair's
 // old index for the adjoint of the remap.
 __global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl *ctl, unsigned cap,
-                                                  const unsigned *cid, Hist *hist,
-                                                  const unsigned *inv_old, const uint2 *ab_old,
+                                                  const unsigned *oslot, Hist *hist,
+                                                  bool have_old, const uint2 *ab_old,
                                                   const unsigned *off_old, const Hist *hist_old,
                                                   unsigned n_old, unsigned cap_old, unsigned *src,
                                                   const unsigned *active)
@@ -637,15 +642,12 @@ __global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl 
         const uint2 e = ab[p];
         Hist h = Hist{0.0, 0.0, 0.0, 0.0};
         unsigned q = kEmpty;
-        if (inv_old) {
-            const unsigned ida = cid[e.x], idb = cid[e.y];
-            if (ida < n_old && idb < n_old) {
-                const unsigned oa = inv_old[ida], ob = inv_old[idb];
-                if (oa < n_old && ob != kEmpty) {
-                    const unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
-                    q = find_second(ab_old, lo, hi, ob);
-                    if (q != kEmpty) h = hist_old[q];
-                }
+        if (have_old) {
+            const unsigned oa = oslot[e.x], ob = oslot[e.y];
+            if (oa < n_old && ob != kEmpty) {
+                const unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
+                q = find_second(ab_old, lo, hi, ob);
+                if (q != kEmpty) h = hist_old[q];
             }
         }
         hist[p] = h;

@@ -202,6 +202,7 @@ struct dem_ctx {
     unsigned *ng = nullptr, *nl = nullptr, *toff = nullptr;
     unsigned long long *rmask = nullptr;   // the count's row masks, 2 kRows words per slot
     unsigned *rs0 = nullptr;               // the count's row starts, kRows words per slot
+    unsigned *oslot = nullptr;             // per new slot, the particle's old-list slot
     unsigned *off[2] = {nullptr, nullptr};
     uint2 *ab[2] = {nullptr, nullptr};
     unsigned cap[2] = {0, 0};
@@ -387,17 +388,20 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
                        c->cstart, c->cend, active);
     stage_event(ev, kEvCells, st);
     int nr = 1 - c->rc;
+    const unsigned n_old = c->have_list ? c->n_list : 0u;   // particles the old list holds
     if (c->batched)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
                            c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
-                           c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, active);
+                           c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid,
+                           c->oslot, n_old, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
                            (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
                            c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
-                           c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, active);
+                           c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, c->oslot, n_old,
+                           active);
     c->rc = nr;
     c->pc = 1 - c->pc;
     stage_event(ev, kEvReorder, st);
@@ -433,7 +437,6 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     InstDev one = c->hinst[0];
     int nl = 1 - c->lc;
     int ol = c->lc;
-    const unsigned *inv_old = c->have_list ? c->inv[1 - c->pc] : nullptr;
     int nh = 1 - c->hc, oh = c->hc;
     unsigned capn = std::min(std::min(c->cap[nl], c->out_cap), c->hcap[nh]);
     unsigned capo = std::min(std::min(c->cap[ol], c->out_cap), c->hcap[oh]);
@@ -453,8 +456,8 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     // histories by pair identity, one lane per pair over the device pair count
     const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
     hipLaunchKernelGGL(k_remap, dim3(rg), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
-                       (const DevCtl *)c->ctl, capn, (const unsigned *)c->cid, c->hist[nh], inv_old,
-                       (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
+                       (const DevCtl *)c->ctl, capn, (const unsigned *)c->oslot, c->hist[nh],
+                       c->have_list, (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
                        (const Hist *)c->hist[oh], c->n_list, capo, c->tape_src, active);
     stage_event(ev, kEvRemap, st);
     hipLaunchKernelGGL(k_transpose, dim3(rg), dim3(kBlock), 0, st, (const unsigned *)c->bo,
@@ -702,7 +705,7 @@ void free_ctx(dem_ctx *c)
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
                     c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
-                    c->nl, c->rmask, c->rs0, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
+                    c->nl, c->rmask, c->rs0, c->oslot, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->jf, c->bo, c->zs, c->ctl};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
@@ -968,6 +971,7 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->rmask, sizeof(unsigned long long) * 2 * kRows * (size_t)n));
     DEM_HIP(hipMalloc(&c->rs0, sizeof(unsigned) * kRows * (size_t)n));
+    DEM_HIP(hipMalloc(&c->oslot, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->toff, sizeof(unsigned) * ((size_t)n + 1)));
     c->ctl_bytes = sizeof(DevCtl) + (size_t)ninst * (sizeof(unsigned long long) + sizeof(unsigned));
     DEM_HIP(hipMalloc(&c->ctl, c->ctl_bytes));
