@@ -10,8 +10,12 @@
  * fixed-width neighbour rows Anei/Aref, inclusive scan sjgi, Lp/Ln pair list).  Each
  * function cites the reference file:line it follows.  Routines whose source is absent
  * from the reference (neighborarrayD, pairlistD, calforceij, sumallforceiD, calforceiw,
- * outputporositycnD, the SDK radix sort and scan) follow the written spec in DESIGN.md;
- * what pins each of them is listed there.  Parity unpinned: calforceij / calforceiw
+ * outputporositycnD, the SDK radix sort and scan) follow one written spec, stated at each
+ * function below and shared with the HIP kernels: candidates are the 27-cell neighbours
+ * within ri + rj + skin, ordered by stencil row and slot; a pair's first member has the lower
+ * index; Aref rows hold own pairs then partner pairs; histories follow the (i, j) identity;
+ * the law is the Hertz-Mindlin model of the contact-law comment.  The checks that pin each of
+ * them are the tests/test_oracle_*.py files.  Parity unpinned: calforceij / calforceiw
  * constants and sign conventions beyond the closed-form cases (reference source absent).
  *
  * Build: see oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off).  Contraction is off so
