@@ -186,15 +186,23 @@ def new_context(dem, ws, split_opt):
     return ctx
 
 
+# Steps the headline context runs on the loaded bed before its warm-up: the bed is handed over
+# as positions, displacements and spins, so the contact histories (and the zero-history bits)
+# start empty and the first steps after loading still rebuild them -- part of preparing the
+# poured state, like the pour itself, not of the timed workload.
+BED_RESUME_STEPS = 200
+
+
 def bench_headline(dem, ws, steps, warmup, split_opt):
-    """`steps` timed steps of the poured bed after `warmup` (displacement trigger: the first
-    warm-up step builds the lists), wall clock around device work only; then the same steps
-    again from the same state with HIP events around every step's kernels (the dominant
-    kernel's duration; the events cost a few percent, so `value` is from the clean pass;
-    the trajectory is the same bit for bit); then one rebuild timed stage by stage."""
+    """`steps` timed steps of the poured bed after BED_RESUME_STEPS steps that rebuild its
+    contact histories and `warmup` more (displacement trigger: the first step builds the
+    lists), wall clock around device work only; then the same steps again from the same
+    state with HIP events around every step's kernels (the dominant kernel's duration; the
+    events cost a few percent, so `value` is from the clean pass; the trajectory is the same
+    bit for bit); then one rebuild timed stage by stage."""
     N = ws.n
     ctx = new_context(dem, ws, split_opt)
-    ctx.step(warmup)
+    ctx.step(BED_RESUME_STEPS + warmup)
     ctx.sync()
     s0 = ctx.stats()
     t0 = time.perf_counter()
@@ -208,7 +216,7 @@ def bench_headline(dem, ws, steps, warmup, split_opt):
         f"path {'fused' if s1.step_path else 'split'}")
     ctx.close()
     ctx = new_context(dem, ws, split_opt)
-    ctx.step(warmup)
+    ctx.step(BED_RESUME_STEPS + warmup)
     ctx.set_option(dem.OPT_TIME_STEPS, 1)
     ctx.step_timing()
     ctx.step(steps)
@@ -819,7 +827,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "plain-500k-lognormal-cylinder-poured-bed", "n": N,
-                   "settle_steps": ws.meta["settle_steps"], "npairs": head["npairs"],
+                   "settle_steps": ws.meta["settle_steps"], "resume_steps": BED_RESUME_STEPS,
+                   "npairs": head["npairs"],
                    "pairs_per_particle": round(head["npairs"] / N, 3),
                    "contacts_per_particle": round(head["contacts"] / N, 3),
                    "rebuild_policy": "displacement trigger |q| > 0.275",
