@@ -319,16 +319,23 @@ def test_queued_steps_equal_synchronous(dem, gpu_ok):
         assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
 
 
-def test_batched_equals_plain(dem, gpu_ok):
+@pytest.mark.parametrize("flen", [0, 1])
+def test_batched_equals_plain(dem, gpu_ok, flen):
+    """Three instances of uneven size and material in one batched context equal three plain
+    contexts bit for bit (with the FP64 length and with the float-length option)."""
     ws = [dense_workload(dem, n=n, seed=40 + i, mu=mu, e=e, box=(bx, bx, 60.0))
           for i, (n, mu, e, bx) in enumerate([(1500, 0.2, 0.4, 12.0), (3100, 0.5, 0.8, 16.0),
                                                (700, 0.3, 0.6, 9.0)])]
+    if flen:
+        for w in ws:
+            w.pdis *= 100.0   # sliding fast enough for vprojec's rescale
     K = 10
     insts = [w.instances[0] for w in ws]
     big = dem.Context(insts)
     cat = lambda k: np.concatenate([getattr(w, k) for w in ws])
     big.set_state(cat("pos"), cat("pdis"), cat("angv"), cat("rad"), cat("rmass"), cat("inert"))
     big.set_option(dem.OPT_REBUILD_INTERVAL, K)
+    big.set_option(dem.OPT_FLOAT_LENGTH, flen)
     big.step(40)
     sb = big.state()
     off = 0
@@ -336,6 +343,7 @@ def test_batched_equals_plain(dem, gpu_ok):
         c = dem.Context(w.instances)
         c.load(w)
         c.set_option(dem.OPT_REBUILD_INTERVAL, K)
+        c.set_option(dem.OPT_FLOAT_LENGTH, flen)
         c.step(40)
         s = c.state()
         for k in ("pos", "pdis", "angv"):
