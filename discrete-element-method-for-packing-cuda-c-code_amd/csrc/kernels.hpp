@@ -1,11 +1,14 @@
 // kernels.hpp -- gfx950 kernels of the DEM step.
 //
 // Context path (cell-sorted slots, CSR pair lists):
-//   rebuild : k_hash -> radix sort -> k_fill / k_cell_range -> k_reorder -> k_nbr_count ->
-//             scans -> k_nbr_fill (pair list + history remap) -> k_transpose
-//   step    : k_pair_force (one lane per candidate pair), k_walls (one lane per particle on
-//             the rebuild's wall-candidate list) -> k_particle (per-particle gather of its
-//             pair forces in the reference order, gravity, wall force, integration, trigger)
+//   rebuild : k_hash -> radix sort -> k_fill / k_cell_range -> k_reorder -> k_nbr_count
+//             (candidate search, row records) -> scans -> k_nbr_fill (pair list) -> k_remap
+//             (tangential histories by pair identity) -> k_transpose
+//   step    : fused (default) k_step -- one lane per particle: wall contact, contact filter,
+//             the laws of its contacts balanced over the wave, the sums in the reference
+//             order, gravity, integration, trigger; or split: k_pair_force (one lane per
+//             candidate pair), k_walls (one lane per wall candidate) -> k_particle (gather
+//             of the pair forces in the reference order, then as k_step)
 // Reference-layout path (drop-in entry points, particle-index order, AoS double3):
 //   c_* kernels with the reference's buffers and semantics.
 #pragma once
@@ -281,42 +284,6 @@ __device__ __forceinline__ bool row_range(const unsigned *cstart, const unsigned
     return s0 != kEmpty;
 }
 
-// Visit every candidate partner b of slot a in ascending slot order (27-cell stencil,
-// z-, y-, x-offset ascending).  f(b) is called for b != a passing the cutoff test.
-template <class F>
-__device__ __forceinline__ void for_candidates(const double4 *cpos, unsigned a, unsigned key,
-                                               const InstDev &I, const unsigned *cstart,
-                                               const unsigned *cend, F &&f)
-{
-    const double4 ra = cpos[a];
-    d3 xa = d3{ra.x, ra.y, ra.z};
-    double rad_a = ra.w;
-    unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2];
-    unsigned local = key - I.cell_off;
-    unsigned gxy = gx * gy;
-    int cz = (int)(local / gxy);
-    int cy = (int)((local - (unsigned)cz * gxy) / gx);
-    int cx = (int)(local - (unsigned)cz * gxy - (unsigned)cy * gx);
-    int x0 = cx > 0 ? cx - 1 : 0;
-    int x1 = cx + 1 < (int)gx ? cx + 1 : (int)gx - 1;
-    for (int dz = -1; dz <= 1; dz++) {
-        int z = cz + dz;
-        if (z < 0 || z >= (int)gz) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int y = cy + dy;
-            if (y < 0 || y >= (int)gy) continue;
-            unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
-            unsigned s0, s1;
-            if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) continue;
-            for (unsigned b = s0; b < s1; b++) {
-                if (b == a) continue;
-                const double4 rb = cpos[b];
-                if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, rad_a, rb.w, I.skin)) f(b);
-            }
-        }
-    }
-}
-
 // Block-wide minimum of lo and maximum of hi over the lanes (every lane of the block calls
 // it; red holds two words per wave).
 __device__ __forceinline__ void block_minmax(unsigned lo, unsigned hi, unsigned *red,
@@ -348,12 +315,13 @@ __device__ __forceinline__ void block_minmax(unsigned lo, unsigned hi, unsigned 
 constexpr unsigned kTile = 512;
 constexpr unsigned kTileMax = 4 * kTile;
 
-// for_candidates for the consecutive slots of a block, every lane of which calls it (live:
-// the lane has a slot a < n).  For each stencil row (z-, y-offset ascending) the lanes' slot
-// ranges -- contiguous, and overlapping for nearby cells -- are united, the union's positions
-// are staged in LDS (tile: kTile entries) by coalesced loads, and each lane tests its own
-// range from there in ascending slot order: the same candidates in the same order as
-// for_candidates, with each position read from HBM once per block instead of once per lane.
+// Candidate search (neighborarrayD's role) for the consecutive slots of a block, every lane
+// of which calls it (live: the lane has a slot a < n): every b != a in the 27 cells around
+// a's cell with is_candidate true, in stencil-row order (z-, y-offset ascending) and
+// ascending slot order within a row -- the list order.  For each row the lanes' slot ranges
+// -- contiguous, and overlapping for nearby cells -- are united, the union's positions are
+// staged in LDS (tile: kTile entries) by coalesced loads, and each lane tests its own range
+// from there, so each position is read from HBM once per block instead of once per lane.
 // f(b, s0, s1, id) gets each candidate b with the lane's range [s0, s1) of its row and b's
 // particle index (cid, staged with the positions); row_done(r, s0, s1) follows each row
 // r = 0..8 on every lane (s0 = kEmpty, s1 = 0: no range).
