@@ -1745,23 +1745,41 @@ __global__ __launch_bounds__(kBlock) void c_sum_forces(int n, int width, const u
     if (i >= n) return;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
     unsigned c = 0;
-    unsigned g = njgi[i];
-    unsigned cnt = g + njli[i];
-    for (unsigned m = 0; m < cnt && m < (unsigned)width; m++) {
-        unsigned p = aref[(size_t)i * width + m];
-        if (p == kEmpty) continue;
-        if (m < g) {
-            f0 += forcepair[3 * (size_t)p]; f1 += forcepair[3 * (size_t)p + 1];
-            f2 += forcepair[3 * (size_t)p + 2];
-            t0 += torqpairi[3 * (size_t)p]; t1 += torqpairi[3 * (size_t)p + 1];
-            t2 += torqpairi[3 * (size_t)p + 2];
-        } else {
-            f0 -= forcepair[3 * (size_t)p]; f1 -= forcepair[3 * (size_t)p + 1];
-            f2 -= forcepair[3 * (size_t)p + 2];
-            t0 += torqpairj[3 * (size_t)p]; t1 += torqpairj[3 * (size_t)p + 1];
-            t2 += torqpairj[3 * (size_t)p + 2];
+    const unsigned g = njgi[i];
+    const unsigned cnt = min(g + njli[i], (unsigned)width);
+    // kSumAhead entries at a time: every load of a group in flight before the group's sums,
+    // which then run in row order (x - y == x + (-y) bit for bit; an empty entry adds
+    // nothing)
+    constexpr unsigned kSumAhead = 4;
+    for (unsigned m0 = 0; m0 < cnt; m0 += kSumAhead) {
+        unsigned p[kSumAhead];
+        d3 fp[kSumAhead], tp[kSumAhead];
+        unsigned cp[kSumAhead];
+#pragma unroll
+        for (unsigned k = 0; k < kSumAhead; k++)
+            p[k] = m0 + k < cnt ? aref[(size_t)i * width + m0 + k] : kEmpty;
+#pragma unroll
+        for (unsigned k = 0; k < kSumAhead; k++) {
+            fp[k] = tp[k] = d3{0.0, 0.0, 0.0};
+            cp[k] = 0u;
+            if (p[k] != kEmpty) {
+                fp[k] = ld3(forcepair, p[k]);
+                tp[k] = ld3(m0 + k < g ? torqpairi : torqpairj, p[k]);
+                cp[k] = contactpair[p[k]];
+            }
         }
-        c += contactpair[p];
+#pragma unroll
+        for (unsigned k = 0; k < kSumAhead; k++) {
+            if (p[k] == kEmpty) continue;
+            const bool own = m0 + k < g;
+            f0 += own ? fp[k].x : -fp[k].x;
+            f1 += own ? fp[k].y : -fp[k].y;
+            f2 += own ? fp[k].z : -fp[k].z;
+            t0 += tp[k].x;
+            t1 += tp[k].y;
+            t2 += tp[k].z;
+            c += cp[k];
+        }
     }
     st3(forcei, i, d3{f0, f1, f2});
     st3(torquei, i, d3{t0, t1, t2});
@@ -1801,8 +1819,9 @@ __global__ __launch_bounds__(kBlock) void c_wall_gravity(int n, WallGeom G, cons
     st3(torquei, i, t);
 }
 
-// updatepositionD (neiglist.cu:308-409); block maxima go to partials, reduced on the device
-// into out2 = (iall, maxhz bits) by the caller's second kernel.
+// updatepositionD (neiglist.cu:308-409); the block's maximum height (as the bits of a
+// non-negative double, whose order is the values' order) and its rebuild flag go to maxz and
+// iall by one atomic each.
 __global__ __launch_bounds__(kBlock) void c_update(int n, double *pos, double *pdis, double *angv,
                                                    const double *rmass, const double *inert,
                                                    const double *forcei, const double *torquei,
@@ -1839,8 +1858,22 @@ __global__ __launch_bounds__(kBlock) void c_update(int n, double *pos, double *p
         zb = oz > zb ? oz : zb;
         flag |= __shfl_xor(flag, d, 64);
     }
+    // one atomic per block, and none when the block cannot raise the maximum (the maximum
+    // only grows, so a stale read only lets an atomic through)
+    __shared__ unsigned long long bz[kBlock / 64];
+    __shared__ unsigned bf[kBlock / 64];
+    const unsigned w = threadIdx.x >> 6;
     if (__lane_id() == 0) {
-        if (zb) atomicMax(maxz, zb);
+        bz[w] = zb;
+        bf[w] = flag;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (unsigned k = 1; k < blockDim.x / 64u; k++) {
+            zb = bz[k] > zb ? bz[k] : zb;
+            flag |= bf[k];
+        }
+        if (zb && zb > *reinterpret_cast<volatile unsigned long long *>(maxz)) atomicMax(maxz, zb);
         if (flag) atomicOr(iall, 1u);
     }
 }
