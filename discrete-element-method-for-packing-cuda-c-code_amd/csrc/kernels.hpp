@@ -1217,20 +1217,27 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
             double2 a0 = ld2(RA), a1 = ld2(RA + 2);
             const float4 xa = pos32(a0.x, a0.y, a1.x, a1.y);
             const unsigned m = no > nt ? no : nt;
-            for (unsigned i = 0; i < m; i += kLook) {
-                unsigned b[kLook], f[kLook];
-                bool vt[kLook];
-                float4 cb[kLook], cf[kLook];
+            // software-pipelined: the partner slots of group i + kLook are loaded while the
+            // positions of group i are in flight, so a group waits for one load, not two
+            // (the pair index jt is read by the law)
+            unsigned b[kLook], f[kLook];
 #pragma unroll
-                for (unsigned k = 0; k < kLook; k++) {
-                    b[k] = i + k < no ? bo[p0 + i + k] : a;
-                    vt[k] = i + k < nt;
-                    f[k] = vt[k] ? jf[q0 + i + k] : a;   // (the pair index jt is read by the law)
-                }
+            for (unsigned k = 0; k < kLook; k++) {
+                b[k] = k < no ? bo[p0 + k] : a;
+                f[k] = k < nt ? jf[q0 + k] : a;
+            }
+            for (unsigned i = 0; i < m; i += kLook) {
+                float4 cb[kLook], cf[kLook];
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
                     cb[k] = cin[b[k]];
                     cf[k] = cin[f[k]];
+                }
+                const unsigned i2 = i + kLook;
+#pragma unroll
+                for (unsigned k = 0; k < kLook; k++) {
+                    b[k] = i2 + k < no ? bo[p0 + i2 + k] : a;
+                    f[k] = i2 + k < nt ? jf[q0 + i2 + k] : a;
                 }
 #pragma unroll
                 for (unsigned k = 0; k < kLook; k++) {
@@ -1239,7 +1246,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                         if (con) mo |= 1ull << (i + k);
                         own_outcome(coop, cbits, p0 + i + k - pb0, zs, hout, p0 + i + k, con, zmode);
                     }
-                    if (vt[k] && maybe_touch(cf[k], xa)) mt |= 1ull << (i + k);
+                    if (i + k < nt && maybe_touch(cf[k], xa)) mt |= 1ull << (i + k);
                 }
             }
         }
