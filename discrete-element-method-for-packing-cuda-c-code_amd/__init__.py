@@ -684,11 +684,11 @@ class Context:
 
     def time_rebuild(self) -> dict:
         """One list rebuild now, timed stage by stage (ms), with the sizes it ran at."""
-        o = np.zeros(14)
+        o = np.zeros(15)
         _check(self._L.dem_time_rebuild(self._h, _ptr(o)), "dem_time_rebuild")
         out = {k: float(o[i]) for i, k in enumerate(self.REBUILD_STAGES)}
         out.update(passes=int(o[9]), npairs=int(o[10]), npairs_old=int(o[11]), ncells=int(o[12]),
-                   n=int(o[13]))
+                   n=int(o[13]), phases_ms=float(o[14]))
         return out
 
     def sync(self):

@@ -1,7 +1,7 @@
 // kernels.hpp -- gfx950 kernels of the DEM step.
 //
 // Context path (cell-sorted slots, CSR pair lists):
-//   rebuild : k_hash -> radix sort -> k_fill / k_cell_range -> k_reorder -> k_nbr_count
+//   rebuild : k_hash (cell counts) -> scan -> k_sort_scatter -> k_reorder -> k_nbr_count
 //             (candidate search, row records) -> scans -> k_nbr_fill (pair list) -> k_remap
 //             (tangential histories by pair identity) -> k_transpose
 //   step    : fused (default) k_step -- one lane per particle: wall contact, contact filter,
@@ -139,12 +139,22 @@ __device__ __forceinline__ unsigned xcd_chunk_block(unsigned b, unsigned nb, uns
 // Rebuild
 // ----------------------------------------------------------------------------------------
 
-// calcHashD semantics (neiglist.cu:77-97) on slot-ordered records: the (cell, index) pair
-// of particle `id` lands at kv[id], i.e. in particle-index order, exactly the array the
-// reference sorts.  Cells are offset per instance so one sort serves every instance.
+// The list rebuild sorts the (cell, index) pairs of calcHashD (neiglist.cu:77-97) by cell,
+// equal cells in ascending particle index -- the result of the reference's stable radix sort
+// of the index-ordered hash array (dempacking.cu:553-555) -- as a counting sort over the
+// cells: k_hash counts each cell's particles (one atomic per particle, which also gives the
+// particle an arrival rank in its cell), an exclusive scan of the counts gives each cell's
+// first slot (cell c holds slots [cstart[c], cstart[c + 1]); cstart has ncells + 1 entries),
+// k_sort_scatter places each pair at its cell's first slot plus its rank, and k_reorder
+// restores the index order inside each cell while it moves the records.  Cells are offset per
+// instance so one sort serves every instance.
+
+// calcHashD on slot-ordered records: kv[s] = (cell, index) of the particle in slot s, its
+// cell's count incremented (cnt zeroed beforehand) and its arrival rank in rank[s].
 template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, const InstDev *inst,
-                                                 InstDev one, uint2 *kv, DevCtl *ctl,
+                                                 InstDev one, uint2 *kv, unsigned *cnt,
+                                                 unsigned *rank, DevCtl *ctl,
                                                  const unsigned *active)
 {
     if (!is_active(active)) return;
@@ -164,8 +174,38 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
     unsigned x = cell_coord(px, I.origin[0], I.cell[0], I.gsize[0]);
     unsigned y = cell_coord(py, I.origin[1], I.cell[1], I.gsize[1]);
     unsigned z = cell_coord(pz, I.origin[2], I.cell[2], I.gsize[2]);
-    unsigned h = z * I.gsize[1] * I.gsize[0] + y * I.gsize[0] + x;
-    kv[id] = make_uint2(h + I.cell_off, id);
+    unsigned h = z * I.gsize[1] * I.gsize[0] + y * I.gsize[0] + x + I.cell_off;
+    kv[s] = make_uint2(h, id);
+    // Slots are in the last rebuild's cell order and few particles change cell, so a wave's
+    // lanes form runs of equal cells: the first lane of each run takes the run's ranks with
+    // one atomic and hands them on (a cell split over several runs takes one atomic per run).
+    // The live lanes (s < n) are a prefix of the wave.
+    const unsigned lane = __lane_id();
+    const unsigned hp = (unsigned)__shfl_up((int)h, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0u || hp != h);
+    const unsigned long long live = __ballot(1);
+    const unsigned long long le = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
+    const unsigned hl = 63u - (unsigned)__builtin_clzll(heads & le);   // this lane's run head
+    const unsigned long long after = heads & ~le;                      // heads past this lane
+    const unsigned end = after ? (unsigned)__builtin_ctzll(after)
+                               : 64u - (unsigned)__builtin_clzll(live);   // one past the run
+    unsigned base = 0;
+    if (lane == hl) base = atomicAdd(&cnt[h], end - hl);
+    base = (unsigned)__shfl((int)base, (int)hl, 64);
+    rank[s] = base + (lane - hl);
+}
+
+// Counting-sort scatter: the pair of slot s goes to its cell's first slot plus its arrival
+// rank (a permutation of [0, n): ranks are 0 .. count - 1 within each cell).
+__global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint2 *kv, const unsigned *rank,
+                                                         const unsigned *cstart, unsigned n,
+                                                         uint2 *out, const unsigned *active)
+{
+    if (!is_active(active)) return;
+    const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint2 e = kv[s];
+    out[cstart[e.x] + rank[s]] = e;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(unsigned *a, unsigned n, unsigned v,
@@ -204,10 +244,14 @@ __device__ __forceinline__ double wall_reach(const WallGeom &g, double x, double
     return (r + slack) - d;
 }
 
-// Move every particle into its sorted slot; q restarts at zero (dempacking.cu:541).  Also
-// lists the particles that may touch a wall before the next rebuild (for k_walls).
+// Move every particle into its sorted slot; q restarts at zero (dempacking.cu:541).  The
+// pairs come from k_sort_scatter in cell order, in arrival order inside a cell: the lane of
+// position s moves the particle it finds there to slot s' = its cell's first slot plus the
+// number of particles of that cell with a smaller index, and writes the pair to sorted[s'].
+// Also lists the particles that may touch a wall before the next rebuild (for k_walls).
 template <bool BATCHED>
-__global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigned n,
+__global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *scat, const unsigned *cstart,
+                                                    uint2 *sorted, unsigned n,
                                                     const PRec *rec_old, PRec *rec_new,
                                                     const Hist *wh_old, Hist *wh_new,
                                                     const unsigned *inv_old, unsigned *inv_new,
@@ -222,17 +266,26 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     unsigned s = blockIdx.x * kBlock + threadIdx.x;
     bool cand = false;
     unsigned id = 0, o = 0;
+    uint2 e = make_uint2(0u, 0u);
     double2 v[8];   // the record, eight 16-byte chunks
     Hist w = Hist{0.0, 0.0, 0.0, 0.0};
     if (s < n) {
-        id = sorted[s].y;
+        e = scat[s];
+        id = e.y;
         o = inv_old[id];
+        const unsigned c0 = cstart[e.x], c1 = cstart[e.x + 1];
         // every load of the record in flight before any store (the stores cannot be moved
         // above loads of a buffer the compiler cannot tell apart from theirs)
         const double *src = reinterpret_cast<const double *>(rec_old + o);
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = ld2(src + 2 * k);
         w = wh_old[o];
+        // stable position inside the cell while the record is in flight (a cell holds few
+        // particles: its edge is the largest diameter plus the skin)
+        unsigned r = 0;
+        for (unsigned k = c0; k < c1; k++) r += scat[k].y < id ? 1u : 0u;
+        s = c0 + r;
+        sorted[s] = e;
         const unsigned ri = (unsigned)(unsigned long long)__double_as_longlong(v[7].y);   // inst
         const InstDev &I = BATCHED ? inst[ri] : one;
         cand = wall_reach(I.walls, v[0].x, v[0].y, v[1].x, v[1].y, 0.5 * I.skin) > 0.0;
@@ -267,21 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *sorted, unsigne
     cid[s] = (unsigned)((unsigned long long)__double_as_longlong(v[7].y) >> 32);
 }
 
-// Slot range [s0, s1) of the up-to-three consecutive cells of one stencil row.  Cells of a
-// row are adjacent in key order and slots are sorted by key, so their particles are
-// contiguous.  Returns false if the row is empty.
-__device__ __forceinline__ bool row_range(const unsigned *cstart, const unsigned *cend,
-                                          unsigned c0, unsigned c1, unsigned &s0, unsigned &s1)
+// Slot range [s0, s1) of the consecutive cells c0 .. c1 of one stencil row.  Cells of a row
+// are adjacent in key order and slots are sorted by key, so their particles are the slots
+// [cstart[c0], cstart[c1 + 1]).  Returns false if the row is empty.
+__device__ __forceinline__ bool row_range(const unsigned *cstart, unsigned c0, unsigned c1,
+                                          unsigned &s0, unsigned &s1)
 {
-    s0 = kEmpty;
-    s1 = 0;
-    for (unsigned c = c0; c <= c1; c++) {
-        unsigned a = cstart[c];
-        if (a == kEmpty) continue;
-        if (s0 == kEmpty) s0 = a;
-        s1 = cend[c];
-    }
-    return s0 != kEmpty;
+    s0 = cstart[c0];
+    s1 = cstart[c1 + 1u];
+    return s0 < s1;
 }
 
 // Block-wide minimum of lo and maximum of hi over the lanes (every lane of the block calls
@@ -329,7 +376,7 @@ template <bool BATCHED, class F, class R>
 __device__ __forceinline__ void for_candidates_block(const double4 *cpos, const unsigned *cid,
                                                      unsigned a, bool live, unsigned key,
                                                      const InstDev &I, const unsigned *cstart,
-                                                     const unsigned *cend, double4 *tile,
+                                                     double4 *tile,
                                                      unsigned *tcid, unsigned *red, F &&f,
                                                      R &&row_done)
 {
@@ -354,7 +401,7 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, const 
             const int z = cz + dz, y = cy + dy;
             if (live && z >= 0 && z < (int)gz && y >= 0 && y < (int)gy) {
                 const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
-                if (!row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) {
+                if (!row_range(cstart, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1)) {
                     s0 = kEmpty;
                     s1 = 0;
                 }
@@ -402,7 +449,7 @@ constexpr unsigned kLongRow = kEmpty - 1u;
 // Slot range [s0, s1) of stencil row r (z offset r / 3 - 1, y offset r % 3 - 1) of the cell
 // with key `key`, as for_candidates_block forms it; false if the row is empty or off the grid.
 __device__ __forceinline__ bool stencil_row(unsigned key, const InstDev &I, const unsigned *cstart,
-                                            const unsigned *cend, int r, unsigned &s0,
+                                            int r, unsigned &s0,
                                             unsigned &s1)
 {
     const unsigned gx = I.gsize[0], gy = I.gsize[1], gz = I.gsize[2], gxy = gx * gy;
@@ -415,7 +462,7 @@ __device__ __forceinline__ bool stencil_row(unsigned key, const InstDev &I, cons
     const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
     if (z < 0 || z >= (int)gz || y < 0 || y >= (int)gy) return false;
     const unsigned rowc = I.cell_off + (unsigned)z * gxy + (unsigned)y * gx;
-    return row_range(cstart, cend, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1);
+    return row_range(cstart, rowc + (unsigned)x0, rowc + (unsigned)x1, s0, s1);
 }
 
 template <bool BATCHED>
@@ -423,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
                                                       const unsigned *cid, const uint2 *sorted,
                                                       unsigned n, const InstDev *inst,
                                                       InstDev one, const unsigned *cstart,
-                                                      const unsigned *cend, unsigned *ng,
+                                                      unsigned *ng,
                                                       unsigned *nl, unsigned long long *rmask,
                                                       unsigned *rs0, const unsigned *active)
 {
@@ -438,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
     unsigned long long mg = 0ull, ml = 0ull;
     const unsigned ida = live ? cid[a] : 0u;
     for_candidates_block<BATCHED>(
-        cpos, cid, a, live, live ? sorted[a].x : 0u, I, cstart, cend, tile, tcid, red,
+        cpos, cid, a, live, live ? sorted[a].x : 0u, I, cstart, tile, tcid, red,
         [&](unsigned b, unsigned r0, unsigned r1, unsigned idb) {
             const unsigned long long bit = r1 - r0 <= kRowBits ? 1ull << (b - r0) : 0ull;
             if (idb > ida) {
@@ -494,7 +541,7 @@ template <bool BATCHED>
 __global__ __launch_bounds__(kBlock) void k_nbr_fill(
     const PRec *rec, const double4 *cpos, const unsigned *cid, const uint2 *sorted, unsigned n,
     const InstDev *inst, InstDev one,
-    const unsigned *cstart, const unsigned *cend, const unsigned *off, const unsigned *toff,
+    const unsigned *cstart, const unsigned *off, const unsigned *toff,
     unsigned cap, uint2 *ab, unsigned *bo, unsigned *jt, unsigned *jf, DevCtl *ctl,
     const unsigned long long *rmask, const unsigned *rs0, const unsigned *active)
 {
@@ -558,7 +605,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
                 }
             } else {
                 unsigned b0, b1;
-                stencil_row(sorted[a].x, I, cstart, cend, r, b0, b1);
+                stencil_row(sorted[a].x, I, cstart, r, b0, b1);
                 const double4 ra = cpos[a];
                 const d3 xa = d3{ra.x, ra.y, ra.z};
                 for (unsigned b = b0; b < b1; b++) {

@@ -48,13 +48,6 @@ int g_last_error = 0;
 
 inline unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
-int key_bits_for(unsigned ncells)
-{
-    int b = 1;
-    while (b < 32 && (1ull << b) < (unsigned long long)ncells) b++;
-    return b;
-}
-
 LawParams law_from(const Matproperties &m)
 {
     return LawParams{m.estarp, m.ufricp, m.vpois, m.dampnp, m.damptp, m.upp};
@@ -176,7 +169,6 @@ struct dem_ctx {
     InstDev *dinst = nullptr;
     unsigned n = 0, ncells = 0;   // n = active particles (<= cap_n, single instance may grow)
     unsigned cap_n = 0;           // allocated particle slots
-    int key_bits = 1;
     int cus = 256;
 
     PRec *rec[2] = {nullptr, nullptr};
@@ -197,8 +189,11 @@ struct dem_ctx {
     double *fdbg = nullptr;
     bool record_forces = false;
 
+    // rebuild sort: kv_hash (cell, index) by slot, kv_a the scatter, kv_sorted (= kv_b) the
+    // sorted pairs; cstart[c] = first slot of cell c (ncells + 1 entries, cstart[ncells] = n);
+    // ccount the per-cell counts
     uint2 *kv_hash = nullptr, *kv_a = nullptr, *kv_b = nullptr, *kv_sorted = nullptr;
-    unsigned *cstart = nullptr, *cend = nullptr;
+    unsigned *cstart = nullptr, *ccount = nullptr;
     unsigned *ng = nullptr, *nl = nullptr, *toff = nullptr;
     unsigned long long *rmask = nullptr;   // the count's row masks, 2 kRows words per slot
     unsigned *rs0 = nullptr;               // the count's row starts, kRows words per slot
@@ -361,7 +356,7 @@ enum { kEvStart, kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount, kEvScan, kEvF
        kEvFill, kEvRemap, kEvTranspose, kEvCountAll };
 inline void stage_event(hipEvent_t *ev, int k, hipStream_t st)
 {
-    if (ev) (void)hipEventRecord(ev[k], st);
+    if (ev && ev[k]) (void)hipEventRecord(ev[k], st);
 }
 
 // Phase A+B of a rebuild: hash, sort, cell ranges, reorder, neighbour counts and scans.
@@ -371,33 +366,37 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     unsigned n = c->n;
     unsigned gb = cdiv(n, kBlock);
     InstDev one = c->hinst[0];
+    // counting sort by cell (kernels.hpp): counts, their scan (the cells' first slots), the
+    // scatter by arrival rank; k_reorder then restores the index order inside each cell.
+    // The ranks live in ng until the neighbour count writes it.
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(c->ncells, kBlock), 4096u)), dim3(kBlock), 0, st,
+                       c->ccount, c->ncells, 0u, active);
     if (c->batched)
         hipLaunchKernelGGL(k_hash<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc], n, c->dinst,
-                           one, c->kv_hash, c->ctl, active);
+                           one, c->kv_hash, c->ccount, c->ng, c->ctl, active);
     else
         hipLaunchKernelGGL(k_hash<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc], n,
-                           c->dinst, one, c->kv_hash, c->ctl, active);
+                           c->dinst, one, c->kv_hash, c->ccount, c->ng, c->ctl, active);
     stage_event(ev, kEvHash, st);
-    int rc;
-    c->kv_sorted = radix_sort(c->S, c->kv_hash, c->kv_a, c->kv_b, n, c->key_bits, st, active, &rc);
-    if (rc) return rc;
-    stage_event(ev, kEvSort, st);
-    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(c->ncells, kBlock), 4096u)), dim3(kBlock), 0, st,
-                       c->cstart, c->ncells, kEmpty, active);
-    hipLaunchKernelGGL(k_cell_range, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted, n,
-                       c->cstart, c->cend, active);
-    stage_event(ev, kEvCells, st);
+    if (scan_excl(c->S, c->ccount, c->ncells, c->cstart, c->S.total + 2, st, active))
+        return DEM_ERR_HIP;
+    hipLaunchKernelGGL(k_sort_scatter, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_hash,
+                       (const unsigned *)c->ng, (const unsigned *)c->cstart, n, c->kv_a, active);
+    stage_event(ev, kEvSort, st);   // the cell ranges are the scan's output
+    c->kv_sorted = c->kv_b;
     int nr = 1 - c->rc;
     const unsigned n_old = c->have_list ? c->n_list : 0u;   // particles the old list holds
     if (c->batched)
-        hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_sorted,
+        hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_a,
+                           (const unsigned *)c->cstart, c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
                            c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
                            c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid,
                            c->oslot, n_old, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
-                           (const uint2 *)c->kv_sorted, n, c->rec[c->rc], c->rec[nr],
+                           (const uint2 *)c->kv_a, (const unsigned *)c->cstart, c->kv_sorted, n,
+                           c->rec[c->rc], c->rec[nr],
                            c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
                            c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, c->oslot, n_old,
@@ -408,12 +407,12 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_count<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart,
                            c->ng, c->nl, c->rmask, c->rs0, active);
     else
         hipLaunchKernelGGL(k_nbr_count<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart,
                            c->ng, c->nl, c->rmask, c->rs0, active);
     stage_event(ev, kEvCount, st);
     int nl = 1 - c->lc;
@@ -443,13 +442,13 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     if (c->batched)
         hipLaunchKernelGGL(k_nbr_fill<true>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart,
                            c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
                            (const unsigned long long *)c->rmask, (const unsigned *)c->rs0, active);
     else
         hipLaunchKernelGGL(k_nbr_fill<false>, dim3(gb), dim3(kBlock), 0, st, c->rec[c->rc],
                            (const double4 *)c->cpos[c->rc], (const unsigned *)c->cid,
-                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart, c->cend,
+                           (const uint2 *)c->kv_sorted, n, c->dinst, one, c->cstart,
                            c->off[nl], c->toff, capn, c->ab[nl], c->bo, c->jt, c->jf, c->ctl,
                            (const unsigned long long *)c->rmask, (const unsigned *)c->rs0, active);
     stage_event(ev, kEvFill, st);
@@ -685,12 +684,11 @@ int install_grid(dem_ctx *c, unsigned cell_off)
     DEM_HIP(hipMemcpy(c->dinst, c->hinst.data(), sizeof(InstDev) * c->ninst, hipMemcpyHostToDevice));
     if (cell_off != c->ncells) {
         DEM_HIP(hipFree(c->cstart));
-        DEM_HIP(hipFree(c->cend));
-        c->cstart = c->cend = nullptr;
-        DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * (size_t)cell_off));
-        DEM_HIP(hipMalloc(&c->cend, sizeof(unsigned) * (size_t)cell_off));
+        DEM_HIP(hipFree(c->ccount));
+        c->cstart = c->ccount = nullptr;
+        DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * ((size_t)cell_off + 1)));
+        DEM_HIP(hipMalloc(&c->ccount, sizeof(unsigned) * (size_t)cell_off));
         c->ncells = cell_off;
-        c->key_bits = key_bits_for(cell_off);
     }
     c->need_rebuild = true;
     c->buf_gen++;
@@ -704,7 +702,7 @@ void free_ctx(dem_ctx *c)
     if (!c) return;
     adj_free(c);
     void *bufs[] = {c->dinst, c->rec[0], c->rec[1], c->whist[0], c->whist[1], c->inv[0], c->inv[1],
-                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->cend, c->ng,
+                    c->contacti, c->wflag, c->wlist, c->wforce, c->fdbg, c->cpos[0], c->cpos[1], c->cp32[0], c->cp32[1], c->cid, c->kv_hash, c->kv_a, c->kv_b, c->cstart, c->ccount, c->ng,
                     c->nl, c->rmask, c->rs0, c->oslot, c->toff, c->off[0], c->off[1], c->ab[0], c->ab[1], c->hist[0],
                     c->hist[1], c->out, c->jt, c->jf, c->bo, c->zs, c->ctl};
     for (void *p : bufs)
@@ -929,7 +927,6 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     c->n = p_off;
     c->cap_n = p_off;
     c->ncells = cell_off;
-    c->key_bits = key_bits_for(c->ncells);
     unsigned n = c->cap_n;
     unsigned pc = pair_capacity ? pair_capacity : (unsigned)std::min<unsigned long long>(0xfffffff0ull, 12ull * n + 1024);
 
@@ -965,8 +962,8 @@ int dem_create(dem_ctx **out, int ninst, const dem_instance *inst, uint pair_cap
     DEM_HIP(hipMalloc(&c->kv_hash, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_a, sizeof(uint2) * (size_t)n));
     DEM_HIP(hipMalloc(&c->kv_b, sizeof(uint2) * (size_t)n));
-    DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * (size_t)c->ncells));
-    DEM_HIP(hipMalloc(&c->cend, sizeof(unsigned) * (size_t)c->ncells));
+    DEM_HIP(hipMalloc(&c->cstart, sizeof(unsigned) * ((size_t)c->ncells + 1)));
+    DEM_HIP(hipMalloc(&c->ccount, sizeof(unsigned) * (size_t)c->ncells));
     DEM_HIP(hipMalloc(&c->ng, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->nl, sizeof(unsigned) * (size_t)n));
     DEM_HIP(hipMalloc(&c->rmask, sizeof(unsigned long long) * 2 * kRows * (size_t)n));
@@ -1840,11 +1837,15 @@ int dem_time_kernels(dem_ctx *c, int nsteps, double *out_ms, int *nrebuild)
 }
 
 // One host-driven rebuild now, timed stage by stage with HIP events on the context stream:
-// out[0..8] = ms of hash, radix sort (all passes), cell ranges, reorder, neighbour count,
-// the two scans (+ capacity check), pair fill, history remap, transpose; out[9] = radix
-// passes; out[10] = pairs of the new list; out[11] = pairs of the old list; out[12] = cells;
-// out[13] = particles.  The trajectory is unchanged (a rebuild at any step gives the same
-// forces: lists hold every contact, sums follow particle order).
+// out[0..8] = ms of hash (cell counts), sort (count scan and scatter), cell ranges (0: the
+// scan gives them), reorder, neighbour count, the two scans (+ capacity check), pair fill,
+// history remap, transpose; out[9] = radix passes (0: counting sort); out[10] = pairs of the
+// new list; out[11] = pairs of the old list; out[12] = cells; out[13] = particles; out[14] =
+// ms of a second rebuild timed with events at its two phases' ends only (count phase from
+// the hash to the scans, fill phase from the fill to the transpose; the host's read of the
+// pair total between them excluded), free of the per-stage events' gaps.  The trajectory is
+// unchanged (a rebuild at any step gives the same forces: lists hold every contact, sums
+// follow particle order).
 int dem_time_rebuild(dem_ctx *c, double *out)
 {
     if (!c || !out) return DEM_ERR_INVALID;
@@ -1856,20 +1857,32 @@ int dem_time_rebuild(dem_ctx *c, double *out)
     rc = rebuild_host(c, nullptr, nullptr, ev);
     if (!rc) rc = hipEventSynchronize(ev[kEvTranspose]) == hipSuccess ? DEM_OK : DEM_ERR_HIP;
     if (!rc) {
-        static const int a[9] = {kEvStart, kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount,
+        static const int a[9] = {kEvStart, kEvHash, kEvSort, kEvSort, kEvReorder, kEvCount,
                                  kEvFillStart, kEvFill, kEvRemap};
-        static const int b[9] = {kEvHash, kEvSort, kEvCells, kEvReorder, kEvCount, kEvScan,
+        static const int b[9] = {kEvHash, kEvSort, kEvSort, kEvReorder, kEvCount, kEvScan,
                                  kEvFill, kEvRemap, kEvTranspose};
         for (int k = 0; k < 9; k++) {
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ev[a[k]], ev[b[k]]) != hipSuccess) rc = DEM_ERR_HIP;
+            if (a[k] != b[k] && hipEventElapsedTime(&ms, ev[a[k]], ev[b[k]]) != hipSuccess)
+                rc = DEM_ERR_HIP;
             out[k] = ms;
         }
-        out[9] = (c->key_bits + 7) / 8;
+        out[9] = 0;
         out[10] = c->npairs;
         out[11] = np_old;
         out[12] = c->ncells;
         out[13] = c->n;
+    }
+    if (!rc) {
+        hipEvent_t ph[kEvCountAll] = {};
+        for (int k : {kEvStart, kEvScan, kEvFillStart, kEvTranspose}) ph[k] = ev[k];
+        rc = rebuild_host(c, nullptr, nullptr, ph);
+        if (!rc) rc = hipEventSynchronize(ev[kEvTranspose]) == hipSuccess ? DEM_OK : DEM_ERR_HIP;
+        float m0 = 0.0f, m1 = 0.0f;
+        if (!rc && (hipEventElapsedTime(&m0, ev[kEvStart], ev[kEvScan]) != hipSuccess ||
+                    hipEventElapsedTime(&m1, ev[kEvFillStart], ev[kEvTranspose]) != hipSuccess))
+            rc = DEM_ERR_HIP;
+        out[14] = (double)m0 + (double)m1;
     }
     for (int k = 0; k < kEvCountAll; k++) (void)hipEventDestroy(ev[k]);
     return rc;
@@ -2106,9 +2119,16 @@ double dem_last_ms(dem_ctx *c) { return c ? (double)c->last_ms : -1.0; }
 
 int dem_get_hash(dem_ctx *c, uint *kv)
 {
-    if (!c || !kv) return DEM_ERR_INVALID;
+    if (!c || !kv || !c->kv_sorted) return DEM_ERR_INVALID;
     DEM_HIP(hipStreamSynchronize(c->stream));
-    DEM_HIP(hipMemcpy(kv, c->kv_hash, sizeof(uint2) * (size_t)c->n, hipMemcpyDeviceToHost));
+    // the device holds the pairs by slot; the reference's array is in particle-index order
+    std::vector<uint2> h(c->n);
+    DEM_HIP(hipMemcpy(h.data(), c->kv_hash, sizeof(uint2) * (size_t)c->n, hipMemcpyDeviceToHost));
+    for (const uint2 &e : h) {
+        if (e.y >= c->n) return DEM_ERR_INVALID;
+        kv[2 * (size_t)e.y] = e.x;
+        kv[2 * (size_t)e.y + 1] = e.y;
+    }
     return DEM_OK;
 }
 
@@ -2122,9 +2142,13 @@ int dem_get_sorted(dem_ctx *c, uint *kv)
 
 int dem_get_cell_start(dem_ctx *c, uint *cs, uint ncells)
 {
-    if (!c || !cs || ncells < c->ncells) return DEM_ERR_INVALID;
+    if (!c || !cs || ncells < c->ncells || !c->kv_sorted) return DEM_ERR_INVALID;
     DEM_HIP(hipStreamSynchronize(c->stream));
-    DEM_HIP(hipMemcpy(cs, c->cstart, sizeof(unsigned) * (size_t)c->ncells, hipMemcpyDeviceToHost));
+    // the device holds each cell's first slot for every cell (ncells + 1 entries); the
+    // reference's cellStart marks empty cells 0xffffffff
+    std::vector<unsigned> pre((size_t)c->ncells + 1);
+    DEM_HIP(hipMemcpy(pre.data(), c->cstart, sizeof(unsigned) * pre.size(), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < c->ncells; k++) cs[k] = pre[k] < pre[k + 1] ? pre[k] : kEmpty;
     return DEM_OK;
 }
 

@@ -328,10 +328,13 @@ int dem_time_kernels(dem_ctx *ctx, int nsteps, double *out_ms, int *nrebuild);
  * steps, sum of pair counts; fused path: walls total, step kernel total, steps, sum of pair
  * counts; rebuild total, rebuilds. */
 int dem_get_step_timing(dem_ctx *ctx, double *out);
-/* One list rebuild now, timed stage by stage with HIP events: out[0..8] = ms of hash, radix
- * sort, cell ranges, reorder, neighbour count, scans, pair fill, history remap, transpose;
- * out[9] = radix passes, out[10] = new pairs, out[11] = old pairs, out[12] = cells,
- * out[13] = particles (out holds 14 doubles).  The trajectory is unchanged. */
+/* One list rebuild now, timed stage by stage with HIP events: out[0..8] = ms of hash (cell
+ * counts), sort (count scan and scatter), cell ranges (0: given by the scan), reorder,
+ * neighbour count, scans, pair fill, history remap, transpose; out[9] = radix passes (0: the
+ * rebuild sorts by counting), out[10] = new pairs, out[11] = old pairs, out[12] = cells,
+ * out[13] = particles, out[14] = ms of a second rebuild timed at its two phases' ends only
+ * (no per-stage events; the host's read of the pair total between the phases excluded)
+ * (out holds 15 doubles).  The trajectory is unchanged. */
 int dem_time_rebuild(dem_ctx *ctx, double *out);
 
 /* Introspection (host buffers), for parity checks against the reference layout. */
