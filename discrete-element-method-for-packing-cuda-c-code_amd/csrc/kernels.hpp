@@ -363,8 +363,8 @@ constexpr unsigned kTile = 512;
 constexpr unsigned kTileMax = 4 * kTile;
 
 // Candidate search (neighborarrayD's role) for the consecutive slots of a block, every lane
-// of which calls it (live: the lane has a slot a < n): every b != a in the 27 cells around
-// a's cell with is_candidate true, in stencil-row order (z-, y-offset ascending) and
+// of which calls it (live: the lane has a slot a < n): every b in the 27 cells around a's
+// cell with is_candidate true (a itself included: the caller drops it), in stencil-row order (z-, y-offset ascending) and
 // ascending slot order within a row -- the list order.  For each row the lanes' slot ranges
 // -- contiguous, and overlapping for nearby cells -- are united, the union's positions are
 // staged in LDS (tile: kTile entries) by coalesced loads, and each lane tests its own range
@@ -410,7 +410,6 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, const 
             block_minmax(s0, s1, red, lo, hi);
             if (lo < hi && hi - lo > kTileMax) {   // uniform: each lane reads its own range
                 for (unsigned b = s0; b < s1; b++) {
-                    if (b == a) continue;
                     const double4 rb = cpos[b];
                     if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
                         f(b, s0, s1, cid[b]);
@@ -424,9 +423,18 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, const 
                         tcid[k] = cid[c0 + k];
                     }
                     __syncthreads();
+                    // two candidates per iteration (both reads in flight before the first
+                    // test); the lane's own slot passes the test and f drops it
                     const unsigned b1 = min(s1, c1);
-                    for (unsigned b = max(s0, c0); b < b1; b++) {
-                        if (b == a) continue;
+                    unsigned b = max(s0, c0);
+                    for (; b + 1u < b1; b += 2u) {
+                        const double4 r0 = tile[b - c0], r1 = tile[b + 1u - c0];
+                        const bool h0 = is_candidate(xa, d3{r0.x, r0.y, r0.z}, ra.w, r0.w, I.skin);
+                        const bool h1 = is_candidate(xa, d3{r1.x, r1.y, r1.z}, ra.w, r1.w, I.skin);
+                        if (h0) f(b, s0, s1, tcid[b - c0]);
+                        if (h1) f(b + 1u, s0, s1, tcid[b + 1u - c0]);
+                    }
+                    if (b < b1) {
                         const double4 rb = tile[b - c0];
                         if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
                             f(b, s0, s1, tcid[b - c0]);
@@ -487,6 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
     for_candidates_block<BATCHED>(
         cpos, cid, a, live, live ? sorted[a].x : 0u, I, cstart, tile, tcid, red,
         [&](unsigned b, unsigned r0, unsigned r1, unsigned idb) {
+            if (b == a) return;   // the lane's own slot
             const unsigned long long bit = r1 - r0 <= kRowBits ? 1ull << (b - r0) : 0ull;
             if (idb > ida) {
                 cg++;
@@ -511,18 +520,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
     }
 }
 
-// Binary search of `target` in the ascending second members of ab[lo, hi); kEmpty if absent.
-__device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, unsigned hi,
-                                                unsigned target)
+// Binary search of `target` in the ascending values get(lo) .. get(hi - 1); kEmpty if absent.
+template <class Get>
+__device__ __forceinline__ unsigned find_sorted(Get get, unsigned lo, unsigned hi, unsigned target)
 {
     while (lo < hi) {
-        unsigned mid = (lo + hi) >> 1;
-        unsigned v = ab[mid].y;
+        const unsigned mid = (lo + hi) >> 1;
+        const unsigned v = get(mid);
         if (v == target) return mid;
         if (v < target) lo = mid + 1;
         else hi = mid;
     }
     return kEmpty;
+}
+
+// `target` among the ascending second members of ab[lo, hi); kEmpty if absent.
+__device__ __forceinline__ unsigned find_second(const uint2 *ab, unsigned lo, unsigned hi,
+                                                unsigned target)
+{
+    return find_sorted([&](unsigned i) { return ab[i].y; }, lo, hi, target);
 }
 
 // Entries of one block's own-pair and transposed ranges the fill stages in LDS and writes
@@ -533,8 +549,9 @@ constexpr unsigned kFillStage = 2048;
 // at [off[a], off[a+1]) -- each pair is stored once, first member = smaller particle index,
 // exactly the reference's Lp < Ln orientation (dempacking.cu:640-644), while segments follow
 // the cell-sorted slot order.  Partners with a smaller index go to the transposed list
-// [toff[a], toff[a+1]): jf = the partner (the pair's first member), jt = a for now; k_transpose
-// turns jt into the pair's index.  The tangential histories are filled by k_remap.  Both
+// [toff[a], toff[a+1]): jf = the partner (the pair's first member), jt = a for now; the
+// transpose turns jt into the pair's index and the remap fills the tangential histories
+// (k_transpose, k_remap).  Both
 // ranges of a block are contiguous; entries are staged in LDS and written out by consecutive
 // lanes (a range longer than kFillStage is written lane by lane).
 template <bool BATCHED>
@@ -637,21 +654,20 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
 // reorder of this rebuild records), else zero -- the
 // reference's pairlistD remap (particleSystem.cu:143-191; new particles have empty old
 // segments, dempacking.cu:204,521-522).  Consecutive lanes take consecutive pairs, mostly
-// of one first member and so of one old segment, and write whole history lines.  Each
-// block takes a contiguous, XCD-aware range of the device pair count (clamped to the
-// capacity), so a fixed grid serves a captured rebuild.  src (optional) records each pair's
-// old index for the adjoint of the remap.
-__global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl *ctl, unsigned cap,
-                                                  const unsigned *oslot, Hist *hist,
-                                                  bool have_old, const uint2 *ab_old,
-                                                  const unsigned *off_old, const Hist *hist_old,
-                                                  unsigned n_old, unsigned cap_old, unsigned *src,
-                                                  const unsigned *active)
+// of one first member and so of one old segment, and write whole history lines.  Block blk of
+// nblk takes a contiguous, XCD-aware range of the device pair count (clamped to the capacity),
+// so a fixed grid serves a captured rebuild.  src (optional) records each pair's old index for
+// the adjoint of the remap.
+__device__ __forceinline__ void remap_blocks(unsigned blk, unsigned nblk, const uint2 *ab,
+                                             const DevCtl *ctl, unsigned cap,
+                                             const unsigned *oslot, Hist *hist, bool have_old,
+                                             const uint2 *ab_old, const unsigned *off_old,
+                                             const Hist *hist_old, unsigned n_old,
+                                             unsigned cap_old, unsigned *src)
 {
-    if (!is_active(active)) return;
     const unsigned np = min(ctl->npairs, cap);
-    const unsigned per = (np + gridDim.x - 1) / gridDim.x;
-    const unsigned p_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned per = (np + nblk - 1) / nblk;
+    const unsigned p_begin = xcd_block(blk, nblk) * per;
     const unsigned p_end = min(p_begin + per, np);
     for (unsigned p = p_begin + threadIdx.x; p < p_end; p += blockDim.x) {
         const uint2 e = ab[p];
@@ -670,33 +686,26 @@ __global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl 
     }
 }
 
-// Second member b of pair index p, for the search below: bo[p] (the list's second slots).
+// `target` among the ascending second slots bo[lo, hi) of a segment; kEmpty if absent.
 __device__ __forceinline__ unsigned find_slot(const unsigned *bo, unsigned lo, unsigned hi,
                                               unsigned target)
 {
-    while (lo < hi) {
-        unsigned mid = (lo + hi) >> 1;
-        unsigned v = bo[mid];
-        if (v == target) return mid;
-        if (v < target) lo = mid + 1;
-        else hi = mid;
-    }
-    return kEmpty;
+    return find_sorted([&](unsigned i) { return bo[i]; }, lo, hi, target);
 }
 
 // Resolve each transposed entry k (first member jf[k], second member jt[k], as the fill left
 // them) to the index of that pair in the first member's segment, whose second slots ascend --
-// one lane per entry over contiguous, XCD-aware ranges of the n particles' toff[n] entries
-// (clamped to the capacity), so a fixed grid serves a captured rebuild.
-__global__ __launch_bounds__(kBlock) void k_transpose(const unsigned *bo, const unsigned *off,
-                                                      const unsigned *toff, unsigned n,
-                                                      unsigned cap, unsigned *jt,
-                                                      const unsigned *jf, const unsigned *active)
+// one lane per entry; block blk of nblk takes a contiguous, XCD-aware range of the n
+// particles' toff[n] entries (clamped to the capacity), so a fixed grid serves a captured
+// rebuild.
+__device__ __forceinline__ void transpose_blocks(unsigned blk, unsigned nblk, const unsigned *bo,
+                                                 const unsigned *off, const unsigned *toff,
+                                                 unsigned n, unsigned cap, unsigned *jt,
+                                                 const unsigned *jf)
 {
-    if (!is_active(active)) return;
     const unsigned nt = min(toff[n], cap);
-    const unsigned per = (nt + gridDim.x - 1) / gridDim.x;
-    const unsigned k_begin = xcd_block(blockIdx.x, gridDim.x) * per;
+    const unsigned per = (nt + nblk - 1) / nblk;
+    const unsigned k_begin = xcd_block(blk, nblk) * per;
     const unsigned k_end = min(k_begin + per, nt);
     for (unsigned k = k_begin + threadIdx.x; k < k_end; k += blockDim.x) {
         const unsigned b = jf[k], a = jt[k];
@@ -704,6 +713,27 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const unsigned *bo, const 
         if (hi > cap) hi = cap;
         jt[k] = lo < hi ? find_slot(bo, lo, hi, a) : kEmpty;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl *ctl, unsigned cap,
+                                                  const unsigned *oslot, Hist *hist,
+                                                  bool have_old, const uint2 *ab_old,
+                                                  const unsigned *off_old, const Hist *hist_old,
+                                                  unsigned n_old, unsigned cap_old, unsigned *src,
+                                                  const unsigned *active)
+{
+    if (!is_active(active)) return;
+    remap_blocks(blockIdx.x, gridDim.x, ab, ctl, cap, oslot, hist, have_old, ab_old, off_old,
+                 hist_old, n_old, cap_old, src);
+}
+
+__global__ __launch_bounds__(kBlock) void k_transpose(const unsigned *bo, const unsigned *off,
+                                                      const unsigned *toff, unsigned n,
+                                                      unsigned cap, unsigned *jt,
+                                                      const unsigned *jf, const unsigned *active)
+{
+    if (!is_active(active)) return;
+    transpose_blocks(blockIdx.x, gridDim.x, bo, off, toff, n, cap, jt, jf);
 }
 
 // Compact positions of restored records: cpos[s] = (px, py, pz, rad) of rec[s], and its FP32

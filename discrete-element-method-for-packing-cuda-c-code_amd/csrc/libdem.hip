@@ -111,6 +111,20 @@ int scan_excl(Scratch &S, const unsigned *in, unsigned n, unsigned *out, unsigne
     return hipGetLastError() == hipSuccess ? DEM_OK : DEM_ERR_HIP;
 }
 
+// Exclusive scans of two n-word arrays at once: out_k[i] = sum(in_k[0..i)), out_k[n] and
+// total[k] = the totals (total: 2 device words).
+int scan2_excl(Scratch &S, const unsigned *in0, const unsigned *in1, unsigned n, unsigned *out0,
+               unsigned *out1, unsigned *total, hipStream_t st, const unsigned *active)
+{
+    unsigned nt = std::max(1u, cdiv(n, kScanTile));
+    if (S.ensure(2 * (size_t)nt, 0)) return DEM_ERR_HIP;
+    hipLaunchKernelGGL(k_scan2_tiles, dim3(nt), dim3(kScanThreads), 0, st, in0, in1, n, S.tiles, active);
+    hipLaunchKernelGGL(k_scan2_top, dim3(1), dim3(kScanThreads), 0, st, S.tiles, nt, total, active);
+    hipLaunchKernelGGL(k_scan2_apply, dim3(nt), dim3(kScanThreads), 0, st, in0, in1, n,
+                       (const unsigned *)S.tiles, out0, out1, (const unsigned *)total, active);
+    return hipGetLastError() == hipSuccess ? DEM_OK : DEM_ERR_HIP;
+}
+
 // Stable LSD radix sort of n (key, value) pairs over the low `bits` key bits.  Reads `in`
 // (left untouched unless it is one of the ping-pong buffers), ping-pongs a <-> b, returns
 // the buffer holding the result.
@@ -416,8 +430,8 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
                            c->ng, c->nl, c->rmask, c->rs0, active);
     stage_event(ev, kEvCount, st);
     int nl = 1 - c->lc;
-    if (scan_excl(c->S, c->ng, n, c->off[nl], c->S.total, st, active)) return DEM_ERR_HIP;
-    if (scan_excl(c->S, c->nl, n, c->toff, c->S.total + 1, st, active)) return DEM_ERR_HIP;
+    if (scan2_excl(c->S, c->ng, c->nl, n, c->off[nl], c->toff, c->S.total, st, active))
+        return DEM_ERR_HIP;
     hipLaunchKernelGGL(k_check_cap, dim3(1), dim3(64), 0, st, c->ctl, (const unsigned *)c->S.total,
                        (const unsigned *)(c->S.total + 1),
                        std::min(std::min(c->cap[nl], c->out_cap), c->hcap[1 - c->hc]), active);

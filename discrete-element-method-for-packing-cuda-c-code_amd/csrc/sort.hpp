@@ -129,6 +129,102 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const unsigned *in,
 }
 
 // ---------------------------------------------------------------------------------------
+// The same exclusive scan of two arrays of n words at once (the own-pair and transposed
+// counts of a rebuild): out_k[i] = sum(in_k[0..i)), out_k[n] = total_k, total_out[k] =
+// total_k.  Tile sums: 2 per tile (ts[2 * tile + k]).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kScanThreads) void k_scan2_tiles(const unsigned *in0,
+                                                              const unsigned *in1, unsigned n,
+                                                              unsigned *ts, const unsigned *active)
+{
+    if (active && *active == 0u) return;
+    __shared__ unsigned wsum[kScanThreads / 64];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    unsigned s0 = 0, s1 = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        size_t i = base + k;
+        if (i < n) {
+            s0 += in0[i];
+            s1 += in1[i];
+        }
+    }
+    unsigned t0, t1;
+    block_excl_scan(s0, wsum, &t0);
+    block_excl_scan(s1, wsum, &t1);
+    if (threadIdx.x == 0) {
+        ts[2 * blockIdx.x] = t0;
+        ts[2 * blockIdx.x + 1] = t1;
+    }
+}
+
+// Single block: exclusive scans of both tile-sum sequences in place; totals to total_out[0..1].
+__global__ __launch_bounds__(kScanThreads) void k_scan2_top(unsigned *ts, unsigned ntiles,
+                                                            unsigned *total_out,
+                                                            const unsigned *active)
+{
+    if (active && *active == 0u) return;
+    __shared__ unsigned wsum[kScanThreads / 64];
+    unsigned c0 = 0, c1 = 0;
+    for (unsigned base = 0; base < ntiles; base += kScanThreads) {
+        unsigned i = base + threadIdx.x;
+        unsigned v0 = i < ntiles ? ts[2 * i] : 0u, v1 = i < ntiles ? ts[2 * i + 1] : 0u;
+        unsigned t0, t1;
+        unsigned e0 = block_excl_scan(v0, wsum, &t0);
+        unsigned e1 = block_excl_scan(v1, wsum, &t1);
+        if (i < ntiles) {
+            ts[2 * i] = c0 + e0;
+            ts[2 * i + 1] = c1 + e1;
+        }
+        c0 += t0;
+        c1 += t1;
+    }
+    if (threadIdx.x == 0) {
+        total_out[0] = c0;
+        total_out[1] = c1;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan2_apply(const unsigned *in0,
+                                                              const unsigned *in1, unsigned n,
+                                                              const unsigned *ts, unsigned *out0,
+                                                              unsigned *out1,
+                                                              const unsigned *total,
+                                                              const unsigned *active)
+{
+    if (active && *active == 0u) return;
+    __shared__ unsigned wsum[kScanThreads / 64];
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    unsigned v0[kScanItems], v1[kScanItems];
+    unsigned s0 = 0, s1 = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        size_t i = base + k;
+        v0[k] = i < n ? in0[i] : 0u;
+        v1[k] = i < n ? in1[i] : 0u;
+        s0 += v0[k];
+        s1 += v1[k];
+    }
+    unsigned t0, t1;
+    unsigned e0 = block_excl_scan(s0, wsum, &t0) + ts[2 * blockIdx.x];
+    unsigned e1 = block_excl_scan(s1, wsum, &t1) + ts[2 * blockIdx.x + 1];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        size_t i = base + k;
+        if (i < n) {
+            out0[i] = e0;
+            out1[i] = e1;
+        }
+        e0 += v0[k];
+        e1 += v1[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out0[n] = total[0];
+        out1[n] = total[1];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Radix sort kernels on uint2 (key, value).
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint2 *kv, unsigned n,
