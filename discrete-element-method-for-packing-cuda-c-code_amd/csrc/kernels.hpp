@@ -1798,11 +1798,17 @@ __global__ __launch_bounds__(kBlock) void c_force_pair(
     unsigned p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= np) return;
     unsigned i = lp[p], j = ln[p];
-    d3 fa, ta, tb, ho;
-    double fo;
-    bool c = law_pair(ld3(pos, i), ld3(pos, j), ld3(pdis, i), ld3(pdis, j), ld3(angv, i),
-                      ld3(angv, j), rad[i], rad[j], rmass[i], rmass[j], ld3(dispt, p), dt, P, fa,
-                      ta, tb, ho, fo);
+    // the contact test first, on the positions and radii alone (law_touch: the law's own test,
+    // the same decision bit for bit); only a pair in contact gathers the displacements,
+    // spins, masses and its history, and the others take the law's all-zero result
+    const d3 xi = ld3(pos, i), xj = ld3(pos, j);
+    const double ri = rad[i], rj = rad[j];
+    d3 fa = d3{0.0, 0.0, 0.0}, ta = fa, tb = fa, ho = fa;
+    double fo = 0.0;
+    bool c = false;
+    if (law_touch(xi, xj, ri, rj))
+        c = law_pair(xi, xj, ld3(pdis, i), ld3(pdis, j), ld3(angv, i), ld3(angv, j), ri, rj,
+                     rmass[i], rmass[j], ld3(dispt, p), dt, P, fa, ta, tb, ho, fo);
     st3(forcepair, p, fa);
     st3(torqpairi, p, ta);
     st3(torqpairj, p, tb);
@@ -1832,9 +1838,12 @@ __global__ __launch_bounds__(kBlock) void c_sum_forces(int n, int width, const u
     const unsigned g = njgi[i];
     const unsigned cnt = min(g + njli[i], (unsigned)width);
     // kSumAhead entries at a time: every load of a group in flight before the group's sums,
-    // which then run in row order (x - y == x + (-y) bit for bit; an empty entry adds
-    // nothing)
-    constexpr unsigned kSumAhead = 4;
+    // which then run in row order (x - y == x + (-y) bit for bit).  An entry whose contact
+    // flag is 0 holds calculateforcepair's out-of-contact result, an exact zero force and
+    // torques, so its force and torques are not read: the running sums start at +0.0 and
+    // are never -0.0, and adding or subtracting a zero leaves them unchanged bit for bit (an
+    // empty entry adds nothing either)
+    constexpr unsigned kSumAhead = 8;
     for (unsigned m0 = 0; m0 < cnt; m0 += kSumAhead) {
         unsigned p[kSumAhead];
         d3 fp[kSumAhead], tp[kSumAhead];
@@ -1843,18 +1852,18 @@ __global__ __launch_bounds__(kBlock) void c_sum_forces(int n, int width, const u
         for (unsigned k = 0; k < kSumAhead; k++)
             p[k] = m0 + k < cnt ? aref[(size_t)i * width + m0 + k] : kEmpty;
 #pragma unroll
+        for (unsigned k = 0; k < kSumAhead; k++) cp[k] = p[k] != kEmpty ? contactpair[p[k]] : 0u;
+#pragma unroll
         for (unsigned k = 0; k < kSumAhead; k++) {
             fp[k] = tp[k] = d3{0.0, 0.0, 0.0};
-            cp[k] = 0u;
-            if (p[k] != kEmpty) {
+            if (cp[k]) {
                 fp[k] = ld3(forcepair, p[k]);
                 tp[k] = ld3(m0 + k < g ? torqpairi : torqpairj, p[k]);
-                cp[k] = contactpair[p[k]];
             }
         }
 #pragma unroll
         for (unsigned k = 0; k < kSumAhead; k++) {
-            if (p[k] == kEmpty) continue;
+            if (!cp[k]) continue;
             const bool own = m0 + k < g;
             f0 += own ? fp[k].x : -fp[k].x;
             f1 += own ? fp[k].y : -fp[k].y;
