@@ -126,6 +126,48 @@ def test_rebuild_bit_exact(dem, orc, gpu_ok, which):
     assert canon(ids) == canon(sim.pairs())
 
 
+def crowded_cells(dem, seed=41):
+    """A lattice of 404 spheres plus 600 spheres outside the grid, which the hash clamps into
+    two boundary cells (300 each, in a row of spheres 0.16 d apart): cells of hundreds of
+    particles (the stable in-cell ranking, long equal-cell runs in a wave, stencil rows past
+    64 slots) and a last wave of 44 live lanes (1,004 spheres)."""
+    w = dense_workload(dem, n=404, seed=seed)
+    inst = w.instances[0]
+    hx, hy = inst.walls.size[0] / 2, inst.walls.size[1] / 2
+    k = np.arange(300, dtype=np.float64)
+    above = np.stack([np.full(300, 0.3), np.full(300, -0.2), inst.walls.size[2] + 2.0 + 0.163 * k], 1)
+    beyond = np.stack([hx + 1.0 + 0.163 * k, np.full(300, hy - 0.4), np.full(300, 3.3)], 1)
+    pos = np.concatenate([w.pos, above, beyond])
+    n = len(pos)
+    rad = np.full(n, 0.5)
+    rmass = 8 * rad ** 3
+    inert = 0.4 * rmass * rad ** 2
+    z = np.zeros((n, 3))
+    inst2 = dem.make_instance(n, inst.walls, inst.walls.size[2], 1.0 + dem.SKIN, inst.mat, inst.dt)
+    return dem.Workload("crowded", [inst2], pos, z, z.copy(), rad, rmass, inert)
+
+
+def test_rebuild_bit_exact_crowded_cells(dem, orc, gpu_ok):
+    w = crowded_cells(dem)
+    say(f"[rebuild] crowded n={w.n}")
+    ctx, sim = make_pair(dem, orc, w, width=128)
+    ctx.rebuild()
+    assert sim.rebuild() == 0
+    cs = sim.field("cell_start")
+    filled = np.diff(np.append(cs[cs != 0xFFFFFFFF], w.n))
+    assert filled.max() >= 300   # the clamped spheres do crowd into single cells
+    assert np.array_equal(ctx.hash(), sim.field("hash"))
+    assert np.array_equal(ctx.sorted(), sim.field("sorted"))
+    assert np.array_equal(ctx.cell_start(), cs)
+    ids, _, _ = ctx.pairs()
+    assert ctx.stats().npairs == sim.scalars()["npairs"]
+    assert canon(ids) == canon(sim.pairs())
+    # a second rebuild from the cell-sorted slots (equal cells now in consecutive lanes)
+    ctx.rebuild()
+    assert np.array_equal(ctx.sorted(), sim.field("sorted"))
+    assert canon(ctx.pairs()[0]) == canon(sim.pairs())
+
+
 @pytest.mark.parametrize("poly,cyl", [(False, False), (True, False), (False, True)])
 def test_first_step_forces_and_state(dem, orc, gpu_ok, poly, cyl):
     w = dense_workload(dem, poly=poly, cyl=cyl)
