@@ -1094,8 +1094,10 @@ constexpr unsigned kLook = 4;
 // 16-byte chunks by consecutive lanes (1 KB of contiguous memory per wave instruction instead
 // of 64 scattered lines), each lane working on its row in between.  A row is the 128-byte
 // record plus 16 bytes of padding, which spreads the lanes' 16-byte row accesses over all
-// LDS banks.  Dynamic LDS: blockDim.x * kRow doubles.
+// LDS banks.  Dynamic LDS: blockDim.x * kRow doubles, then blockDim.x words (k_step_lds).
 constexpr unsigned kRow = 18;
+// Dynamic LDS bytes of k_step for blocks of bd particles: the record rows and the zs words.
+inline size_t k_step_lds(unsigned bd) { return (size_t)bd * (kRow * sizeof(double) + sizeof(unsigned)); }
 // Own pairs of a block out of contact get their zero history in one cooperative pass over the
 // block's contiguous own-pair range (consecutive lanes on consecutive pairs), steered by a
 // contact bitmap in LDS; a block with more than kBits own pairs zeroes them lane by lane.
@@ -1214,7 +1216,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
 {
     extern __shared__ double lrec[];
     __shared__ unsigned cbits[kBits / 32];
-    __shared__ unsigned zsl[kBlock];
+    // one zs word per lane, after the record rows in the dynamic allocation (sized by the
+    // block, so a 128-particle block takes 19,968 bytes and 8 blocks fit a CU's 160 KB)
+    unsigned *zsl = reinterpret_cast<unsigned *>(lrec + blockDim.x * kRow);
     // Every load of the prologue is in bounds whether or not the step is gated off and
     // whatever the lane, so all of them are issued before any of their values is used: the
     // gate, the list offsets, the zs words and the block's records are in flight together.

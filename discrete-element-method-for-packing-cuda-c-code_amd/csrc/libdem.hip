@@ -582,7 +582,7 @@ void launch_force_step_t(dem_ctx *c, unsigned np_grid, hipEvent_t mid, hipEvent_
         // the other record buffer takes the whole updated record; wall histories are per
         // slot and updated in place by the step's wall contact, so they move with the records
         if (c->batched)
-            hipLaunchKernelGGL((k_step<true, FLEN>), dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
+            hipLaunchKernelGGL((k_step<true, FLEN>), dim3(gb), dim3(pb), k_step_lds(pb), st, rin, c->rec[nr],
                                (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
@@ -591,7 +591,7 @@ void launch_force_step_t(dem_ctx *c, unsigned np_grid, hipEvent_t mid, hipEvent_
                                c->tape_rec, c->tape_hist, c->tape_whist, kgate, c->fused_fast,
                                c->xcd_chunk, ktrig, queued ? slot : nullptr);
         else
-            hipLaunchKernelGGL((k_step<false, FLEN>), dim3(gb), dim3(pb), pb * kRow * sizeof(double), st, rin, c->rec[nr],
+            hipLaunchKernelGGL((k_step<false, FLEN>), dim3(gb), dim3(pb), k_step_lds(pb), st, rin, c->rec[nr],
                                (const unsigned *)c->bo, (const unsigned *)c->off[c->lc],
                                (const unsigned *)c->toff, (const unsigned *)c->jt, (const unsigned *)c->jf,
                                (const Hist *)c->hist[c->hc], c->hist[nh], (const unsigned *)c->wflag,
