@@ -654,12 +654,15 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(
 // reorder of this rebuild records), else zero -- the
 // reference's pairlistD remap (particleSystem.cu:143-191; new particles have empty old
 // segments, dempacking.cu:204,521-522).  Consecutive lanes take consecutive pairs, mostly
-// of one first member and so of one old segment, and write whole history lines.  Block blk of
+// of one first member and so of one old segment, and write whole history lines.  A rebuild
+// changes few partners, so the pair is first looked for in the old segment at the place it
+// has in the new one (one load, kept when the second member matches: second members are
+// distinct in a segment), and searched for only when that misses.  Block blk of
 // nblk takes a contiguous, XCD-aware range of the device pair count (clamped to the capacity),
 // so a fixed grid serves a captured rebuild.  src (optional) records each pair's old index for
 // the adjoint of the remap.
 __device__ __forceinline__ void remap_blocks(unsigned blk, unsigned nblk, const uint2 *ab,
-                                             const DevCtl *ctl, unsigned cap,
+                                             const unsigned *off, const DevCtl *ctl, unsigned cap,
                                              const unsigned *oslot, Hist *hist, bool have_old,
                                              const uint2 *ab_old, const unsigned *off_old,
                                              const Hist *hist_old, unsigned n_old,
@@ -675,9 +678,11 @@ __device__ __forceinline__ void remap_blocks(unsigned blk, unsigned nblk, const 
         unsigned q = kEmpty;
         if (have_old) {
             const unsigned oa = oslot[e.x], ob = oslot[e.y];
+            const unsigned j = p - off[e.x];   // the pair's place in its new segment
             if (oa < n_old && ob != kEmpty) {
                 const unsigned lo = min(off_old[oa], cap_old), hi = min(off_old[oa + 1], cap_old);
-                q = find_second(ab_old, lo, hi, ob);
+                const unsigned qp = lo + j;
+                q = (qp < hi && ab_old[qp].y == ob) ? qp : find_second(ab_old, lo, hi, ob);
                 if (q != kEmpty) h = hist_old[q];
             }
         }
@@ -715,7 +720,7 @@ __device__ __forceinline__ void transpose_blocks(unsigned blk, unsigned nblk, co
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl *ctl, unsigned cap,
+__global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const unsigned *off, const DevCtl *ctl, unsigned cap,
                                                   const unsigned *oslot, Hist *hist,
                                                   bool have_old, const uint2 *ab_old,
                                                   const unsigned *off_old, const Hist *hist_old,
@@ -723,7 +728,7 @@ __global__ __launch_bounds__(kBlock) void k_remap(const uint2 *ab, const DevCtl 
                                                   const unsigned *active)
 {
     if (!is_active(active)) return;
-    remap_blocks(blockIdx.x, gridDim.x, ab, ctl, cap, oslot, hist, have_old, ab_old, off_old,
+    remap_blocks(blockIdx.x, gridDim.x, ab, off, ctl, cap, oslot, hist, have_old, ab_old, off_old,
                  hist_old, n_old, cap_old, src);
 }
 

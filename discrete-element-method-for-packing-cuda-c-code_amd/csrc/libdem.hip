@@ -469,7 +469,7 @@ int rebuild_fill(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     // histories by pair identity, one lane per pair over the device pair count
     const unsigned rg = std::max(1u, std::min(cdiv(std::max(capn, 1u), kBlock), (unsigned)c->cus * 16u));
     hipLaunchKernelGGL(k_remap, dim3(rg), dim3(kBlock), 0, st, (const uint2 *)c->ab[nl],
-                       (const DevCtl *)c->ctl, capn, (const unsigned *)c->oslot, c->hist[nh],
+                       (const unsigned *)c->off[nl], (const DevCtl *)c->ctl, capn, (const unsigned *)c->oslot, c->hist[nh],
                        c->have_list, (const uint2 *)c->ab[ol], (const unsigned *)c->off[ol],
                        (const Hist *)c->hist[oh], c->n_list, capo, c->tape_src, active);
     stage_event(ev, kEvRemap, st);
