@@ -196,16 +196,21 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
 }
 
 // Counting-sort scatter: the pair of slot s goes to its cell's first slot plus its arrival
-// rank (a permutation of [0, n): ranks are 0 .. count - 1 within each cell).
+// rank (a permutation of [0, n): ranks are 0 .. count - 1 within each cell), and the slot s
+// itself -- the particle's slot in the records being sorted -- goes to the same place of
+// oslot_out, so the reorder needs no index-to-slot lookup.
 __global__ __launch_bounds__(kBlock) void k_sort_scatter(const uint2 *kv, const unsigned *rank,
                                                          const unsigned *cstart, unsigned n,
-                                                         uint2 *out, const unsigned *active)
+                                                         uint2 *out, unsigned *oslot_out,
+                                                         const unsigned *active)
 {
     if (!is_active(active)) return;
     const unsigned s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint2 e = kv[s];
-    out[cstart[e.x] + rank[s]] = e;
+    const unsigned d = cstart[e.x] + rank[s];
+    out[d] = e;
+    oslot_out[d] = s;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(unsigned *a, unsigned n, unsigned v,
@@ -250,11 +255,12 @@ __device__ __forceinline__ double wall_reach(const WallGeom &g, double x, double
 // number of particles of that cell with a smaller index, and writes the pair to sorted[s'].
 // Also lists the particles that may touch a wall before the next rebuild (for k_walls).
 template <bool BATCHED>
-__global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *scat, const unsigned *cstart,
+__global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *scat, const unsigned *sslot,
+                                                    const unsigned *cstart,
                                                     uint2 *sorted, unsigned n,
                                                     const PRec *rec_old, PRec *rec_new,
                                                     const Hist *wh_old, Hist *wh_new,
-                                                    const unsigned *inv_old, unsigned *inv_new,
+                                                    unsigned *inv_new,
                                                     const InstDev *inst, InstDev one,
                                                     unsigned *wflag, unsigned *wlist,
                                                     DevCtl *ctl, unsigned *perm,
@@ -272,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint2 *scat, const uns
     if (s < n) {
         e = scat[s];
         id = e.y;
-        o = inv_old[id];
+        o = sslot[s];   // the particle's slot in rec_old (k_sort_scatter)
         const unsigned c0 = cstart[e.x], c1 = cstart[e.x + 1];
         // every load of the record in flight before any store (the stores cannot be moved
         // above loads of a buffer the compiler cannot tell apart from theirs)

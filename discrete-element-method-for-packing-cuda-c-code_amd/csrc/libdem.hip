@@ -382,7 +382,8 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     InstDev one = c->hinst[0];
     // counting sort by cell (kernels.hpp): counts, their scan (the cells' first slots), the
     // scatter by arrival rank; k_reorder then restores the index order inside each cell.
-    // The ranks live in ng until the neighbour count writes it.
+    // The ranks live in ng and the scattered pairs' old slots in nl until the neighbour count
+    // writes them.
     hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(c->ncells, kBlock), 4096u)), dim3(kBlock), 0, st,
                        c->ccount, c->ncells, 0u, active);
     if (c->batched)
@@ -395,23 +396,25 @@ int rebuild_count(dem_ctx *c, const unsigned *active, hipEvent_t *ev = nullptr)
     if (scan_excl(c->S, c->ccount, c->ncells, c->cstart, c->S.total + 2, st, active))
         return DEM_ERR_HIP;
     hipLaunchKernelGGL(k_sort_scatter, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_hash,
-                       (const unsigned *)c->ng, (const unsigned *)c->cstart, n, c->kv_a, active);
+                       (const unsigned *)c->ng, (const unsigned *)c->cstart, n, c->kv_a, c->nl,
+                       active);
     stage_event(ev, kEvSort, st);   // the cell ranges are the scan's output
     c->kv_sorted = c->kv_b;
     int nr = 1 - c->rc;
     const unsigned n_old = c->have_list ? c->n_list : 0u;   // particles the old list holds
     if (c->batched)
         hipLaunchKernelGGL(k_reorder<true>, dim3(gb), dim3(kBlock), 0, st, (const uint2 *)c->kv_a,
-                           (const unsigned *)c->cstart, c->kv_sorted,
+                           (const unsigned *)c->nl, (const unsigned *)c->cstart, c->kv_sorted,
                            n, c->rec[c->rc], c->rec[nr], c->whist[c->pc], c->whist[1 - c->pc],
-                           c->inv[c->pc], c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
+                           c->inv[1 - c->pc], (const InstDev *)c->dinst, one, c->wflag,
                            c->wlist, c->ctl, c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid,
                            c->oslot, n_old, active);
     else
         hipLaunchKernelGGL(k_reorder<false>, dim3(gb), dim3(kBlock), 0, st,
-                           (const uint2 *)c->kv_a, (const unsigned *)c->cstart, c->kv_sorted, n,
+                           (const uint2 *)c->kv_a, (const unsigned *)c->nl,
+                           (const unsigned *)c->cstart, c->kv_sorted, n,
                            c->rec[c->rc], c->rec[nr],
-                           c->whist[c->pc], c->whist[1 - c->pc], c->inv[c->pc], c->inv[1 - c->pc],
+                           c->whist[c->pc], c->whist[1 - c->pc], c->inv[1 - c->pc],
                            (const InstDev *)c->dinst, one, c->wflag, c->wlist, c->ctl,
                            c->tape_perm, c->cpos[nr], c->cp32[nr], c->cid, c->oslot, n_old,
                            active);
