@@ -149,6 +149,29 @@ __device__ __forceinline__ unsigned xcd_chunk_block(unsigned b, unsigned nb, uns
 // restores the index order inside each cell while it moves the records.  Cells are offset per
 // instance so one sort serves every instance.
 
+// Adds one to cnt[h] for every calling lane and returns the lane's arrival rank there.  The
+// calling lanes must be a prefix of the wave (lanes 0 .. k-1).  Lanes that ask for the same
+// cell as their neighbour form runs, and the first lane of each run takes the run's ranks
+// with one atomic and hands them on (a cell split over several runs takes one atomic per run),
+// so inputs in cell order cost one atomic per run, not per lane (atomics on one address
+// serialize).
+__device__ __forceinline__ unsigned run_count(unsigned *cnt, unsigned h)
+{
+    const unsigned lane = __lane_id();
+    const unsigned hp = (unsigned)__shfl_up((int)h, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0u || hp != h);
+    const unsigned long long live = __ballot(1);
+    const unsigned long long le = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
+    const unsigned hl = 63u - (unsigned)__builtin_clzll(heads & le);   // this lane's run head
+    const unsigned long long after = heads & ~le;                      // heads past this lane
+    const unsigned end = after ? (unsigned)__builtin_ctzll(after)
+                               : 64u - (unsigned)__builtin_clzll(live);   // one past the run
+    unsigned base = 0;
+    if (lane == hl) base = atomicAdd(&cnt[h], end - hl);
+    base = (unsigned)__shfl((int)base, (int)hl, 64);
+    return base + (lane - hl);
+}
+
 // calcHashD on slot-ordered records: kv[s] = (cell, index) of the particle in slot s, its
 // cell's count incremented (cnt zeroed beforehand) and its arrival rank in rank[s].
 template <bool BATCHED>
@@ -176,23 +199,8 @@ __global__ __launch_bounds__(kBlock) void k_hash(const PRec *rec, unsigned n, co
     unsigned z = cell_coord(pz, I.origin[2], I.cell[2], I.gsize[2]);
     unsigned h = z * I.gsize[1] * I.gsize[0] + y * I.gsize[0] + x + I.cell_off;
     kv[s] = make_uint2(h, id);
-    // Slots are in the last rebuild's cell order and few particles change cell, so a wave's
-    // lanes form runs of equal cells: the first lane of each run takes the run's ranks with
-    // one atomic and hands them on (a cell split over several runs takes one atomic per run).
-    // The live lanes (s < n) are a prefix of the wave.
-    const unsigned lane = __lane_id();
-    const unsigned hp = (unsigned)__shfl_up((int)h, 1, 64);
-    const unsigned long long heads = __ballot(lane == 0u || hp != h);
-    const unsigned long long live = __ballot(1);
-    const unsigned long long le = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
-    const unsigned hl = 63u - (unsigned)__builtin_clzll(heads & le);   // this lane's run head
-    const unsigned long long after = heads & ~le;                      // heads past this lane
-    const unsigned end = after ? (unsigned)__builtin_ctzll(after)
-                               : 64u - (unsigned)__builtin_clzll(live);   // one past the run
-    unsigned base = 0;
-    if (lane == hl) base = atomicAdd(&cnt[h], end - hl);
-    base = (unsigned)__shfl((int)base, (int)hl, 64);
-    rank[s] = base + (lane - hl);
+    // slots are in the last rebuild's cell order and few particles change cell
+    rank[s] = run_count(cnt, h);
 }
 
 // Counting-sort scatter: the pair of slot s goes to its cell's first slot plus its arrival
@@ -1657,50 +1665,64 @@ __global__ __launch_bounds__(kBlock) void c_calc_hash(const double *pos, unsigne
     hash[i] = make_uint2(compat_hash(pos + 3 * (size_t)i, G), i);
 }
 
-// neighborarrayD (absent; call particleSystem.cu:85-96): candidates |xj - xi| < cub with
-// unit-diameter spheres, written in stencil order into a row of `width` slots.
-__global__ __launch_bounds__(kBlock) void c_neighbor_array(const double *pos, const uint2 *sorted,
-                                                           const unsigned *cstart, unsigned n,
-                                                           CompatGrid G, int width, double skin,
-                                                           unsigned *anei, unsigned *njgi,
-                                                           unsigned *njli, unsigned *overflow)
+// neighborarrayD (absent; call particleSystem.cu:85-96): candidates |xj - xi| < cub for
+// unit-diameter spheres, in stencil order (rows z, then y ascending; in a row the cells x - 1,
+// x, x + 1; in a cell its sorted entries), written into particle i's row of `width` slots;
+// njgi / njli count the candidates with a larger / smaller index.  The search runs in sorted
+// order: c_sorted_copy gathers the positions into sorted order once, c_cell_count and a scan
+// give every cell's slot range, and c_neighbor_search stages each block's stencil rows in LDS
+// (for_candidates_block) instead of gathering every candidate from the particle-index-ordered
+// positions.
+
+// spos[s] = (position of the particle in sorted slot s, radius 0.5), sid[s] = its index.
+__global__ __launch_bounds__(kBlock) void c_sorted_copy(const double *pos, const uint2 *sorted,
+                                                        unsigned n, double4 *spos, unsigned *sid)
 {
-    unsigned i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const double *pi = pos + 3 * (size_t)i;
-    d3 xi = d3{pi[0], pi[1], pi[2]};
-    unsigned h = compat_hash(pi, G);
-    unsigned gxy = G.g[0] * G.g[1];
-    int cz = (int)(h / gxy);
-    int cy = (int)((h - (unsigned)cz * gxy) / G.g[0]);
-    int cx = (int)(h - (unsigned)cz * gxy - (unsigned)cy * G.g[0]);
+    const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const unsigned i = sorted[s].y;
+    const double *p = pos + 3 * (size_t)i;
+    spos[s] = make_double4(p[0], p[1], p[2], 0.5);
+    sid[s] = i;
+}
+
+// Particles per cell of the sorted hash (cnt zeroed beforehand); its exclusive scan gives
+// every cell's first slot, cell c holding slots [pre[c], pre[c + 1]).
+__global__ __launch_bounds__(kBlock) void c_cell_count(const uint2 *sorted, unsigned n,
+                                                       unsigned ncells, unsigned *cnt)
+{
+    const unsigned s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    (void)run_count(cnt, min(sorted[s].x, ncells - 1u));
+}
+
+__global__ __launch_bounds__(kBlock) void c_neighbor_search(const double4 *spos,
+                                                            const unsigned *sid,
+                                                            const uint2 *sorted,
+                                                            const unsigned *pre, unsigned n,
+                                                            InstDev I, int width, unsigned *anei,
+                                                            unsigned *njgi, unsigned *njli,
+                                                            unsigned *overflow)
+{
+    __shared__ double4 tile[kTile];
+    __shared__ unsigned tcid[kTile];
+    __shared__ unsigned red[2 * (kBlock / 64)];
+    const unsigned a = blockIdx.x * kBlock + threadIdx.x;   // sorted slot
+    const bool live = a < n;
+    const unsigned i = live ? sid[a] : 0u;
     int cnt = 0;
     unsigned ng = 0, nl = 0;
-    for (int dz = -1; dz <= 1; dz++) {
-        int z = cz + dz;
-        if (z < 0 || z >= (int)G.g[2]) continue;
-        for (int dy = -1; dy <= 1; dy++) {
-            int y = cy + dy;
-            if (y < 0 || y >= (int)G.g[1]) continue;
-            for (int dx = -1; dx <= 1; dx++) {
-                int x = cx + dx;
-                if (x < 0 || x >= (int)G.g[0]) continue;
-                unsigned c = (unsigned)z * gxy + (unsigned)y * G.g[0] + (unsigned)x;
-                unsigned s = cstart[c];
-                if (s == kEmpty) continue;
-                for (; s < n && sorted[s].x == c; s++) {
-                    unsigned j = sorted[s].y;
-                    if (j == i) continue;
-                    const double *pj = pos + 3 * (size_t)j;
-                    if (!is_candidate(xi, d3{pj[0], pj[1], pj[2]}, 0.5, 0.5, skin)) continue;
-                    if (cnt < width) anei[(size_t)i * width + cnt] = j;
-                    cnt++;
-                    if (j > i) ng++;
-                    else nl++;
-                }
-            }
-        }
-    }
+    for_candidates_block<false>(
+        spos, sid, a, live, live ? sorted[a].x : 0u, I, pre, tile, tcid, red,
+        [&](unsigned b, unsigned, unsigned, unsigned j) {
+            if (b == a) return;   // the particle itself
+            if (cnt < width) anei[(size_t)i * width + cnt] = j;
+            cnt++;
+            if (j > i) ng++;
+            else nl++;
+        },
+        [&](int, unsigned, unsigned) {});
+    if (!live) return;
     for (int k = cnt; k < width; k++) anei[(size_t)i * width + k] = kEmpty;
     njgi[i] = ng;
     njli[i] = nl;

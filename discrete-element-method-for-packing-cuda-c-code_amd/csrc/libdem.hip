@@ -2371,7 +2371,7 @@ Wallproperties g_parw = {0, 0, 0, 0, 0, 0};
 bool g_walls_set = false;
 dem_walls g_walls = {DEM_WALLS_BOX, 0, {0, 0, 0}, 0};
 Scratch g_S;
-unsigned *g_words = nullptr;                 // [0] iall, [1] overflow, [2] pcntot, [4..5] maxz
+unsigned *g_words = nullptr;                 // [0] iall, [1] overflow, [2] pcntot, [3] scan total, [4..5] maxz
 
 constexpr double kCua = 1.0;                 // neiglist.cu:27
 constexpr double kCub = 1.55;                // neiglist.cu:28
@@ -2381,6 +2381,36 @@ int compat_words()
     if (!g_words) DEM_HIP(hipMalloc(&g_words, 8 * sizeof(unsigned)));
     return DEM_OK;
 }
+
+// neighborarray's sorted copies and cell prefix, grown on demand.
+struct NbrScratch {
+    double4 *spos = nullptr;
+    unsigned *sid = nullptr, *pre = nullptr, *cnt = nullptr;
+    size_t n_cap = 0, c_cap = 0;
+    int ensure(size_t n, size_t cells)
+    {
+        if (n > n_cap) {
+            if (spos) DEM_HIP(hipFree(spos));
+            if (sid) DEM_HIP(hipFree(sid));
+            spos = nullptr;
+            sid = nullptr;
+            n_cap = 0;
+            DEM_HIP(hipMalloc(&spos, n * sizeof(double4)));
+            DEM_HIP(hipMalloc(&sid, n * sizeof(unsigned)));
+            n_cap = n;
+        }
+        if (cells + 1 > c_cap) {
+            if (pre) DEM_HIP(hipFree(pre));
+            if (cnt) DEM_HIP(hipFree(cnt));
+            pre = cnt = nullptr;
+            c_cap = 0;
+            DEM_HIP(hipMalloc(&pre, (cells + 1) * sizeof(unsigned)));
+            DEM_HIP(hipMalloc(&cnt, (cells + 1) * sizeof(unsigned)));
+            c_cap = cells + 1;
+        }
+        return DEM_OK;
+    }
+} g_nbr;
 
 CompatGrid make_grid(const uint g[3], const double cell[3], const double origin[3])
 {
@@ -2540,12 +2570,33 @@ void neighborarray(double *pos, uint *particleHash, uint *cellStart, uint gridSi
                    int numBodies, uint *Anei, uint *njgi, uint *njli)
 {
     if (numBodies <= 0 || maxParticlesPerCell <= 0) return;
-    if (compat_words()) return;
+    (void)cellStart;   // the cell ranges follow from the sorted hash itself (c_cell_count)
+    const unsigned long long cells = (unsigned long long)gridSize[0] * gridSize[1] * gridSize[2];
+    if (cells == 0 || cells >= 0xffffffffull) { g_last_error = DEM_ERR_INVALID; return; }
+    if (compat_words() || g_nbr.ensure((size_t)numBodies, (size_t)cells)) {
+        g_last_error = DEM_ERR_HIP;
+        return;
+    }
+    const unsigned n = (unsigned)numBodies;
+    InstDev I{};
+    for (int k = 0; k < 3; k++) I.gsize[k] = gridSize[k];
+    I.cell_off = 0;
+    I.skin = kCub - kCua;
     DEM_HIP_V(hipMemsetAsync(g_words + 1, 0, sizeof(unsigned), 0));
-    hipLaunchKernelGGL(c_neighbor_array, dim3(cdiv(numBodies, kBlock)), dim3(kBlock), 0, 0,
-                       (const double *)pos, (const uint2 *)particleHash, (const unsigned *)cellStart,
-                       (unsigned)numBodies, make_grid(gridSize, cellSize, worldOrigin),
-                       maxParticlesPerCell, kCub - kCua, Anei, njgi, njli, g_words + 1);
+    hipLaunchKernelGGL(c_sorted_copy, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, 0, (const double *)pos,
+                       (const uint2 *)particleHash, n, g_nbr.spos, g_nbr.sid);
+    hipLaunchKernelGGL(k_fill, dim3(std::min(cdiv(cells, kBlock), 4096u)), dim3(kBlock), 0, 0,
+                       g_nbr.cnt, (unsigned)cells, 0u, (const unsigned *)nullptr);
+    hipLaunchKernelGGL(c_cell_count, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, 0,
+                       (const uint2 *)particleHash, n, (unsigned)cells, g_nbr.cnt);
+    if (scan_excl(g_S, g_nbr.cnt, (unsigned)cells, g_nbr.pre, g_words + 3, 0, nullptr)) {
+        g_last_error = DEM_ERR_HIP;
+        return;
+    }
+    hipLaunchKernelGGL(c_neighbor_search, dim3(cdiv(n, kBlock)), dim3(kBlock), 0, 0,
+                       (const double4 *)g_nbr.spos, (const unsigned *)g_nbr.sid,
+                       (const uint2 *)particleHash, (const unsigned *)g_nbr.pre, n, I,
+                       maxParticlesPerCell, Anei, njgi, njli, g_words + 1);
     check_launch("neighborarray");
     unsigned ov = 0;
     DEM_HIP_V(hipMemcpy(&ov, g_words + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
