@@ -1674,6 +1674,17 @@ __global__ __launch_bounds__(kBlock) void c_calc_hash(const double *pos, unsigne
 // (for_candidates_block) instead of gathering every candidate from the particle-index-ordered
 // positions.
 
+// row[from .. width) = v: 4-byte stores up to a 16-byte boundary, then 16-byte stores (a
+// quarter of the store instructions for the tail of a mostly empty row), then the rest.
+__device__ __forceinline__ void fill_row_tail(unsigned *row, unsigned from, unsigned width,
+                                              unsigned v)
+{
+    unsigned m = from;
+    while (m < width && ((reinterpret_cast<uintptr_t>(row + m) & 15u) != 0u)) row[m++] = v;
+    for (; m + 4u <= width; m += 4u) *reinterpret_cast<uint4 *>(row + m) = make_uint4(v, v, v, v);
+    for (; m < width; m++) row[m] = v;
+}
+
 // spos[s] = (position of the particle in sorted slot s, radius 0.5), sid[s] = its index.
 __global__ __launch_bounds__(kBlock) void c_sorted_copy(const double *pos, const uint2 *sorted,
                                                         unsigned n, double4 *spos, unsigned *sid)
@@ -1723,7 +1734,7 @@ __global__ __launch_bounds__(kBlock) void c_neighbor_search(const double4 *spos,
         },
         [&](int, unsigned, unsigned) {});
     if (!live) return;
-    for (int k = cnt; k < width; k++) anei[(size_t)i * width + k] = kEmpty;
+    fill_row_tail(anei + (size_t)i * width, (unsigned)min(cnt, width), (unsigned)width, kEmpty);
     njgi[i] = ng;
     njli[i] = nl;
     if (cnt > width) atomicOr(overflow, 1u);
@@ -1751,7 +1762,16 @@ __global__ __launch_bounds__(kBlock) void c_pair_list(
     unsigned k = 0, r = 0;
     unsigned os = i > 0 ? oldsjgi[i - 1] : 0u;
     unsigned oe = oldsjgi[i];
-    for (unsigned m = 0; m < (unsigned)width; m++) aref[(size_t)i * width + m] = kEmpty;
+    // the old segment's partners, read once (kOldHeld of them held in registers; a longer
+    // segment's rest is scanned in memory)
+    constexpr unsigned kOldHeld = 16;
+    unsigned ol[kOldHeld];
+#pragma unroll
+    for (unsigned t = 0; t < kOldHeld; t++) ol[t] = os + t < oe ? oldln[os + t] : kEmpty;
+    // with cnt <= width every entry below cnt is written below (own slots, then transposed
+    // marks); an overflowed row (neighborarray reported it) is cleared whole first
+    fill_row_tail(aref + (size_t)i * width, cnt <= (unsigned)width ? cnt : 0u, (unsigned)width,
+                  kEmpty);
     for (unsigned m = 0; m < cnt && m < (unsigned)width; m++) {
         unsigned j = anei[(size_t)i * width + m];
         if (j > i) {
@@ -1761,14 +1781,19 @@ __global__ __launch_bounds__(kBlock) void c_pair_list(
             aref[(size_t)i * width + k] = slot;
             k++;
             double h0 = 0.0, h1 = 0.0, h2 = 0.0, f = 0.0;
-            for (unsigned q = os; q < oe; q++) {
-                if (oldln[q] == j) {
-                    h0 = olddispt[3 * (size_t)q];
-                    h1 = olddispt[3 * (size_t)q + 1];
-                    h2 = olddispt[3 * (size_t)q + 2];
-                    f = oldfricp[q];
-                    break;
-                }
+            // the first old entry of partner j, as a scan of the old segment finds it
+            unsigned q = kEmpty;
+#pragma unroll
+            for (unsigned t = kOldHeld; t-- > 0;)
+                if (ol[t] == j) q = os + t;
+            if (q == kEmpty)
+                for (unsigned t = os + kOldHeld; t < oe; t++)
+                    if (oldln[t] == j) { q = t; break; }
+            if (q != kEmpty) {
+                h0 = olddispt[3 * (size_t)q];
+                h1 = olddispt[3 * (size_t)q + 1];
+                h2 = olddispt[3 * (size_t)q + 2];
+                f = oldfricp[q];
             }
             dispt[3 * (size_t)slot] = h0;
             dispt[3 * (size_t)slot + 1] = h1;
