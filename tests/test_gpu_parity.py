@@ -588,6 +588,10 @@ def test_dropin_reference_sequence(dem, orc, gpu_ok):
                 assert np.array_equal(hash0.host().reshape(n, 2), sim_h.field("sorted"))
                 assert np.array_equal(cstart.host(), sim_h.field("cell_start"))
             L.neighborarray(pos.p, hash0.p, cstart.p, G, cell, orig, W, n, anei.p, njgi.p, njli.p)
+            if it == 0:   # whole rows, the empty tails included
+                assert np.array_equal(anei.host().reshape(n, W), sim_h.field("anei").reshape(n, W))
+                assert np.array_equal(njgi.host(), sim_h.field("njgi"))
+                assert np.array_equal(njli.host(), sim_h.field("njli"))
             L.prefixsum(n, njgi.p, sjgi.p, oldsjgi.p, ctypes.byref(P))
             L.pairlist(sjgi.p, oldsjgi.p, oldln.p, anei.p, njgi.p, njli.p, olddispt.p,
                        oldfricp.p, W, W, n, aref.p, lp.p, ln.p, dispt.p, fricp.p)
