@@ -375,6 +375,9 @@ __device__ __forceinline__ void block_minmax(unsigned lo, unsigned hi, unsigned 
 // lanes read their ranges from cpos directly instead).
 constexpr unsigned kTile = 512;
 constexpr unsigned kTileMax = 4 * kTile;
+// Candidates a lane tests per window of the staged search (tile arrays hold kTile + kWin
+// entries: a window may read past the staged ones and discards those bits).
+constexpr unsigned kWin = 8;
 
 // Candidate search (neighborarrayD's role) for the consecutive slots of a block, every lane
 // of which calls it (live: the lane has a slot a < n): every b in the 27 cells around a's
@@ -437,21 +440,50 @@ __device__ __forceinline__ void for_candidates_block(const double4 *cpos, const 
                         tcid[k] = cid[c0 + k];
                     }
                     __syncthreads();
-                    // two candidates per iteration (both reads in flight before the first
-                    // test); the lane's own slot passes the test and f drops it
-                    const unsigned b1 = min(s1, c1);
-                    unsigned b = max(s0, c0);
-                    for (; b + 1u < b1; b += 2u) {
-                        const double4 r0 = tile[b - c0], r1 = tile[b + 1u - c0];
-                        const bool h0 = is_candidate(xa, d3{r0.x, r0.y, r0.z}, ra.w, r0.w, I.skin);
-                        const bool h1 = is_candidate(xa, d3{r1.x, r1.y, r1.z}, ra.w, r1.w, I.skin);
-                        if (h0) f(b, s0, s1, tcid[b - c0]);
-                        if (h1) f(b + 1u, s0, s1, tcid[b + 1u - c0]);
-                    }
-                    if (b < b1) {
-                        const double4 rb = tile[b - c0];
-                        if (is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w, I.skin))
+                    // The lane's slots [b0, b0 + len) of this chunk, tested kWin at a time
+                    // with no branch per candidate: window w of every lane starts at the same
+                    // offset from the lane's own b0, so the hit bits land at fixed positions
+                    // of a 64-slot mask, and f runs afterwards for the set bits in ascending
+                    // slot order (a candidate test hits ~4% of the time, but some lane of the
+                    // wave almost always does, so a branch per candidate ran f's body for
+                    // nearly every candidate).  A lane whose range ends inside a window reads
+                    // up to kWin - 1 entries past it (tile holds kTile + kWin entries) and
+                    // clears those bits; the lane's own slot passes the test and f drops it.
+                    const unsigned b0 = max(s0, c0), b1 = min(s1, c1);
+                    const unsigned len = b1 > b0 ? b1 - b0 : 0u;
+                    for (unsigned w0 = 0; __ballot(w0 < len) != 0ull; w0 += 64u) {
+                        unsigned mlo = 0u, mhi = 0u;
+                        for (unsigned w = 0; w < 64u; w += kWin) {
+                            const bool act = w0 + w < len;
+                            if (__ballot(act) == 0ull) break;
+                            const double4 *tw = tile + (act ? b0 - c0 + w0 + w : 0u);
+                            unsigned m = 0u;
+#pragma unroll
+                            for (unsigned t = 0; t < kWin; t++) {
+                                const double4 rb = tw[t];
+                                m |= (unsigned)is_candidate(xa, d3{rb.x, rb.y, rb.z}, ra.w, rb.w,
+                                                            I.skin) << t;
+                            }
+                            if (w < 32u) mlo |= m << w;
+                            else mhi |= m << (w - 32u);
+                        }
+                        const unsigned rem = len > w0 ? len - w0 : 0u;   // slots left in range
+                        if (rem < 32u) {
+                            mlo &= (1u << rem) - 1u;
+                            mhi = 0u;
+                        } else if (rem < 64u) {
+                            mhi &= (1u << (rem - 32u)) - 1u;
+                        }
+                        while (mlo) {
+                            const unsigned b = b0 + w0 + (unsigned)__builtin_ctz(mlo);
+                            mlo &= mlo - 1u;
                             f(b, s0, s1, tcid[b - c0]);
+                        }
+                        while (mhi) {
+                            const unsigned b = b0 + w0 + 32u + (unsigned)__builtin_ctz(mhi);
+                            mhi &= mhi - 1u;
+                            f(b, s0, s1, tcid[b - c0]);
+                        }
                     }
                 }
             }
@@ -497,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_count(const PRec *rec, const dou
                                                       unsigned *rs0, const unsigned *active)
 {
     if (!is_active(active)) return;   // uniform over the grid
-    __shared__ double4 tile[kTile];
+    __shared__ double4 tile[kTile + kWin];
     __shared__ unsigned tcid[kTile];
     __shared__ unsigned red[2 * (kBlock / 64)];
     const unsigned a = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -1715,7 +1747,7 @@ __global__ __launch_bounds__(kBlock) void c_neighbor_search(const double4 *spos,
                                                             unsigned *njgi, unsigned *njli,
                                                             unsigned *overflow)
 {
-    __shared__ double4 tile[kTile];
+    __shared__ double4 tile[kTile + kWin];
     __shared__ unsigned tcid[kTile];
     __shared__ unsigned red[2 * (kBlock / 64)];
     const unsigned a = blockIdx.x * kBlock + threadIdx.x;   // sorted slot
