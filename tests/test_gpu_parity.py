@@ -237,6 +237,30 @@ def test_trajectory_parity_fixed_interval(dem, orc, gpu_ok):
     assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
 
 
+def test_timed_rebuild_is_a_rebuild(dem, orc, gpu_ok):
+    """dem_time_rebuild (two rebuilds in a row, timed by stage and by phase) leaves the context
+    as one rebuild at that step does: the trajectory equals the oracle's with a rebuild forced
+    at the same step, and the fixed-interval schedule counts from it."""
+    w = dense_workload(dem, n=3000, seed=3)
+    K = 20
+    ctx, sim = make_pair(dem, orc, w, interval=K)
+    ctx.step(50)
+    rb = ctx.time_rebuild()
+    ctx.step(50)
+    rebuild_at = {0, 20, 40, 50, 70, 90}
+    for s in range(100):
+        sim.set_iall(1 if s in rebuild_at else 0)
+        assert sim.step(1) == 0
+    st = ctx.state()
+    for k in ("pdis", "angv"):
+        assert rel_err(st[k], sim.field(k)) <= TOL, k
+    assert rel_err(st["pos"] - w.pos, sim.field("pos") - w.pos) <= TOL
+    assert rb["n"] == w.n and rb["npairs"] > 0 and rb["npairs_old"] > 0
+    assert rb["ncells"] == int(np.prod(w.instances[0].gsize[:]))
+    assert all(rb[k] >= 0.0 for k in ctx.REBUILD_STAGES) and rb["phases_ms"] > 0.0
+    assert ctx.stats().rebuilds == 7   # 0, 20, 40, the two timed ones, 70, 90
+
+
 def test_trajectory_parity_displacement_trigger(dem, orc, gpu_ok):
     """Reference rebuild policy (|q| > 0.5*skin triggers a rebuild at the next step): both
     sides must rebuild at the same steps."""
