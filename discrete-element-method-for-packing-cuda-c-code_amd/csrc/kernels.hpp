@@ -1438,19 +1438,16 @@ __global__ __launch_bounds__(kBlock, 4) void k_step(
                 double2 h0 = ld2(H), h1 = ld2(H + 2);
                 // the partner's record: from the block's LDS rows when the partner is one of
                 // the block's particles (the same values as in rin: rows are updated only
-                // after the barrier that ends the pair work), else from rin.  The batched form
-                // keeps the global read (its instantiation measured 25% slower with the LDS
-                // branch, while the plain one gains 2.6% on a poured bed and nothing elsewhere).
-                double2 b0, b1, b2, b3, b4, b5;
-                if (!BATCHED && pb - s0 < nb) {
-                    const double *B = lrec + (pb - s0) * kRow;
-                    b0 = ld2(B); b1 = ld2(B + 2); b2 = ld2(B + 4);
-                    b3 = ld2(B + 6); b4 = ld2(B + 8); b5 = ld2(B + 10);
-                } else {
-                    const double *B = reinterpret_cast<const double *>(rin + pb);
-                    b0 = ld2(B); b1 = ld2(B + 2); b2 = ld2(B + 4);
-                    b3 = ld2(B + 6); b4 = ld2(B + 8); b5 = ld2(B + 10);
-                }
+                // after the barrier that ends the pair work), else from rin, through one
+                // generic pointer so that both cases are the same loads with no branch.  The
+                // batched form keeps the global read (its instantiation measured 25% slower
+                // reading partners from LDS, while the plain one gains 4% on a poured bed and
+                // nothing elsewhere).
+                const double *B = (!BATCHED && pb - s0 < nb)
+                                      ? static_cast<const double *>(lrec + (pb - s0) * kRow)
+                                      : reinterpret_cast<const double *>(rin + pb);
+                double2 b0 = ld2(B), b1 = ld2(B + 2), b2 = ld2(B + 4), b3 = ld2(B + 6),
+                        b4 = ld2(B + 8), b5 = ld2(B + 10);
                 double2 s0v = ld2(S), s1 = ld2(S + 2), s2 = ld2(S + 4), s3 = ld2(S + 6),
                         s4 = ld2(S + 8), s5 = ld2(S + 10);
                 const d3 xs = d3{s0v.x, s0v.y, s1.x}, vs = d3{s2.x, s2.y, s3.x},
