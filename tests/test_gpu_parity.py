@@ -315,14 +315,17 @@ def test_fused_and_split_paths_identical(dem, gpu_ok, mode):
         w.pdis = w.pdis * 100.0   # fast enough for displacement-triggered rebuilds
     res = []
     # fused (one-pass contact tests), split, auto (timed probes switch paths mid-run), fused
-    # with every particle on the chunked path, and fused and split with one contiguous range
-    # of blocks per XCD
-    for fused, fast, chunk in ((1, 64, 8), (0, 64, 8), (2, 64, 8), (1, 0, 8), (1, 64, 0),
-                               (0, 64, 0)):
+    # with every particle on the chunked path, fused and split with one contiguous range of
+    # blocks per XCD, and fused with blocks of 64 and 256 particles (a partner inside the
+    # block is read from the block's LDS rows in the plain form, so the block edges move)
+    for fused, fast, chunk, pblk in ((1, 64, 8, 128), (0, 64, 8, 128), (2, 64, 8, 128),
+                                     (1, 0, 8, 128), (1, 64, 0, 128), (0, 64, 0, 128),
+                                     (1, 64, 8, 64), (1, 64, 8, 256)):
         c = dem.Context(w.instances)
         c.set_option(dem.OPT_FUSED_STEP, fused)
         c.set_option(dem.OPT_FUSED_FAST, fast)
         c.set_option(dem.OPT_XCD_CHUNK, chunk)
+        c.set_option(dem.OPT_PARTICLE_BLOCK, pblk)
         c.load(w)
         c.set_option(dem.OPT_RECORD_FORCES, 1)
         if mode == "trigger":
