@@ -1651,8 +1651,43 @@ __global__ void k_publish(DevCtl *ctl, unsigned char *mirror, const unsigned *ga
 // u32[ninst] totals) must be zero on entry.
 __global__ __launch_bounds__(kBlock) void k_inst_reduce(const PRec *rec, const unsigned *contacti,
                                                         unsigned n, unsigned long long *acc_z,
-                                                        unsigned *acc_c)
+                                                        unsigned *acc_c, unsigned single)
 {
+    if (single) {
+        // one instance: each lane folds its share of the grid-stride loop, then the block
+        // reduces through cross-lane reads and LDS and adds one atomic pair (atomics on one
+        // address serialize, so one pair per wave per iteration took ~180 us at 500K)
+        __shared__ unsigned long long rz[kBlock / 64];
+        __shared__ unsigned rc[kBlock / 64];
+        unsigned long long zm = 0ull;
+        unsigned cs = 0;
+        for (unsigned s = blockIdx.x * kBlock + threadIdx.x; s < n; s += gridDim.x * kBlock) {
+            const double z = rec[s].pz;
+            const unsigned long long zb = (unsigned long long)__double_as_longlong(z > 0.0 ? z : 0.0);
+            zm = zb > zm ? zb : zm;
+            cs += contacti[s];
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const unsigned long long oz = __shfl_xor(zm, d, 64);
+            zm = oz > zm ? oz : zm;
+            cs += __shfl_xor(cs, d, 64);
+        }
+        if (__lane_id() == 0) {
+            rz[threadIdx.x >> 6] = zm;
+            rc[threadIdx.x >> 6] = cs;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (unsigned w = 1; w < kBlock / 64; w++) {
+                zm = rz[w] > zm ? rz[w] : zm;
+                cs += rc[w];
+            }
+            atomicMax(&acc_z[0], zm);
+            atomicAdd(&acc_c[0], cs);
+        }
+        return;
+    }
     for (unsigned base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         unsigned s = base + threadIdx.x;
         bool live = s < n;

@@ -1938,7 +1938,8 @@ static int reduce_instances(dem_ctx *c)
     DEM_HIP(hipMemsetAsync(dt, 0, tail, c->stream));
     unsigned g = std::max(1u, std::min(cdiv(c->n, kBlock), (unsigned)c->cus * 4u));
     hipLaunchKernelGGL(k_inst_reduce, dim3(g), dim3(kBlock), 0, c->stream,
-                       (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti, c->n, dz, dp);
+                       (const PRec *)c->rec[c->rc], (const unsigned *)c->contacti, c->n, dz, dp,
+                       c->ninst == 1 ? 1u : 0u);
     DEM_HIP(hipGetLastError());
     DEM_HIP(hipMemcpyAsync(c->mirror_h + sizeof(DevCtl), dt, tail, hipMemcpyDeviceToHost, c->stream));
     DEM_HIP(hipStreamSynchronize(c->stream));
